@@ -1,0 +1,187 @@
+"""Ray-cast benchmark: incoherent closest-hit MRays/s on one MI355X.
+
+Workload: 1,000,000-triangle soup (centroids U[0,1]^3, edge exp(-5 + 0.5 z), random rotation),
+2^23 incoherent rays (origins U[0,1]^3, directions uniform on the sphere) with Rendu's default
+interval (1e-4, 1e8). One step = one closest-hit pass (rdu_raycaster_intersect, near-first
+order) over the 2^23 incoherent rays, inputs and outputs resident in HBM. Coherent primaries
+(2^23, pinhole 4096x2048) and any-hit rates are reported alongside.
+
+Prints one JSON line. The CPU baseline is the oracle's restatement of Rendu's Raycaster
+(same tree, same traversal, forParallel's max(hw-1, 1) thread policy) on a bounded sample of the
+same incoherent rays, on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_TRIS = 1_000_000
+N_HALF = 1 << 23
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 21, help="rays in the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--reference-order", action="store_true", help="time Rendu's exact visiting order")
+    return ap.parse_args()
+
+
+def load_traffic(tag: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if one exists."""
+    path = os.path.join(ROOT, "profiles", f"{tag}_pmc.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box in this setup has a single MI355X; "
+              "the multi-GPU sharding design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from rendu_amd import workloads
+    from rendu_amd.raycaster import Raycaster
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    torch.cuda.init()
+    t0 = time.perf_counter()
+    mesh = workloads.triangle_soup(N_TRIS, seed=1)
+    rays = workloads.soup_rays(N_HALF, N_HALF, seed=2)
+    gen_s = time.perf_counter() - t0
+    rc = Raycaster()
+    rc.addMesh(mesh)
+    t0 = time.perf_counter()
+    rc.updateHierarchy()
+    build_s = time.perf_counter() - t0
+    info = rc.info()
+
+    coherent = np.ascontiguousarray(rays[:N_HALF])
+    incoherent = np.ascontiguousarray(rays[N_HALF:])
+    d_inc = torch.from_numpy(incoherent.view(np.uint8)).cuda()
+    d_coh = torch.from_numpy(coherent.view(np.uint8)).cuda()
+    d_hits = torch.empty(N_HALF * 24, dtype=torch.uint8, device="cuda")
+    d_occ = torch.empty(N_HALF, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    ref = bool(args.reference_order)
+
+    def closest(d):
+        rc.intersect_device(d.data_ptr(), N_HALF, d_hits.data_ptr(), sh, reference_order=ref)
+
+    def anyhit(d):
+        rc.intersect_device(d.data_ptr(), N_HALF, d_occ.data_ptr(), sh, any_hit=True, reference_order=ref)
+
+    def timed(fn, d, steps, warmup):
+        for _ in range(warmup):
+            fn(d)
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        w0 = time.perf_counter()
+        for a, b in ev:
+            a.record(stream)
+            fn(d)
+            b.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - w0
+        kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+        return wall, kernel_ms
+
+    wall, k_ms = timed(closest, d_inc, args.steps, args.warmup)
+    value = args.steps * N_HALF / wall / 1e6
+    hits = d_hits.cpu().numpy().view(np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("hit", "<u4"),
+                                                ("mesh_id", "<u4"), ("local_id", "<u4")]))
+    hit_rate = float((hits["hit"] == 1).mean())
+    steps2 = max(3, args.steps // 2)
+    wall_c, kc_ms = timed(closest, d_coh, steps2, 1)
+    wall_ai, kai_ms = timed(anyhit, d_inc, steps2, 1)
+    wall_ac, kac_ms = timed(anyhit, d_coh, steps2, 1)
+
+    # Algorithmic bytes per launch: each ray read once (32 B), each hit written once (24 B), and
+    # the scene (internal nodes 64 B, triangles 48 B, roots 32 B) read once per launch.
+    scene_bytes = info["device_bytes"]
+    alg_bytes = N_HALF * (32 + 24) + scene_bytes
+    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": "incoherent MRays/s at 1 GPU (closest hit, 1M-triangle soup)",
+        "value": round(value, 2),
+        "unit": "MRays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded soup and rays, generated in-process)",
+        "config": {
+            "workload": "ray-cast microbench: 1M-triangle soup, 2^23 incoherent closest-hit rays per step",
+            "triangles": N_TRIS,
+            "rays_per_step": N_HALF,
+            "bvh_nodes": info["nodes"],
+            "bvh_depth": info["max_depth"],
+            "visiting_order": "reference" if ref else "near-first",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": load_traffic("trace_closest_soup1m"),
+        },
+        "extra": {
+            "kernel_ms_incoherent_closest": round(k_ms, 4),
+            "hit_rate_incoherent": round(hit_rate, 4),
+            "coherent_closest_mrays": round(steps2 * N_HALF / wall_c / 1e6, 2),
+            "incoherent_any_mrays": round(steps2 * N_HALF / wall_ai / 1e6, 2),
+            "coherent_any_mrays": round(steps2 * N_HALF / wall_ac / 1e6, 2),
+            "bvh_build_s": round(build_s, 3),
+            "input_generation_s": round(gen_s, 3),
+        },
+    }
+
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        ncpu = os.cpu_count() or 1
+        threads = max(1, min(ncpu - 1, 15))
+        ora = orc.OracleRaycaster()
+        ora.add_mesh(mesh.positions, mesh.indices)
+        ora.update()
+        sample = incoherent[: args.cpu_sample]
+        c0 = time.perf_counter()
+        ora.intersect(sample, "closest", threads=threads)
+        cpu_s = time.perf_counter() - c0
+        out["cpu_baseline"] = {
+            "value": round(sample.shape[0] / cpu_s / 1e6, 4),
+            "unit": "MRays/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"first {sample.shape[0]} of the same incoherent rays, closest hit, {cpu_s:.1f} s",
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

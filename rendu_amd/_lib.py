@@ -1,0 +1,107 @@
+"""ctypes binding of librdu.so, the C ABI declared in include/rdu.h.
+
+The library is loaded from this package directory. There is no fallback: if librdu.so is absent
+the import fails with the command that builds it, and queries without a HIP device fail with the
+library's own RDU_ERR_HIP message.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_float, c_int, c_size_t, c_uint8, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librdu.so")
+
+RDU_OK = 0
+RDU_ERR_INVALID = -1
+RDU_ERR_STATE = -2
+RDU_ERR_HIP = -3
+RDU_ERR_ALLOC = -4
+RDU_ERR_UNSUPPORTED = -5
+
+RDU_DEVICE_POINTERS = 1
+RDU_REFERENCE_ORDER = 2
+
+RAY_DTYPE = np.dtype([("ox", "<f4"), ("oy", "<f4"), ("oz", "<f4"), ("tmin", "<f4"),
+                      ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"),
+                      ("hit", "<u4"), ("mesh_id", "<u4"), ("local_id", "<u4")])
+
+
+class RduError(RuntimeError):
+    """A non-zero status returned by librdu, with rdu_last_error()'s message."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"rdu error {code}: {message}")
+        self.code = code
+        self.message = message
+
+
+class _Info(ctypes.Structure):
+    _fields_ = [("meshes", c_uint64), ("triangles", c_uint64), ("vertices", c_uint64), ("nodes", c_uint64),
+                ("internal_nodes", c_uint64), ("max_depth", c_uint64), ("device_bytes", c_uint64)]
+
+
+def _load() -> ctypes.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+                          "(or `make -C rendu_amd`) first")
+    lib = ctypes.CDLL(LIB_PATH)
+    fp = POINTER(c_float)
+    sig = {
+        "rdu_raycaster_create": (c_int, [POINTER(c_void_p)]),
+        "rdu_raycaster_destroy": (None, [c_void_p]),
+        "rdu_raycaster_add_mesh": (c_int, [c_void_p, fp, c_size_t, POINTER(c_uint32), c_size_t, fp]),
+        "rdu_raycaster_update_hierarchy": (c_int, [c_void_p]),
+        "rdu_raycaster_intersect": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_uint32, c_void_p]),
+        "rdu_raycaster_intersect_any": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_uint32, c_void_p]),
+        "rdu_raycaster_intersect1": (c_int, [c_void_p, fp, fp, c_float, c_float, c_void_p]),
+        "rdu_raycaster_intersect_any1": (c_int, [c_void_p, fp, fp, c_float, c_float, POINTER(c_int)]),
+        "rdu_raycaster_visible": (c_int, [c_void_p, fp, fp, POINTER(c_int)]),
+        "rdu_raycaster_info": (c_int, [c_void_p, POINTER(_Info)]),
+        "rdu_raycaster_export_nodes": (c_int, [c_void_p, fp, POINTER(c_uint64), POINTER(c_uint64), POINTER(c_uint8)]),
+        "rdu_raycaster_export_triangles": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint32)]),
+        "rdu_raycaster_trace_stats": (c_int, [c_void_p, c_void_p, c_size_t, c_int, c_uint32,
+                                              POINTER(c_uint64), POINTER(c_uint64)]),
+        "rdu_last_error": (c_char_p, []),
+        "rdu_version": (c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(status: int) -> None:
+    if status != RDU_OK:
+        raise RduError(status, lib.rdu_last_error().decode("utf-8", "replace"))
+
+
+def fptr(a: np.ndarray):
+    return a.ctypes.data_as(POINTER(c_float))
+
+
+def u32ptr(a: np.ndarray):
+    return a.ctypes.data_as(POINTER(c_uint32))
+
+
+def u64ptr(a: np.ndarray):
+    return a.ctypes.data_as(POINTER(c_uint64))
+
+
+def u8ptr(a: np.ndarray):
+    return a.ctypes.data_as(POINTER(c_uint8))
+
+
+def vptr(a: np.ndarray):
+    return c_void_p(a.ctypes.data)
+
+
+Info = _Info

@@ -1,0 +1,207 @@
+// Rendu's BVH build on the host and its flattening into the device layout (see bvh.hpp).
+#include "bvh.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <limits>
+
+namespace rdu {
+
+namespace {
+
+HBox triBox(f3 a, f3 b, f3 c) {
+	// Bounds.cpp:12-15
+	return {gmin(gmin(a, b), c), gmax(gmax(a, b), c)};
+}
+
+void mergeInto(HBox & dst, const HBox & o) {
+	// Bounds.cpp:17-20
+	dst.mn = gmin(dst.mn, o.mn);
+	dst.mx = gmax(dst.mx, o.mx);
+}
+
+int32_t leafCode(uint64_t first, uint64_t count) { return ~int32_t((first << 1) | (count - 1)); }
+
+uint32_t asBits(float f) {
+	uint32_t u;
+	std::memcpy(&u, &f, 4);
+	return u;
+}
+float fromBits(uint32_t u) {
+	float f;
+	std::memcpy(&f, &u, 4);
+	return f;
+}
+
+} // namespace
+
+void HostScene::addMesh(const float * xyz, size_t nv, const uint32_t * idx, size_t ni, const float * model) {
+	const uint32_t indexOffset = uint32_t(vertices.size());
+	const size_t start = vertices.size();
+	vertices.reserve(start + nv);
+	for(size_t i = 0; i < nv; ++i)
+		vertices.push_back(mk(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]));
+	bool identity = true;
+	if(model)
+		for(int c = 0; c < 4; ++c)
+			for(int r = 0; r < 4; ++r)
+				identity = identity && (model[4 * c + r] == (c == r ? 1.0f : 0.0f));
+	if(model && !identity) {
+		// vec3(model * vec4(v, 1)), glm mat4*vec4 order (type_mat4x4.inl:558-569):
+		// (m[0]*x + m[1]*y) + (m[2]*z + m[3]*1) per component.
+		for(size_t vid = start; vid < vertices.size(); ++vid) {
+			const f3 p = vertices[vid];
+			float o[3];
+			for(int r = 0; r < 3; ++r) {
+				const float a0 = model[r] * p.x + model[4 + r] * p.y;
+				const float a1 = model[8 + r] * p.z + model[12 + r] * 1.0f;
+				o[r] = a0 + a1;
+			}
+			vertices[vid] = mk(o[0], o[1], o[2]);
+		}
+	}
+	const size_t startTri = tris.size();
+	const size_t count = ni / 3;
+	tris.reserve(startTri + count);
+	for(size_t t = 0; t < count; ++t) {
+		HTri tr;
+		tr.v0 = indexOffset + idx[3 * t + 0];
+		tr.v1 = indexOffset + idx[3 * t + 1];
+		tr.v2 = indexOffset + idx[3 * t + 2];
+		tr.localId = uint32_t(3 * t);
+		tr.meshId = meshCount;
+		tr.box = triBox(vertices[tr.v0], vertices[tr.v1], vertices[tr.v2]);
+		tr.c = 0.5f * (tr.box.mn + tr.box.mx); // Bounds.cpp:61-63
+		tris.push_back(tr);
+	}
+	HNode root;
+	root.left = startTri;
+	root.right = count;
+	nodes.push_back(root);
+	++meshCount;
+}
+
+void HostScene::build() {
+	struct Work {
+		uint64_t id, begin, count;
+	};
+	std::vector<Work> stack;
+	stack.reserve(64 + meshCount);
+	for(uint64_t r = 0; r < meshCount; ++r)
+		stack.push_back({r, nodes[r].left, nodes[r].right});
+	nodes.reserve(nodes.size() + 2 * tris.size());
+
+	while(!stack.empty()) {
+		const Work cur = stack.back();
+		stack.pop_back();
+		const uint64_t begin = cur.begin, count = cur.count;
+		HBox global = tris[begin].box;
+		for(uint64_t t = 1; t < count; ++t)
+			mergeInto(global, tris[begin + t].box);
+		nodes[cur.id].box = global;
+		if(count < 3) {
+			nodes[cur.id].leaf = true;
+			nodes[cur.id].left = begin;
+			nodes[cur.id].right = count;
+			continue;
+		}
+		nodes[cur.id].leaf = false;
+		uint64_t split = 0;
+		const f3 s = global.mx - global.mn;
+		const int axis = (s.x >= s.y && s.x >= s.z) ? 0 : (s.y >= s.z ? 1 : 2);
+		HTri * first = tris.data() + begin;
+		HTri * last = first + count;
+		if(count >= 5) {
+			// Sequential float sum in triangle order, then one division (Raycaster.cpp:102-106).
+			float mean = 0.0f;
+			for(uint64_t t = 0; t < count; ++t)
+				mean += comp(first[t].c, axis);
+			mean /= float(count);
+			HTri * mid = std::partition(first, last, [mean, axis](const HTri & a) { return comp(a.c, axis) < mean; });
+			split = uint64_t(mid - first);
+		}
+		if(split == 0 || split == count || count < 5) {
+			split = count / 2;
+			std::nth_element(first, first + split, last,
+				[axis](const HTri & a, const HTri & b) { return comp(a.c, axis) < comp(b.c, axis); });
+		}
+		nodes.emplace_back();
+		const uint64_t leftPos = nodes.size() - 1;
+		nodes[cur.id].left = leftPos;
+		stack.push_back({leftPos, begin, split});
+		nodes.emplace_back();
+		const uint64_t rightPos = nodes.size() - 1;
+		nodes[cur.id].right = rightPos;
+		stack.push_back({rightPos, begin + split, count - split});
+	}
+	built = true;
+}
+
+void HostScene::flatten(DeviceLayout & out) const {
+	// Depth-first preorder numbering of internal nodes (left subtree first) so that a parent and
+	// its near subtree share cache lines as traversal descends.
+	std::vector<int32_t> gpuIndex(nodes.size(), -1);
+	std::vector<uint64_t> order;
+	order.reserve(nodes.size() / 2 + 1);
+	uint32_t maxDepth = 0;
+	std::vector<std::pair<uint64_t, uint32_t>> st;
+	for(uint64_t r = 0; r < meshCount; ++r) {
+		st.push_back({r, 1});
+		while(!st.empty()) {
+			const auto cur = st.back();
+			st.pop_back();
+			maxDepth = std::max(maxDepth, cur.second);
+			const HNode & n = nodes[cur.first];
+			if(n.leaf)
+				continue;
+			gpuIndex[cur.first] = int32_t(order.size());
+			order.push_back(cur.first);
+			st.push_back({n.right, cur.second + 1});
+			st.push_back({n.left, cur.second + 1});
+		}
+	}
+	auto code = [&](uint64_t id) -> int32_t {
+		const HNode & n = nodes[id];
+		return n.leaf ? leafCode(n.left, n.right) : gpuIndex[id];
+	};
+	out.internalCount = uint32_t(order.size());
+	out.maxDepth = maxDepth;
+	out.nodes.assign(order.size() * 16, 0.0f);
+	for(size_t i = 0; i < order.size(); ++i) {
+		const HNode & n = nodes[order[i]];
+		const HBox & L = nodes[n.left].box;
+		const HBox & R = nodes[n.right].box;
+		float * o = out.nodes.data() + 16 * i;
+		const float v[12] = {L.mn.x, L.mn.y, L.mn.z, L.mx.x, L.mx.y, L.mx.z, R.mn.x, R.mn.y, R.mn.z, R.mx.x, R.mx.y, R.mx.z};
+		std::memcpy(o, v, sizeof(v));
+		o[12] = fromBits(uint32_t(code(n.left)));
+		o[13] = fromBits(uint32_t(code(n.right)));
+	}
+	out.tris.assign(tris.size() * 12, 0.0f);
+	for(size_t i = 0; i < tris.size(); ++i) {
+		const HTri & t = tris[i];
+		const f3 v0 = vertices[t.v0];
+		const f3 e1 = vertices[t.v1] - v0; // Raycaster.cpp:235
+		const f3 e2 = vertices[t.v2] - v0; // Raycaster.cpp:236
+		float * o = out.tris.data() + 12 * i;
+		const float v[9] = {v0.x, v0.y, v0.z, e1.x, e1.y, e1.z, e2.x, e2.y, e2.z};
+		std::memcpy(o, v, sizeof(v));
+		o[9] = fromBits(t.meshId);
+		o[10] = fromBits(t.localId);
+	}
+	out.roots.assign(size_t(meshCount) * 8, 0.0f);
+	for(uint32_t r = 0; r < meshCount; ++r) {
+		const HNode & n = nodes[r];
+		float * o = out.roots.data() + 8 * r;
+		o[0] = n.box.mn.x;
+		o[1] = n.box.mn.y;
+		o[2] = n.box.mn.z;
+		o[3] = fromBits(uint32_t(code(r)));
+		o[4] = n.box.mx.x;
+		o[5] = n.box.mx.y;
+		o[6] = n.box.mx.z;
+	}
+	(void)asBits;
+}
+
+} // namespace rdu

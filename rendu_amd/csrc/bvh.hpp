@@ -1,0 +1,65 @@
+// Host-side scene store and Rendu's BVH build, plus the flattening into the device layout.
+//
+// The tree is Rendu's own (src/engine/raycaster/Raycaster.cpp:54-142): one root per mesh, mean
+// centroid split on the largest axis with an nth_element median fallback, leaves of 1-2
+// triangles, nodes appended in the order of a LIFO work stack. The device layout keeps every
+// box and every leaf of that tree; only the storage changes (see DeviceLayout).
+#pragma once
+#include "rdu_vec.hpp"
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+namespace rdu {
+
+struct HBox {
+	f3 mn, mx;
+};
+
+// One triangle as the build moves it (Raycaster.hpp:117-124 TriangleInfos, plus its centroid,
+// which Rendu recomputes as 0.5*(min+max) at every comparison: storing it changes no value).
+struct HTri {
+	HBox box;
+	f3 c;
+	uint32_t v0, v1, v2;
+	uint32_t localId;
+	uint32_t meshId;
+};
+
+// Raycaster.hpp:127-132 Node.
+struct HNode {
+	HBox box;
+	uint64_t left = 0, right = 0;
+	bool leaf = true;
+};
+
+// Device layout, all float4-aligned arrays:
+//  nodes: 4 float4 per internal node (depth-first preorder, left subtree first):
+//         {lmin.xyz, lmax.x} {lmax.yz, rmin.xy} {rmin.z, rmax.xyz} {lcode, rcode, 0, 0 as int bits}
+//         child code >= 0: internal node index; < 0: leaf ~(first << 1 | (count - 1)).
+//  tris:  3 float4 per triangle in Rendu's permuted order:
+//         {v0.xyz, e1.x} {e1.yz, e2.xy} {e2.z, meshId, localId, 0 as int bits}, e1 = v1 - v0 and
+//         e2 = v2 - v0 computed in float exactly as the Moller-Trumbore test does.
+//  roots: 2 float4 per mesh {min.xyz, code} {max.xyz, 0}.
+struct DeviceLayout {
+	std::vector<float> nodes, tris, roots;
+	uint32_t internalCount = 0;
+	uint32_t maxDepth = 0;
+};
+
+class HostScene {
+public:
+	std::vector<f3> vertices;
+	std::vector<HTri> tris;
+	std::vector<HNode> nodes;
+	uint32_t meshCount = 0;
+	bool built = false;
+
+	// Raycaster.cpp:14-52. model: glm column-major 4x4 or nullptr.
+	void addMesh(const float * xyz, size_t nv, const uint32_t * idx, size_t ni, const float * model);
+	// Raycaster.cpp:54-142.
+	void build();
+	void flatten(DeviceLayout & out) const;
+};
+
+} // namespace rdu

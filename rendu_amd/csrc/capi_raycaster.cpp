@@ -1,0 +1,289 @@
+// C ABI of the raycaster (include/rdu.h): the drop-in boundary for Rendu's Raycaster
+// (src/engine/raycaster/Raycaster.hpp:10-157). Host-side bookkeeping only; all queries run on
+// the GPU through trace.hip. There is no CPU query path: without a HIP device the queries fail
+// with RDU_ERR_HIP.
+#include "capi_common.hpp"
+#include "bvh.hpp"
+#include "trace.hpp"
+#include "../../include/rdu.h"
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+struct rdu_raycaster {
+	rdu::HostScene scene;
+	rdu::DeviceLayout layout;
+	rdu::DevBVH dev;
+	rdu::DevBuffer nodes, tris, roots;
+	bool uploaded = false;
+	int device = -1;
+	// Staging for host-pointer queries and global stack spill; guarded by `lock`.
+	mutable std::mutex lock;
+	mutable rdu::DevBuffer stageIn, stageOut, gstack, stats;
+};
+
+using namespace rdu;
+
+namespace {
+
+int checkBuilt(const rdu_raycaster * rc) {
+	if(!rc)
+		return fail(RDU_ERR_INVALID, "null raycaster");
+	if(!rc->scene.built)
+		return fail(RDU_ERR_STATE, "query before updateHierarchy");
+	if(!rc->uploaded)
+		return fail(RDU_ERR_HIP, "the hierarchy is not resident on a GPU (no HIP device or upload failed)");
+	return RDU_OK;
+}
+
+int ensureGlobalStack(const rdu_raycaster * rc, size_t n) {
+	const size_t ints = globalStackInts(rc->dev, n);
+	if(ints == 0)
+		return RDU_OK;
+	return rc->gstack.reserve(ints * sizeof(int));
+}
+
+template<bool ANY>
+int query(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, void * out, uint32_t flags, void * stream,
+	uint64_t * statsOut) {
+	int st = checkBuilt(rc);
+	if(st)
+		return st;
+	if(n == 0)
+		return RDU_OK;
+	if(!rays || !out)
+		return fail(RDU_ERR_INVALID, "null rays or output");
+	if(n > (size_t(1) << 31))
+		return fail(RDU_ERR_UNSUPPORTED, "batch larger than 2^31 rays");
+	const bool ref = (flags & RDU_REFERENCE_ORDER) != 0;
+	const size_t outBytes = ANY ? n : n * sizeof(rdu_hit);
+	std::lock_guard<std::mutex> guard(rc->lock);
+	if((st = setDevice(rc->device)))
+		return st;
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	if((st = ensureGlobalStack(rc, n)))
+		return st;
+	uint64_t * dstats = nullptr;
+	if(statsOut) {
+		if((st = rc->stats.reserve(2 * sizeof(uint64_t))))
+			return st;
+		dstats = static_cast<uint64_t *>(rc->stats.ptr);
+		if((st = hipCheck(hipMemsetAsync(dstats, 0, 2 * sizeof(uint64_t), s), "hipMemsetAsync")))
+			return st;
+	}
+	const float4 * dRays;
+	void * dOut;
+	const bool device = (flags & RDU_DEVICE_POINTERS) != 0;
+	if(device) {
+		dRays = reinterpret_cast<const float4 *>(rays);
+		dOut = out;
+	} else {
+		if((st = rc->stageIn.reserve(n * sizeof(rdu_ray))) || (st = rc->stageOut.reserve(outBytes)))
+			return st;
+		if((st = hipCheck(hipMemcpyAsync(rc->stageIn.ptr, rays, n * sizeof(rdu_ray), hipMemcpyHostToDevice, s),
+				"hipMemcpyAsync H2D")))
+			return st;
+		dRays = static_cast<const float4 *>(rc->stageIn.ptr);
+		dOut = rc->stageOut.ptr;
+	}
+	const int e = ANY ? launchAny(rc->dev, dRays, n, static_cast<uint8_t *>(dOut), ref, static_cast<int *>(rc->gstack.ptr),
+							  dstats, s)
+					  : launchClosest(rc->dev, dRays, n, dOut, ref, static_cast<int *>(rc->gstack.ptr), dstats, s);
+	if((st = hipCheck(hipError_t(e), "trace kernel launch")))
+		return st;
+	if(!device) {
+		if((st = hipCheck(hipMemcpyAsync(out, dOut, outBytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync D2H")))
+			return st;
+	}
+	if(statsOut) {
+		if((st = hipCheck(hipMemcpyAsync(statsOut, dstats, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s), "stats copy")))
+			return st;
+	}
+	if(!device || statsOut) {
+		if((st = hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize")))
+			return st;
+	}
+	return RDU_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int rdu_raycaster_create(rdu_raycaster ** out) {
+	clearError();
+	if(!out)
+		return fail(RDU_ERR_INVALID, "null output handle");
+	*out = new(std::nothrow) rdu_raycaster();
+	if(!*out)
+		return fail(RDU_ERR_ALLOC, "host allocation failed");
+	return RDU_OK;
+}
+
+void rdu_raycaster_destroy(rdu_raycaster * rc) {
+	if(!rc)
+		return;
+	if(rc->device >= 0)
+		setDevice(rc->device);
+	delete rc;
+}
+
+int rdu_raycaster_add_mesh(rdu_raycaster * rc, const float * xyz, size_t nv, const uint32_t * idx, size_t ni,
+	const float * model) {
+	clearError();
+	if(!rc)
+		return fail(RDU_ERR_INVALID, "null raycaster");
+	if(rc->scene.built)
+		return fail(RDU_ERR_STATE, "addMesh after updateHierarchy: mesh roots must precede all child nodes");
+	if(ni % 3 != 0)
+		return fail(RDU_ERR_INVALID, "index count is not a multiple of 3");
+	if((nv && !xyz) || (ni && !idx))
+		return fail(RDU_ERR_INVALID, "null vertex or index pointer");
+	for(size_t i = 0; i < ni; ++i)
+		if(idx[i] >= nv)
+			return fail(RDU_ERR_INVALID, "index out of range of the mesh vertices");
+	if(rc->scene.vertices.size() + nv >= (size_t(1) << 32) || rc->scene.tris.size() + ni / 3 >= (size_t(1) << 30))
+		return fail(RDU_ERR_UNSUPPORTED, "more than 2^30 triangles or 2^32 vertices");
+	rc->scene.addMesh(xyz, nv, idx, ni, model);
+	return RDU_OK;
+}
+
+int rdu_raycaster_update_hierarchy(rdu_raycaster * rc) {
+	clearError();
+	if(!rc)
+		return fail(RDU_ERR_INVALID, "null raycaster");
+	if(rc->scene.built)
+		return fail(RDU_ERR_STATE, "updateHierarchy called twice");
+	for(size_t m = 0; m < rc->scene.meshCount; ++m)
+		if(rc->scene.nodes[m].right == 0)
+			return fail(RDU_ERR_INVALID, "mesh with no triangle (Rendu reads past its triangle list here)");
+	rc->scene.build();
+	rc->scene.flatten(rc->layout);
+	int st = currentDevice(&rc->device);
+	if(st)
+		return st;
+	if((st = rc->nodes.upload(rc->layout.nodes.data(), rc->layout.nodes.size() * 4)) ||
+		(st = rc->tris.upload(rc->layout.tris.data(), rc->layout.tris.size() * 4)) ||
+		(st = rc->roots.upload(rc->layout.roots.data(), rc->layout.roots.size() * 4)))
+		return st;
+	rc->dev.nodes = static_cast<const float4 *>(rc->nodes.ptr);
+	rc->dev.tris = static_cast<const float4 *>(rc->tris.ptr);
+	rc->dev.roots = static_cast<const float4 *>(rc->roots.ptr);
+	rc->dev.nRoots = rc->scene.meshCount;
+	rc->dev.maxDepth = rc->layout.maxDepth;
+	rc->uploaded = true;
+	return RDU_OK;
+}
+
+int rdu_raycaster_intersect(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, rdu_hit * hits, uint32_t flags,
+	void * stream) {
+	clearError();
+	return query<false>(rc, rays, n, hits, flags, stream, nullptr);
+}
+
+int rdu_raycaster_intersect_any(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, uint8_t * occluded,
+	uint32_t flags, void * stream) {
+	clearError();
+	return query<true>(rc, rays, n, occluded, flags, stream, nullptr);
+}
+
+int rdu_raycaster_intersect1(const rdu_raycaster * rc, const float o[3], const float d[3], float mini, float maxi,
+	rdu_hit * out) {
+	clearError();
+	if(!o || !d || !out)
+		return fail(RDU_ERR_INVALID, "null argument");
+	const rdu_ray r = {o[0], o[1], o[2], mini, d[0], d[1], d[2], maxi};
+	return query<false>(rc, &r, 1, out, RDU_REFERENCE_ORDER, nullptr, nullptr);
+}
+
+int rdu_raycaster_intersect_any1(const rdu_raycaster * rc, const float o[3], const float d[3], float mini, float maxi,
+	int * occluded) {
+	clearError();
+	if(!o || !d || !occluded)
+		return fail(RDU_ERR_INVALID, "null argument");
+	const rdu_ray r = {o[0], o[1], o[2], mini, d[0], d[1], d[2], maxi};
+	uint8_t occ = 0;
+	const int st = query<true>(rc, &r, 1, &occ, RDU_REFERENCE_ORDER, nullptr, nullptr);
+	*occluded = occ;
+	return st;
+}
+
+int rdu_raycaster_visible(const rdu_raycaster * rc, const float p0[3], const float p1[3], int * visible) {
+	clearError();
+	if(!p0 || !p1 || !visible)
+		return fail(RDU_ERR_INVALID, "null argument");
+	// Raycaster.cpp:226-230
+	const f3 dir = mk(p1[0], p1[1], p1[2]) - mk(p0[0], p0[1], p0[2]);
+	const float maxi = length(dir);
+	const float d[3] = {dir.x, dir.y, dir.z};
+	int occ = 0;
+	const int st = rdu_raycaster_intersect_any1(rc, p0, d, 0.0001f, maxi, &occ);
+	*visible = occ ? 0 : 1;
+	return st;
+}
+
+int rdu_raycaster_info(const rdu_raycaster * rc, rdu_bvh_info * out) {
+	clearError();
+	if(!rc || !out)
+		return fail(RDU_ERR_INVALID, "null argument");
+	std::memset(out, 0, sizeof(*out));
+	out->meshes = rc->scene.meshCount;
+	out->triangles = rc->scene.tris.size();
+	out->vertices = rc->scene.vertices.size();
+	out->nodes = rc->scene.nodes.size();
+	out->internal_nodes = rc->layout.internalCount;
+	out->max_depth = rc->layout.maxDepth;
+	out->device_bytes = (rc->layout.nodes.size() + rc->layout.tris.size() + rc->layout.roots.size()) * 4;
+	return RDU_OK;
+}
+
+int rdu_raycaster_export_nodes(const rdu_raycaster * rc, float * boxes, uint64_t * left, uint64_t * right, uint8_t * leaf) {
+	clearError();
+	if(!rc || !boxes || !left || !right || !leaf)
+		return fail(RDU_ERR_INVALID, "null argument");
+	if(!rc->scene.built)
+		return fail(RDU_ERR_STATE, "export before updateHierarchy");
+	for(size_t i = 0; i < rc->scene.nodes.size(); ++i) {
+		const HNode & nd = rc->scene.nodes[i];
+		const float b[6] = {nd.box.mn.x, nd.box.mn.y, nd.box.mn.z, nd.box.mx.x, nd.box.mx.y, nd.box.mx.z};
+		std::memcpy(boxes + 6 * i, b, sizeof(b));
+		left[i] = nd.left;
+		right[i] = nd.right;
+		leaf[i] = nd.leaf ? 1 : 0;
+	}
+	return RDU_OK;
+}
+
+int rdu_raycaster_export_triangles(const rdu_raycaster * rc, uint32_t * mesh_id, uint32_t * local_id) {
+	clearError();
+	if(!rc || !mesh_id || !local_id)
+		return fail(RDU_ERR_INVALID, "null argument");
+	if(!rc->scene.built)
+		return fail(RDU_ERR_STATE, "export before updateHierarchy");
+	for(size_t i = 0; i < rc->scene.tris.size(); ++i) {
+		mesh_id[i] = rc->scene.tris[i].meshId;
+		local_id[i] = rc->scene.tris[i].localId;
+	}
+	return RDU_OK;
+}
+
+int rdu_raycaster_trace_stats(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, int any_hit, uint32_t flags,
+	uint64_t * node_visits, uint64_t * tri_tests) {
+	clearError();
+	if(!node_visits || !tri_tests)
+		return fail(RDU_ERR_INVALID, "null argument");
+	if(flags & RDU_DEVICE_POINTERS)
+		return fail(RDU_ERR_INVALID, "trace_stats takes host rays");
+	uint64_t s[2] = {0, 0};
+	std::vector<uint8_t> out(any_hit ? n : n * sizeof(rdu_hit));
+	const int st = any_hit ? query<true>(rc, rays, n, out.data(), flags, nullptr, s)
+						   : query<false>(rc, rays, n, out.data(), flags, nullptr, s);
+	*node_visits = s[0];
+	*tri_tests = s[1];
+	return st;
+}
+
+} // extern "C"

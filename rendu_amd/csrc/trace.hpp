@@ -1,0 +1,82 @@
+// Device-side ray/BVH traversal: shared primitives (usable from other kernels of the library)
+// and the launchers of the batch kernels.
+#pragma once
+#include "rdu_vec.hpp"
+#include <cstddef>
+#include <cstdint>
+
+namespace rdu {
+
+// Device copy of a flattened BVH (layout documented in bvh.hpp).
+struct DevBVH {
+	const float4 * nodes = nullptr;
+	const float4 * tris = nullptr;
+	const float4 * roots = nullptr;
+	uint32_t nRoots = 0;
+	uint32_t maxDepth = 0;
+};
+
+struct DevRay {
+	f3 pos, dir, inv;
+	float mini, maxi;
+};
+
+// Ray::Ray (Intersection.cpp:3-5): dir = normalize(d), invdir = 1 / normalize(d).
+RDU_HD DevRay makeRay(f3 o, f3 d, float mini, float maxi) {
+	DevRay r;
+	r.pos = o;
+	r.dir = normalize(d);
+	r.inv = rcp3(normalize(d));
+	r.mini = mini;
+	r.maxi = maxi;
+	return r;
+}
+
+// Intersection::box (Intersection.cpp:22-32): slab test with select-based min/max.
+RDU_HD bool slab(const DevRay & r, f3 bmin, f3 bmax, float mini, float maxi, float & entry) {
+	const f3 a = (bmin - r.pos) * r.inv;
+	const f3 b = (bmax - r.pos) * r.inv;
+	const f3 mn = gmin(a, b);
+	const f3 mx = gmax(a, b);
+	const float closest = gmax(mn.x, gmax(mn.y, mn.z));
+	const float furthest = gmin(mx.x, gmin(mx.y, mx.z));
+	entry = gmax(closest, mini);
+	return entry <= gmin(furthest, maxi);
+}
+
+// Moller-Trumbore (Raycaster.cpp:232-262) on a precomputed {v0, e1 = v1 - v0, e2 = v2 - v0}.
+RDU_HD bool mollerTrumbore(const DevRay & r, f3 v0, f3 e1, f3 e2, float mini, float maxi, float & t, float & u,
+	float & v) {
+	const f3 p = cross(r.dir, e2);
+	const float det = dot(e1, p);
+	if(__builtin_fabsf(det) < 1.19209290e-07f) // FLT_EPSILON
+		return false;
+	const float invDet = 1.0f / det;
+	const f3 q = r.pos - v0;
+	u = invDet * dot(q, p);
+	if(u < 0.0f || u > 1.0f)
+		return false;
+	const f3 rr = cross(q, e1);
+	v = invDet * dot(r.dir, rr);
+	if(v < 0.0f || (u + v) > 1.0f)
+		return false;
+	t = invDet * dot(e2, rr);
+	return t > mini && t < maxi;
+}
+
+enum TraceMode : int { kClosest = 0, kAny = 1 };
+
+// Launchers. rays: device rdu_ray array (float4 pairs); outputs are device pointers.
+// Returns hipError_t as int.
+int launchClosest(const DevBVH & bvh, const float4 * rays, size_t n, void * hits, bool referenceOrder,
+	int * gstack, uint64_t * stats, hipStream_t stream);
+int launchAny(const DevBVH & bvh, const float4 * rays, size_t n, uint8_t * occluded, bool referenceOrder,
+	int * gstack, uint64_t * stats, hipStream_t stream);
+
+// Number of stack entries the kernels keep in LDS per ray; deeper trees spill to `gstack`,
+// which then needs (maxDepth + 2 - kLdsStack) ints per ray slot of the launch.
+constexpr int kLdsStack = 32;
+int ldsStackDepth();
+size_t globalStackInts(const DevBVH & bvh, size_t n);
+
+} // namespace rdu

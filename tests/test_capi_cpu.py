@@ -1,0 +1,134 @@
+"""C ABI checks that need no GPU: symbols, argument/state errors, and the host-built tree.
+
+The product builds Rendu's tree on the host (update_hierarchy) before uploading it; without a HIP
+device the upload fails loudly (RDU_ERR_HIP) but the host tree is already there, so it can be
+compared node for node with the oracle's here.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import GPU, ROOT
+from oracle import oracle as orc
+from rendu_amd import _lib, workloads
+from rendu_amd.raycaster import Raycaster
+
+F32 = np.float32
+
+
+def _declared_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"\b(rdu_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_every_declared_symbol_is_exported():
+    names = _declared_functions()
+    assert "rdu_raycaster_intersect" in names and len(names) >= 14
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_version_and_error_string():
+    assert b"gfx950" in _lib.lib.rdu_version()
+    assert _lib.lib.rdu_last_error() is not None
+
+
+def _update(rc):
+    try:
+        rc.updateHierarchy()
+    except _lib.RduError as e:
+        assert not GPU and e.code == _lib.RDU_ERR_HIP, e
+        assert "GPU" in e.message or "device" in e.message
+
+
+def _scene_pair(meshes):
+    ours, ref = Raycaster(), orc.OracleRaycaster()
+    for mesh, model in meshes:
+        ours.addMesh(mesh, model)
+        ref.add_mesh(mesh.positions, mesh.indices, model)
+    _update(ours)
+    ref.update()
+    return ours, ref
+
+
+def _meshes_mixed():
+    out = [(workloads.triangle_soup(5000, seed=11), None)]
+    for k in range(3):
+        s = workloads.icosphere(2 + (k % 2), radius=0.2, center=(0.2 * k, 0.1, -0.3))
+        m = np.eye(4, dtype=F32)
+        m[:3, :3] = np.array([[0.9, 0.1, 0.0], [-0.1, 0.9, 0.2], [0.0, -0.2, 1.1]], F32)
+        m[:3, 3] = [0.5, 0.25 * k, 0.125]
+        out.append((s, m))
+    for q in workloads.box_quads((0, 0, 0), (1, 1, 1)):
+        out.append((q, None))
+    return out
+
+
+@pytest.mark.parametrize("case", ["soup", "mixed", "tiny"])
+def test_host_tree_equals_oracle_tree(case):
+    if case == "soup":
+        meshes = [(workloads.triangle_soup(20000, seed=3), None)]
+    elif case == "mixed":
+        meshes = _meshes_mixed()
+    else:
+        meshes = [(workloads.triangle_soup(n, seed=n), None) for n in (1, 2, 3, 4, 5, 9)]
+    ours, ref = _scene_pair(meshes)
+    a, b = ours.export_nodes(), ref.nodes()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    mesh, local = ours.export_triangles()
+    _, rlocal, rmesh, _ = ref.triangles()
+    assert np.array_equal(mesh, rmesh) and np.array_equal(local, rlocal.astype(np.uint32))
+    info = ours.info()
+    assert info["nodes"] == len(b[0]) and info["max_depth"] == ref.max_depth()
+    assert info["meshes"] == len(meshes)
+
+
+def test_state_and_argument_errors():
+    rc = Raycaster()
+    with pytest.raises(_lib.RduError) as e:
+        rc.intersect_batch(np.zeros(1, _lib.RAY_DTYPE))
+    assert e.value.code == _lib.RDU_ERR_STATE
+    bad = workloads.MeshData(np.zeros((3, 3), F32), np.array([0, 1, 3], np.uint32))
+    with pytest.raises(_lib.RduError) as e:
+        rc.addMesh(bad)
+    assert e.value.code == _lib.RDU_ERR_INVALID
+    with pytest.raises(_lib.RduError) as e:
+        rc.addMesh(workloads.MeshData(np.zeros((3, 3), F32), np.array([0, 1], np.uint32)))
+    assert e.value.code == _lib.RDU_ERR_INVALID
+    rc.addMesh(workloads.triangle_soup(10, seed=1))
+    _update(rc)
+    with pytest.raises(_lib.RduError) as e:
+        rc.addMesh(workloads.triangle_soup(10, seed=2))
+    assert e.value.code == _lib.RDU_ERR_STATE
+    with pytest.raises(_lib.RduError) as e:
+        rc.updateHierarchy()
+    assert e.value.code == _lib.RDU_ERR_STATE
+
+
+def test_zero_triangle_mesh_rejected():
+    rc = Raycaster()
+    rc.addMesh(workloads.MeshData(np.zeros((3, 3), F32), np.zeros(0, np.uint32)))
+    with pytest.raises(_lib.RduError) as e:
+        rc.updateHierarchy()
+    assert e.value.code == _lib.RDU_ERR_INVALID
+
+
+@pytest.mark.skipif(GPU, reason="checks the no-device path")
+def test_queries_fail_loudly_without_a_gpu():
+    rc = Raycaster()
+    rc.addMesh(workloads.triangle_soup(100, seed=1))
+    _update(rc)
+    with pytest.raises(_lib.RduError) as e:
+        rc.intersect_batch(workloads.soup_rays(0, 16))
+    assert e.value.code == _lib.RDU_ERR_HIP
+    with pytest.raises(_lib.RduError):
+        rc.intersects([0.5, 0.5, -1], [0, 0, 1])
