@@ -50,6 +50,8 @@ typedef struct {
 #define RDU_DEVICE_POINTERS 1u /* rays/outputs are device memory: asynchronous on `stream` */
 #define RDU_REFERENCE_ORDER 2u /* visit nodes in Rendu's exact depth-first order
                                   (Raycaster.cpp:144-185): also equal hits on exact ties */
+#define RDU_INCOHERENT 4u      /* hint: rays are incoherent; the library may process them in
+                                  another order (results still land at their input index) */
 
 typedef struct rdu_raycaster rdu_raycaster;
 
@@ -98,7 +100,10 @@ int rdu_raycaster_visible(const rdu_raycaster* rc, const float p0[3], const floa
 typedef struct {
 	uint64_t meshes, triangles, vertices, nodes; /* nodes = Rendu's _hierarchy.size() */
 	uint64_t internal_nodes, max_depth;          /* depth counts nodes on the longest root-leaf path */
-	uint64_t device_bytes;                       /* node + triangle + root arrays on the device */
+	uint64_t device_bytes;                       /* every array resident on the device */
+	uint64_t wide_nodes, wide_depth;             /* 4-wide collapse used by near-first queries */
+	uint64_t binary_bytes;                       /* bytes a reference-order query can touch */
+	uint64_t wide_bytes;                         /* bytes a near-first query can touch */
 } rdu_bvh_info;
 int rdu_raycaster_info(const rdu_raycaster* rc, rdu_bvh_info* out);
 

@@ -24,6 +24,7 @@ RDU_ERR_UNSUPPORTED = -5
 
 RDU_DEVICE_POINTERS = 1
 RDU_REFERENCE_ORDER = 2
+RDU_INCOHERENT = 4
 
 RAY_DTYPE = np.dtype([("ox", "<f4"), ("oy", "<f4"), ("oz", "<f4"), ("tmin", "<f4"),
                       ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("tmax", "<f4")])
@@ -42,7 +43,9 @@ class RduError(RuntimeError):
 
 class _Info(ctypes.Structure):
     _fields_ = [("meshes", c_uint64), ("triangles", c_uint64), ("vertices", c_uint64), ("nodes", c_uint64),
-                ("internal_nodes", c_uint64), ("max_depth", c_uint64), ("device_bytes", c_uint64)]
+                ("internal_nodes", c_uint64), ("max_depth", c_uint64), ("device_bytes", c_uint64),
+                ("wide_nodes", c_uint64), ("wide_depth", c_uint64), ("binary_bytes", c_uint64),
+                ("wide_bytes", c_uint64)]
 
 
 def _load() -> ctypes.CDLL:

@@ -189,6 +189,75 @@ void HostScene::flatten(DeviceLayout & out) const {
 		o[9] = fromBits(t.meshId);
 		o[10] = fromBits(t.localId);
 	}
+	// 4-wide collapse, depth-first preorder numbering.
+	out.wide.clear();
+	out.wideCount = 0;
+	out.wideDepth = 0;
+	auto area = [&](uint64_t id) {
+		const f3 s = nodes[id].box.mx - nodes[id].box.mn;
+		return s.x * s.y + s.y * s.z + s.z * s.x;
+	};
+	std::vector<int32_t> wideRoot(meshCount, 0);
+	struct WItem {
+		uint64_t id; // internal node of Rendu's tree heading this 4-wide node
+		uint32_t slot; // wide index assigned to it
+		uint32_t depth;
+	};
+	std::vector<WItem> wst;
+	for(uint32_t r = 0; r < meshCount; ++r) {
+		if(nodes[r].leaf) {
+			wideRoot[r] = leafCode(nodes[r].left, nodes[r].right);
+			continue;
+		}
+		wideRoot[r] = int32_t(out.wideCount++);
+		out.wide.resize(size_t(out.wideCount) * 32, 0.0f);
+		wst.push_back({r, uint32_t(wideRoot[r]), 1});
+		while(!wst.empty()) {
+			const WItem it = wst.back();
+			wst.pop_back();
+			out.wideDepth = std::max(out.wideDepth, it.depth);
+			std::vector<uint64_t> kids = {nodes[it.id].left, nodes[it.id].right};
+			while(kids.size() < 4) {
+				int best = -1;
+				float bestArea = -1.0f;
+				for(size_t k = 0; k < kids.size(); ++k)
+					if(!nodes[kids[k]].leaf && area(kids[k]) > bestArea) {
+						best = int(k);
+						bestArea = area(kids[k]);
+					}
+				if(best < 0)
+					break;
+				const uint64_t e = kids[size_t(best)];
+				kids[size_t(best)] = nodes[e].left;
+				kids.insert(kids.begin() + best + 1, nodes[e].right);
+			}
+			int32_t codes[4] = {kWideEmpty, kWideEmpty, kWideEmpty, kWideEmpty};
+			float * o = out.wide.data() + size_t(it.slot) * 32;
+			std::vector<WItem> children;
+			for(size_t k = 0; k < kids.size(); ++k) {
+				const HNode & c = nodes[kids[k]];
+				o[0 + k] = c.box.mn.x;
+				o[4 + k] = c.box.mx.x;
+				o[8 + k] = c.box.mn.y;
+				o[12 + k] = c.box.mx.y;
+				o[16 + k] = c.box.mn.z;
+				o[20 + k] = c.box.mx.z;
+				if(c.leaf) {
+					codes[k] = leafCode(c.left, c.right);
+				} else {
+					codes[k] = int32_t(out.wideCount++);
+					children.push_back({kids[k], uint32_t(codes[k]), it.depth + 1});
+				}
+			}
+			out.wide.resize(size_t(out.wideCount) * 32, 0.0f);
+			o = out.wide.data() + size_t(it.slot) * 32;
+			for(int k = 0; k < 4; ++k)
+				o[24 + k] = fromBits(uint32_t(codes[k]));
+			for(auto c = children.rbegin(); c != children.rend(); ++c)
+				wst.push_back(*c);
+		}
+	}
+
 	out.roots.assign(size_t(meshCount) * 8, 0.0f);
 	for(uint32_t r = 0; r < meshCount; ++r) {
 		const HNode & n = nodes[r];
@@ -200,6 +269,14 @@ void HostScene::flatten(DeviceLayout & out) const {
 		o[4] = n.box.mx.x;
 		o[5] = n.box.mx.y;
 		o[6] = n.box.mx.z;
+		o[7] = fromBits(uint32_t(wideRoot[r]));
+		if(r == 0) {
+			out.sceneMin = n.box.mn;
+			out.sceneMax = n.box.mx;
+		} else {
+			out.sceneMin = gmin(out.sceneMin, n.box.mn);
+			out.sceneMax = gmax(out.sceneMax, n.box.mx);
+		}
 	}
 	(void)asBits;
 }

@@ -40,11 +40,21 @@ struct HNode {
 //  tris:  3 float4 per triangle in Rendu's permuted order:
 //         {v0.xyz, e1.x} {e1.yz, e2.xy} {e2.z, meshId, localId, 0 as int bits}, e1 = v1 - v0 and
 //         e2 = v2 - v0 computed in float exactly as the Moller-Trumbore test does.
-//  roots: 2 float4 per mesh {min.xyz, code} {max.xyz, 0}.
+//  roots: 2 float4 per mesh {min.xyz, code} {max.xyz, wide code}.
+//  wide:  the same tree with every internal node's children expanded (largest surface area
+//         first) until it holds up to 4 of Rendu's nodes: 8 float4 per 4-wide node,
+//         {lo.x[4]} {hi.x[4]} {lo.y[4]} {hi.y[4]} {lo.z[4]} {hi.z[4]} {code[4]} {pad}; every box is
+//         a box of Rendu's tree, so a child passes the slab test only if all the boxes between it
+//         and its 4-wide parent do (the test is monotone in the box under IEEE rounding); an
+//         unused slot has code kWideEmpty (rdu_vec.hpp). Used by the near-first kernels.
+
 struct DeviceLayout {
-	std::vector<float> nodes, tris, roots;
+	std::vector<float> nodes, tris, roots, wide;
 	uint32_t internalCount = 0;
 	uint32_t maxDepth = 0;
+	uint32_t wideCount = 0;
+	uint32_t wideDepth = 0;
+	f3 sceneMin{0, 0, 0}, sceneMax{0, 0, 0};
 };
 
 class HostScene {

@@ -23,6 +23,8 @@ struct rdu_raycaster {
 	// Staging for host-pointer queries and global stack spill; guarded by `lock`.
 	mutable std::mutex lock;
 	mutable rdu::DevBuffer stageIn, stageOut, gstack, stats;
+	mutable rdu::DevBuffer keys, keysAlt, perm, permAlt, sortTemp; // incoherent-batch ordering
+	rdu::DevBuffer wide;
 };
 
 using namespace rdu;
@@ -39,12 +41,25 @@ int checkBuilt(const rdu_raycaster * rc) {
 	return RDU_OK;
 }
 
-int ensureGlobalStack(const rdu_raycaster * rc, size_t n) {
-	const size_t ints = globalStackInts(rc->dev, n);
+int ensureGlobalStack(const rdu_raycaster * rc, const TraceOptions & opt, size_t n) {
+	const size_t ints = globalStackInts(rc->dev, opt, n);
 	if(ints == 0)
 		return RDU_OK;
 	return rc->gstack.reserve(ints * sizeof(int));
 }
+
+// Traversal choice for a batch. RDU_TRACE_LAYOUT=binary selects the binary near-first kernels
+// (comparison runs); the 4-wide tree is the default for near-first queries.
+TraceOptions traceOptions(uint32_t flags) {
+	TraceOptions o;
+	o.referenceOrder = (flags & RDU_REFERENCE_ORDER) != 0;
+	const char * lay = std::getenv("RDU_TRACE_LAYOUT");
+	o.wide = !(lay && std::strcmp(lay, "binary") == 0);
+	return o;
+}
+
+// Incoherent batches below this size are traced in input order: the sort does not pay off.
+constexpr size_t kReorderMin = size_t(1) << 16;
 
 template<bool ANY>
 int query(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, void * out, uint32_t flags, void * stream,
@@ -58,13 +73,13 @@ int query(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, void * out, 
 		return fail(RDU_ERR_INVALID, "null rays or output");
 	if(n > (size_t(1) << 31))
 		return fail(RDU_ERR_UNSUPPORTED, "batch larger than 2^31 rays");
-	const bool ref = (flags & RDU_REFERENCE_ORDER) != 0;
+	TraceOptions opt = traceOptions(flags);
 	const size_t outBytes = ANY ? n : n * sizeof(rdu_hit);
 	std::lock_guard<std::mutex> guard(rc->lock);
 	if((st = setDevice(rc->device)))
 		return st;
 	hipStream_t s = static_cast<hipStream_t>(stream);
-	if((st = ensureGlobalStack(rc, n)))
+	if((st = ensureGlobalStack(rc, opt, n)))
 		return st;
 	uint64_t * dstats = nullptr;
 	if(statsOut) {
@@ -89,9 +104,21 @@ int query(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, void * out, 
 		dRays = static_cast<const float4 *>(rc->stageIn.ptr);
 		dOut = rc->stageOut.ptr;
 	}
-	const int e = ANY ? launchAny(rc->dev, dRays, n, static_cast<uint8_t *>(dOut), ref, static_cast<int *>(rc->gstack.ptr),
+	if((flags & RDU_INCOHERENT) && !opt.referenceOrder && n >= kReorderMin) {
+		const size_t tb = sortTempBytes(n);
+		if((st = rc->keys.reserve(n * 4)) || (st = rc->keysAlt.reserve(n * 4)) || (st = rc->perm.reserve(n * 4)) ||
+			(st = rc->permAlt.reserve(n * 4)) || (st = rc->sortTemp.reserve(tb)))
+			return st;
+		if((st = hipCheck(hipError_t(launchRayOrder(rc->dev, dRays, n, static_cast<uint32_t *>(rc->keys.ptr),
+								  static_cast<uint32_t *>(rc->keysAlt.ptr), static_cast<uint32_t *>(rc->perm.ptr),
+								  static_cast<uint32_t *>(rc->permAlt.ptr), rc->sortTemp.ptr, tb, s)),
+				"ray ordering")))
+			return st;
+		opt.perm = static_cast<const uint32_t *>(rc->perm.ptr);
+	}
+	const int e = ANY ? launchAny(rc->dev, dRays, n, static_cast<uint8_t *>(dOut), opt, static_cast<int *>(rc->gstack.ptr),
 							  dstats, s)
-					  : launchClosest(rc->dev, dRays, n, dOut, ref, static_cast<int *>(rc->gstack.ptr), dstats, s);
+					  : launchClosest(rc->dev, dRays, n, dOut, opt, static_cast<int *>(rc->gstack.ptr), dstats, s);
 	if((st = hipCheck(hipError_t(e), "trace kernel launch")))
 		return st;
 	if(!device) {
@@ -167,13 +194,20 @@ int rdu_raycaster_update_hierarchy(rdu_raycaster * rc) {
 		return st;
 	if((st = rc->nodes.upload(rc->layout.nodes.data(), rc->layout.nodes.size() * 4)) ||
 		(st = rc->tris.upload(rc->layout.tris.data(), rc->layout.tris.size() * 4)) ||
-		(st = rc->roots.upload(rc->layout.roots.data(), rc->layout.roots.size() * 4)))
+		(st = rc->roots.upload(rc->layout.roots.data(), rc->layout.roots.size() * 4)) ||
+		(st = rc->wide.upload(rc->layout.wide.data(), rc->layout.wide.size() * 4)))
 		return st;
 	rc->dev.nodes = static_cast<const float4 *>(rc->nodes.ptr);
 	rc->dev.tris = static_cast<const float4 *>(rc->tris.ptr);
 	rc->dev.roots = static_cast<const float4 *>(rc->roots.ptr);
+	rc->dev.wide = static_cast<const float4 *>(rc->wide.ptr);
 	rc->dev.nRoots = rc->scene.meshCount;
 	rc->dev.maxDepth = rc->layout.maxDepth;
+	rc->dev.wideDepth = rc->layout.wideDepth;
+	const f3 lo = rc->layout.sceneMin, hi = rc->layout.sceneMax;
+	const float l3[3] = {lo.x, lo.y, lo.z}, h3[3] = {hi.x, hi.y, hi.z};
+	std::memcpy(rc->dev.sceneMin, l3, sizeof(l3));
+	std::memcpy(rc->dev.sceneMax, h3, sizeof(h3));
 	rc->uploaded = true;
 	return RDU_OK;
 }
@@ -236,7 +270,12 @@ int rdu_raycaster_info(const rdu_raycaster * rc, rdu_bvh_info * out) {
 	out->nodes = rc->scene.nodes.size();
 	out->internal_nodes = rc->layout.internalCount;
 	out->max_depth = rc->layout.maxDepth;
-	out->device_bytes = (rc->layout.nodes.size() + rc->layout.tris.size() + rc->layout.roots.size()) * 4;
+	const uint64_t tr = (rc->layout.tris.size() + rc->layout.roots.size()) * 4;
+	out->binary_bytes = rc->layout.nodes.size() * 4 + tr;
+	out->wide_bytes = rc->layout.wide.size() * 4 + tr;
+	out->device_bytes = out->binary_bytes + rc->layout.wide.size() * 4;
+	out->wide_nodes = rc->layout.wideCount;
+	out->wide_depth = rc->layout.wideDepth;
 	return RDU_OK;
 }
 

@@ -16,6 +16,9 @@
 
 namespace rdu {
 
+// Child code of an unused slot of a 4-wide node.
+constexpr int32_t kWideEmpty = 0x7fffffff;
+
 struct f3 {
 	float x, y, z;
 };

@@ -1,53 +1,97 @@
 // Batch ray/BVH traversal kernels for gfx950: closest hit (Raycaster::intersects,
 // src/engine/raycaster/Raycaster.cpp:144-185) and any hit (Raycaster::intersectsAny,
-// Raycaster.cpp:187-224) over the flattened tree of bvh.hpp.
+// Raycaster.cpp:187-224) over the flattened trees of bvh.hpp.
 //
 // One ray per lane, 64-lane wavefronts. Each lane keeps its traversal stack in LDS, laid out
 // [depth][lane] so that a wavefront's pushes and pops of one level hit 64 consecutive banks;
-// trees deeper than the LDS budget spill the excess levels to a global array laid out
-// [level][ray] (coalesced). Internal nodes hold both children's boxes, so one node fetch
-// (4 x 16 B) feeds the two slab tests of an expansion; triangles are 3 x 16 B records.
+// stacks deeper than the LDS budget spill the excess levels to a global array laid out
+// [level][lane slot] (coalesced).
 //
-// Two visiting orders:
-//  - reference order: Rendu's depth-first order exactly (roots pushed 0..M-1 and tested with the
-//    query's maxi, children tested left then right with the current maxi when their parent is
-//    expanded, right popped first). Same boxes, same tests, same order: hits are the reference's
-//    bit for bit, including which of several equal-t triangles is returned.
-//  - near-first order: the nearer child (by slab entry) is visited first and the farther one is
-//    pushed. Box tests are conservative, so the closest hit is the same triangle with the same
-//    t, u, v, except where two triangles tie at equal t (shared edges), where the reference keeps
-//    whichever its order met first. Any-hit results are order independent.
+// Visiting orders:
+//  - reference order (binary tree): Rendu's depth-first order exactly (roots pushed 0..M-1 and
+//    tested with the query's maxi, children tested left then right with the current maxi when
+//    their parent is expanded, right popped first). Same boxes, same tests, same order: hits are
+//    the reference's bit for bit, including which of several equal-t triangles is returned.
+//  - near-first order (binary or 4-wide tree): the nearest child by slab entry is visited first,
+//    the others are pushed farthest first. Every box is one of Rendu's and the tests are the
+//    same, so the closest hit is the same triangle with the same t, u, v, except where
+//    triangles tie within rounding (shared edges), where the reference keeps whichever its order
+//    met first. Any-hit results are order independent.
+//
+// Slab tests use Rendu's select-based min/max, whose NaN behaviour differs from IEEE minNum.
+// When no lane of a wavefront has an infinite reciprocal direction or a non-finite origin, no
+// slab product can be NaN and the selects equal v_min/v_max (values are only compared, so the
+// sign of a zero result is irrelevant): such wavefronts take the min3/max3 path, the others the
+// select path. The choice is wave-uniform (ballot), so it costs no divergence.
 #include "trace.hpp"
 #include "../../include/rdu.h"
 
 #include <cfloat>
 #include <cstdlib>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 namespace rdu {
 
 namespace {
 
 constexpr int kBlock = 128;
-
+constexpr int kWideLds = 24;
 
 __device__ __forceinline__ f3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
 
-template<bool SPILL>
-__device__ __forceinline__ void push(int (*ls)[kBlock], int * gs, size_t gstride, size_t gid, int lds, int & sp, int v) {
-	if(!SPILL || sp < lds)
-		ls[sp][threadIdx.x] = v;
+template<bool EXACT>
+__device__ __forceinline__ float vmin(float a, float b) {
+	if constexpr(EXACT)
+		return gmin(a, b);
 	else
-		gs[size_t(sp - lds) * gstride + gid] = v;
-	++sp;
+		return __builtin_fminf(a, b);
+}
+template<bool EXACT>
+__device__ __forceinline__ float vmax(float a, float b) {
+	if constexpr(EXACT)
+		return gmax(a, b);
+	else
+		return __builtin_fmaxf(a, b);
 }
 
-template<bool SPILL>
-__device__ __forceinline__ int pop(int (*ls)[kBlock], const int * gs, size_t gstride, size_t gid, int lds, int & sp) {
-	--sp;
-	if(!SPILL || sp < lds)
-		return ls[sp][threadIdx.x];
-	return gs[size_t(sp - lds) * gstride + gid];
+// Intersection::box (Intersection.cpp:22-32) on one box given by its six bounds.
+template<bool EXACT>
+__device__ __forceinline__ bool slab6(const DevRay & r, float lx, float ly, float lz, float hx, float hy, float hz,
+	float mini, float maxi, float & entry) {
+	const float ax = (lx - r.pos.x) * r.inv.x, bx = (hx - r.pos.x) * r.inv.x;
+	const float ay = (ly - r.pos.y) * r.inv.y, by = (hy - r.pos.y) * r.inv.y;
+	const float az = (lz - r.pos.z) * r.inv.z, bz = (hz - r.pos.z) * r.inv.z;
+	const float mnx = vmin<EXACT>(ax, bx), mny = vmin<EXACT>(ay, by), mnz = vmin<EXACT>(az, bz);
+	const float mxx = vmax<EXACT>(ax, bx), mxy = vmax<EXACT>(ay, by), mxz = vmax<EXACT>(az, bz);
+	const float closest = vmax<EXACT>(mnx, vmax<EXACT>(mny, mnz));
+	const float furthest = vmin<EXACT>(mxx, vmin<EXACT>(mxy, mxz));
+	entry = vmax<EXACT>(closest, mini);
+	return entry <= vmin<EXACT>(furthest, maxi);
 }
+
+struct LaneStack {
+	int (*ls)[kBlock];
+	int * gs;
+	size_t gstride, slot;
+	int lds;
+	int sp = 0;
+	template<bool SPILL>
+	__device__ __forceinline__ void push(int v) {
+		if(!SPILL || sp < lds)
+			ls[sp][threadIdx.x] = v;
+		else
+			gs[size_t(sp - lds) * gstride + slot] = v;
+		++sp;
+	}
+	template<bool SPILL>
+	__device__ __forceinline__ int pop() {
+		--sp;
+		if(!SPILL || sp < lds)
+			return ls[sp][threadIdx.x];
+		return gs[size_t(sp - lds) * gstride + slot];
+	}
+};
 
 __device__ __forceinline__ uint64_t waveSum(uint64_t v) {
 	for(int off = 32; off > 0; off >>= 1)
@@ -55,113 +99,215 @@ __device__ __forceinline__ uint64_t waveSum(uint64_t v) {
 	return v;
 }
 
-template<bool ANY, bool REF, bool SPILL, bool STATS>
-__global__ __launch_bounds__(kBlock) void traceKernel(DevBVH bvh, const float4 * __restrict__ rays, size_t n,
-	rdu_hit * __restrict__ hits, uint8_t * __restrict__ occ, int * __restrict__ gstack, int lds, uint64_t * stats) {
-	__shared__ int ls[kLdsStack][kBlock];
-	const size_t gid = size_t(blockIdx.x) * kBlock + threadIdx.x;
-	const size_t gstride = size_t(gridDim.x) * kBlock;
+struct TraceState {
+	float maxi;
+	int best = -1;
+	float bt = FLT_MAX, bu = 0.0f, bv = 0.0f;
+	bool found = false;
 	uint64_t nodeVisits = 0, triTests = 0;
-	if(gid < n) {
-		const float4 ra = rays[2 * gid];
-		const float4 rb = rays[2 * gid + 1];
-		const DevRay r = makeRay(xyz(ra), xyz(rb), ra.w, rb.w);
-		const float mini = r.mini;
-		const float maxi0 = r.maxi;
-		float maxi = maxi0;
-		int best = -1;
-		float bt = FLT_MAX, bu = 0.0f, bv = 0.0f;
-		bool found = false;
+};
 
-		// Rendu pushes every root whose box passes (tested with the query's maxi), then pops the
-		// last one first: visiting roots from M-1 down to 0 is the same order.
-		for(int ri = int(bvh.nRoots) - 1; ri >= 0 && !found; --ri) {
-			const float4 rmin = bvh.roots[2 * ri];
-			const float4 rmax = bvh.roots[2 * ri + 1];
-			float e;
-			if(!slab(r, xyz(rmin), xyz(rmax), mini, REF ? maxi0 : maxi, e))
-				continue;
-			int sp = 0;
-			int c = __float_as_int(rmin.w);
-			while(true) {
-				if(c >= 0) {
-					if(STATS)
-						++nodeVisits;
-					const float4 n0 = bvh.nodes[4 * c + 0];
-					const float4 n1 = bvh.nodes[4 * c + 1];
-					const float4 n2 = bvh.nodes[4 * c + 2];
-					const float4 n3 = bvh.nodes[4 * c + 3];
-					float eL, eR;
-					const bool hL = slab(r, mk(n0.x, n0.y, n0.z), mk(n0.w, n1.x, n1.y), mini, maxi, eL);
-					const bool hR = slab(r, mk(n1.z, n1.w, n2.x), mk(n2.y, n2.z, n2.w), mini, maxi, eR);
-					const int lc = __float_as_int(n3.x);
-					const int rc = __float_as_int(n3.y);
-					if(REF) {
-						// Push left then right; the right child is popped first.
-						if(hL)
-							push<SPILL>(ls, gstack, gstride, gid, lds, sp, lc);
-						if(hR) {
-							c = rc;
-							continue;
-						}
-					} else {
-						if(hL && hR) {
-							const bool leftNear = eL <= eR;
-							push<SPILL>(ls, gstack, gstride, gid, lds, sp, leftNear ? rc : lc);
-							c = leftNear ? lc : rc;
-							continue;
-						}
-						if(hL) {
-							c = lc;
-							continue;
-						}
-						if(hR) {
-							c = rc;
-							continue;
-						}
-					}
-				} else {
-					const int code = ~c;
-					const int first = code >> 1;
-					const int cnt = (code & 1) + 1;
-					for(int k = 0; k < cnt; ++k) {
-						if(STATS)
-							++triTests;
-						const int ti = first + k;
-						const float4 t0 = bvh.tris[3 * ti + 0];
-						const float4 t1 = bvh.tris[3 * ti + 1];
-						const float4 t2 = bvh.tris[3 * ti + 2];
-						float tt, uu, vv;
-						if(mollerTrumbore(r, xyz(t0), mk(t0.w, t1.x, t1.y), mk(t1.z, t1.w, t2.x), mini, maxi, tt, uu, vv)) {
-							if(ANY) {
-								found = true;
-								break;
-							}
-							// hit.dist < bestHit.dist holds whenever the test passed: t < maxi = best.
-							best = ti;
-							bt = tt;
-							bu = uu;
-							bv = vv;
-							maxi = tt;
-						}
-					}
-					if(ANY && found)
-						break;
-				}
-				if(sp == 0)
-					break;
-				c = pop<SPILL>(ls, gstack, gstride, gid, lds, sp);
+// Leaf: Moller-Trumbore on its 1-2 triangles in order. Returns true when an any-hit query is done.
+template<bool ANY, bool STATS>
+__device__ __forceinline__ bool testLeaf(const DevBVH & bvh, const DevRay & r, int c, float mini, TraceState & s) {
+	const int code = ~c;
+	const int first = code >> 1;
+	const int cnt = (code & 1) + 1;
+	for(int k = 0; k < cnt; ++k) {
+		if(STATS)
+			++s.triTests;
+		const int ti = first + k;
+		const float4 t0 = bvh.tris[3 * ti + 0];
+		const float4 t1 = bvh.tris[3 * ti + 1];
+		const float4 t2 = bvh.tris[3 * ti + 2];
+		float tt, uu, vv;
+		if(mollerTrumbore(r, xyz(t0), mk(t0.w, t1.x, t1.y), mk(t1.z, t1.w, t2.x), mini, s.maxi, tt, uu, vv)) {
+			if(ANY) {
+				s.found = true;
+				return true;
 			}
+			// hit.dist < bestHit.dist holds whenever the test passed: t < maxi = best.
+			s.best = ti;
+			s.bt = tt;
+			s.bu = uu;
+			s.bv = vv;
+			s.maxi = tt;
 		}
+	}
+	return false;
+}
+
+// Binary tree, reference or near-first order, from one root code.
+template<bool ANY, bool REF, bool SPILL, bool EXACT, bool STATS>
+__device__ __forceinline__ void traverseBinary(const DevBVH & bvh, const DevRay & r, int c, float mini,
+	LaneStack & st, TraceState & s) {
+	st.sp = 0;
+	while(true) {
+		if(c >= 0) {
+			if(STATS)
+				++s.nodeVisits;
+			const float4 n0 = bvh.nodes[4 * c + 0];
+			const float4 n1 = bvh.nodes[4 * c + 1];
+			const float4 n2 = bvh.nodes[4 * c + 2];
+			const float4 n3 = bvh.nodes[4 * c + 3];
+			float eL, eR;
+			const bool hL = slab6<EXACT>(r, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, mini, s.maxi, eL);
+			const bool hR = slab6<EXACT>(r, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, mini, s.maxi, eR);
+			const int lc = __float_as_int(n3.x);
+			const int rc = __float_as_int(n3.y);
+			if(REF) {
+				// Push left then right; the right child is popped first.
+				if(hL)
+					st.push<SPILL>(lc);
+				if(hR) {
+					c = rc;
+					continue;
+				}
+			} else {
+				if(hL && hR) {
+					const bool leftNear = eL <= eR;
+					st.push<SPILL>(leftNear ? rc : lc);
+					c = leftNear ? lc : rc;
+					continue;
+				}
+				if(hL || hR) {
+					c = hL ? lc : rc;
+					continue;
+				}
+			}
+		} else if(testLeaf<ANY, STATS>(bvh, r, c, mini, s)) {
+			return;
+		}
+		if(st.sp == 0)
+			return;
+		c = st.pop<SPILL>();
+	}
+}
+
+__device__ __forceinline__ void cswap(float & a, int & ca, float & b, int & cb) {
+	const bool sw = b < a;
+	const float ta = sw ? b : a, tb = sw ? a : b;
+	const int tca = sw ? cb : ca, tcb = sw ? ca : cb;
+	a = ta;
+	b = tb;
+	ca = tca;
+	cb = tcb;
+}
+
+// 4-wide tree, near-first order, from one root code.
+template<bool ANY, bool SPILL, bool EXACT, bool STATS>
+__device__ __forceinline__ void traverseWide(const DevBVH & bvh, const DevRay & r, int c, float mini, LaneStack & st,
+	TraceState & s) {
+	st.sp = 0;
+	while(true) {
+		if(c >= 0) {
+			if(STATS)
+				++s.nodeVisits;
+			const float4 * nd = bvh.wide + 8 * size_t(c);
+			const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+			const float4 cf = nd[6];
+			int c0 = __float_as_int(cf.x), c1 = __float_as_int(cf.y), c2 = __float_as_int(cf.z),
+				c3 = __float_as_int(cf.w);
+			float e0, e1, e2 = 0.0f, e3 = 0.0f;
+			const bool h0 = slab6<EXACT>(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, mini, s.maxi, e0);
+			const bool h1 = slab6<EXACT>(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, mini, s.maxi, e1);
+			const bool h2 = (c2 != kWideEmpty) && slab6<EXACT>(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, mini, s.maxi, e2);
+			const bool h3 = (c3 != kWideEmpty) && slab6<EXACT>(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, mini, s.maxi, e3);
+			const int nh = int(h0) + int(h1) + int(h2) + int(h3);
+			if(nh == 0) {
+				if(st.sp == 0)
+					return;
+				c = st.pop<SPILL>();
+				continue;
+			}
+			// Hit entries are finite (clamped below +inf), misses are +inf: after the sorting
+			// network the nh hit children come first, nearest first.
+			e0 = h0 ? __builtin_fminf(e0, FLT_MAX) : __builtin_inff();
+			e1 = h1 ? __builtin_fminf(e1, FLT_MAX) : __builtin_inff();
+			e2 = h2 ? __builtin_fminf(e2, FLT_MAX) : __builtin_inff();
+			e3 = h3 ? __builtin_fminf(e3, FLT_MAX) : __builtin_inff();
+			cswap(e0, c0, e1, c1);
+			cswap(e2, c2, e3, c3);
+			cswap(e0, c0, e2, c2);
+			cswap(e1, c1, e3, c3);
+			cswap(e1, c1, e2, c2);
+			if(nh > 3)
+				st.push<SPILL>(c3);
+			if(nh > 2)
+				st.push<SPILL>(c2);
+			if(nh > 1)
+				st.push<SPILL>(c1);
+			c = c0;
+			continue;
+		}
+		if(testLeaf<ANY, STATS>(bvh, r, c, mini, s))
+			return;
+		if(st.sp == 0)
+			return;
+		c = st.pop<SPILL>();
+	}
+}
+
+template<bool ANY, bool REF, bool WIDE, bool SPILL, bool EXACT, bool STATS>
+__device__ __forceinline__ void traceRay(const DevBVH & bvh, const DevRay & r, LaneStack & st, TraceState & s) {
+	const float mini = r.mini;
+	const float maxi0 = r.maxi;
+	// Rendu pushes every root whose box passes (tested with the query's maxi), then pops the last
+	// one first: visiting roots from M-1 down to 0 is the same order.
+	for(int ri = int(bvh.nRoots) - 1; ri >= 0; --ri) {
+		const float4 rmin = bvh.roots[2 * ri];
+		const float4 rmax = bvh.roots[2 * ri + 1];
+		float e;
+		if(!slab6<EXACT>(r, rmin.x, rmin.y, rmin.z, rmax.x, rmax.y, rmax.z, mini, REF ? maxi0 : s.maxi, e))
+			continue;
+		if(WIDE)
+			traverseWide<ANY, SPILL, EXACT, STATS>(bvh, r, __float_as_int(rmax.w), mini, st, s);
+		else
+			traverseBinary<ANY, REF, SPILL, EXACT, STATS>(bvh, r, __float_as_int(rmin.w), mini, st, s);
+		if(ANY && s.found)
+			return;
+	}
+}
+
+__device__ __forceinline__ bool finite3(f3 v) {
+	return __builtin_isfinite(v.x) && __builtin_isfinite(v.y) && __builtin_isfinite(v.z);
+}
+
+template<bool ANY, bool REF, bool WIDE, bool SPILL, bool STATS>
+__global__ __launch_bounds__(kBlock) void traceKernel(DevBVH bvh, const float4 * __restrict__ rays, size_t n,
+	const uint32_t * __restrict__ perm, rdu_hit * __restrict__ hits, uint8_t * __restrict__ occ,
+	int * __restrict__ gstack, int lds, uint64_t * stats) {
+	__shared__ int ls[WIDE ? kWideLds : kLdsStack][kBlock];
+	const size_t gid = size_t(blockIdx.x) * kBlock + threadIdx.x;
+	LaneStack st{ls, gstack, size_t(gridDim.x) * kBlock, gid, lds};
+	TraceState s;
+	const bool active = gid < n;
+	size_t ri = gid;
+	DevRay r;
+	if(active) {
+		ri = perm ? size_t(perm[gid]) : gid;
+		const float4 ra = rays[2 * ri];
+		const float4 rb = rays[2 * ri + 1];
+		r = makeRay(xyz(ra), xyz(rb), ra.w, rb.w);
+	} else {
+		r = makeRay(mk(0, 0, 0), mk(0, 0, 1), 0.0f, 0.0f);
+	}
+	s.maxi = r.maxi;
+	const bool clean = finite3(r.inv) && finite3(r.pos);
+	if(active) {
+		if(__all(clean))
+			traceRay<ANY, REF, WIDE, SPILL, false, STATS>(bvh, r, st, s);
+		else
+			traceRay<ANY, REF, WIDE, SPILL, true, STATS>(bvh, r, st, s);
 		if(ANY) {
-			occ[gid] = found ? 1 : 0;
+			occ[ri] = s.found ? 1 : 0;
 		} else {
 			rdu_hit h;
-			if(best >= 0) {
-				const float4 t2 = bvh.tris[3 * best + 2];
-				h.t = bt;
-				h.u = bu;
-				h.v = bv;
+			if(s.best >= 0) {
+				const float4 t2 = bvh.tris[3 * s.best + 2];
+				h.t = s.bt;
+				h.u = s.bu;
+				h.v = s.bv;
 				h.hit = 1u;
 				h.mesh_id = __float_as_uint(t2.y);
 				h.local_id = __float_as_uint(t2.z);
@@ -173,12 +319,12 @@ __global__ __launch_bounds__(kBlock) void traceKernel(DevBVH bvh, const float4 *
 				h.mesh_id = 0u;
 				h.local_id = 0u;
 			}
-			hits[gid] = h;
+			hits[ri] = h;
 		}
 	}
 	if(STATS) {
-		const uint64_t a = waveSum(nodeVisits);
-		const uint64_t b = waveSum(triTests);
+		const uint64_t a = waveSum(s.nodeVisits);
+		const uint64_t b = waveSum(s.triTests);
 		if((threadIdx.x & 63) == 0) {
 			atomicAdd(reinterpret_cast<unsigned long long *>(stats), (unsigned long long)a);
 			atomicAdd(reinterpret_cast<unsigned long long *>(stats + 1), (unsigned long long)b);
@@ -187,31 +333,73 @@ __global__ __launch_bounds__(kBlock) void traceKernel(DevBVH bvh, const float4 *
 }
 
 template<bool ANY>
-int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits, uint8_t * occ, bool ref, int * gstack,
-	uint64_t * stats, hipStream_t stream) {
+int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits, uint8_t * occ, const TraceOptions & opt,
+	int * gstack, uint64_t * stats, hipStream_t stream) {
 	if(n == 0)
 		return 0;
-	const int lds = ldsStackDepth();
-	const bool spill = int(bvh.maxDepth) + 2 > lds;
+	const bool wide = opt.wide && !opt.referenceOrder && bvh.wide != nullptr;
+	const int lds = wide ? (ldsStackDepth() < kWideLds ? ldsStackDepth() : kWideLds) : ldsStackDepth();
+	TraceOptions o = opt;
+	o.wide = wide;
+	const bool spill = stackNeed(bvh, o) > lds;
 	const dim3 grid(unsigned((n + kBlock - 1) / kBlock));
 	const dim3 block(kBlock);
-#define RDU_LAUNCH(R, S, ST) \
-	hipLaunchKernelGGL((traceKernel<ANY, R, S, ST>), grid, block, 0, stream, bvh, rays, n, hits, occ, gstack, lds, stats)
+	const uint32_t * perm = opt.perm;
+#define RDU_LAUNCH(R, W, S, ST)                                                                               \
+	hipLaunchKernelGGL((traceKernel<ANY, R, W, S, ST>), grid, block, 0, stream, bvh, rays, n, perm, hits, occ, gstack, \
+		lds, stats)
+#define RDU_BY_SPILL(R, W, ST)     \
+	if(spill)                      \
+		RDU_LAUNCH(R, W, true, ST); \
+	else                           \
+		RDU_LAUNCH(R, W, false, ST)
 	if(stats) {
-		if(ref) {
-			if(spill) RDU_LAUNCH(true, true, true); else RDU_LAUNCH(true, false, true);
+		if(opt.referenceOrder) {
+			RDU_BY_SPILL(true, false, true);
+		} else if(wide) {
+			RDU_BY_SPILL(false, true, true);
 		} else {
-			if(spill) RDU_LAUNCH(false, true, true); else RDU_LAUNCH(false, false, true);
+			RDU_BY_SPILL(false, false, true);
 		}
 	} else {
-		if(ref) {
-			if(spill) RDU_LAUNCH(true, true, false); else RDU_LAUNCH(true, false, false);
+		if(opt.referenceOrder) {
+			RDU_BY_SPILL(true, false, false);
+		} else if(wide) {
+			RDU_BY_SPILL(false, true, false);
 		} else {
-			if(spill) RDU_LAUNCH(false, true, false); else RDU_LAUNCH(false, false, false);
+			RDU_BY_SPILL(false, false, false);
 		}
 	}
+#undef RDU_BY_SPILL
 #undef RDU_LAUNCH
 	return int(hipGetLastError());
+}
+
+// Morton code of the origin's cell in the scene box, kOrderBits per axis.
+__device__ __forceinline__ uint32_t spread3(uint32_t x) {
+	x &= 0x3ff;
+	x = (x | (x << 16)) & 0x030000ff;
+	x = (x | (x << 8)) & 0x0300f00f;
+	x = (x | (x << 4)) & 0x030c30c3;
+	x = (x | (x << 2)) & 0x09249249;
+	return x;
+}
+
+__global__ __launch_bounds__(256) void rayKeysKernel(const float4 * __restrict__ rays, size_t n, float3 lo, float3 scale,
+	uint32_t * __restrict__ keys, uint32_t * __restrict__ idx) {
+	const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+	if(i >= n)
+		return;
+	const float4 o = rays[2 * i];
+	const float cells = float(1u << kOrderBits);
+	auto q = [cells](float v) {
+		// NaN and out-of-box origins clamp to the border cells.
+		const float c = __builtin_fminf(__builtin_fmaxf(v, 0.0f), cells - 1.0f);
+		return uint32_t(c);
+	};
+	const uint32_t x = q((o.x - lo.x) * scale.x), y = q((o.y - lo.y) * scale.y), z = q((o.z - lo.z) * scale.z);
+	keys[i] = (spread3(x) << 2) | (spread3(y) << 1) | spread3(z);
+	idx[i] = uint32_t(i);
 }
 
 } // namespace
@@ -224,22 +412,59 @@ int ldsStackDepth() {
 	return (k >= 1 && k <= kLdsStack) ? k : kLdsStack;
 }
 
-size_t globalStackInts(const DevBVH & bvh, size_t n) {
-	const int need = int(bvh.maxDepth) + 2 - ldsStackDepth();
+int stackNeed(const DevBVH & bvh, const TraceOptions & opt) {
+	const bool wide = opt.wide && !opt.referenceOrder && bvh.wide != nullptr;
+	return wide ? 3 * int(bvh.wideDepth) + 2 : int(bvh.maxDepth) + 2;
+}
+
+size_t globalStackInts(const DevBVH & bvh, const TraceOptions & opt, size_t n) {
+	const bool wide = opt.wide && !opt.referenceOrder && bvh.wide != nullptr;
+	const int lds = wide ? (ldsStackDepth() < kWideLds ? ldsStackDepth() : kWideLds) : ldsStackDepth();
+	const int need = stackNeed(bvh, opt) - lds;
 	if(need <= 0)
 		return 0;
 	const size_t slots = ((n + kBlock - 1) / kBlock) * kBlock;
 	return size_t(need) * slots;
 }
 
-int launchClosest(const DevBVH & bvh, const float4 * rays, size_t n, void * hits, bool referenceOrder, int * gstack,
+int launchClosest(const DevBVH & bvh, const float4 * rays, size_t n, void * hits, const TraceOptions & opt, int * gstack,
 	uint64_t * stats, hipStream_t stream) {
-	return launchImpl<false>(bvh, rays, n, static_cast<rdu_hit *>(hits), nullptr, referenceOrder, gstack, stats, stream);
+	return launchImpl<false>(bvh, rays, n, static_cast<rdu_hit *>(hits), nullptr, opt, gstack, stats, stream);
 }
 
-int launchAny(const DevBVH & bvh, const float4 * rays, size_t n, uint8_t * occluded, bool referenceOrder, int * gstack,
-	uint64_t * stats, hipStream_t stream) {
-	return launchImpl<true>(bvh, rays, n, nullptr, occluded, referenceOrder, gstack, stats, stream);
+int launchAny(const DevBVH & bvh, const float4 * rays, size_t n, uint8_t * occluded, const TraceOptions & opt,
+	int * gstack, uint64_t * stats, hipStream_t stream) {
+	return launchImpl<true>(bvh, rays, n, nullptr, occluded, opt, gstack, stats, stream);
+}
+
+size_t sortTempBytes(size_t n) {
+	size_t bytes = 0;
+	(void)rocprim::radix_sort_pairs(nullptr, bytes, static_cast<uint32_t *>(nullptr), static_cast<uint32_t *>(nullptr),
+		static_cast<uint32_t *>(nullptr), static_cast<uint32_t *>(nullptr), n, 0, 3 * kOrderBits);
+	return bytes;
+}
+
+int launchRayOrder(const DevBVH & bvh, const float4 * rays, size_t n, uint32_t * keys, uint32_t * keysAlt,
+	uint32_t * perm, uint32_t * permAlt, void * temp, size_t tempBytes, hipStream_t stream) {
+	if(n == 0)
+		return 0;
+	const float cells = float(1u << kOrderBits);
+	float3 lo, scale;
+	float ext[3];
+	for(int a = 0; a < 3; ++a) {
+		ext[a] = bvh.sceneMax[a] - bvh.sceneMin[a];
+		ext[a] = ext[a] > 0.0f ? ext[a] : 1.0f;
+	}
+	lo = make_float3(bvh.sceneMin[0], bvh.sceneMin[1], bvh.sceneMin[2]);
+	scale = make_float3(cells / ext[0], cells / ext[1], cells / ext[2]);
+	hipLaunchKernelGGL(rayKeysKernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream, rays, n, lo, scale, keysAlt,
+		permAlt);
+	hipError_t e = hipGetLastError();
+	if(e != hipSuccess)
+		return int(e);
+	size_t bytes = tempBytes;
+	e = rocprim::radix_sort_pairs(temp, bytes, keysAlt, keys, permAlt, perm, n, 0, 3 * kOrderBits, stream);
+	return int(e);
 }
 
 } // namespace rdu

@@ -12,8 +12,11 @@ struct DevBVH {
 	const float4 * nodes = nullptr;
 	const float4 * tris = nullptr;
 	const float4 * roots = nullptr;
+	const float4 * wide = nullptr;
 	uint32_t nRoots = 0;
 	uint32_t maxDepth = 0;
+	uint32_t wideDepth = 0;
+	float sceneMin[3] = {0, 0, 0}, sceneMax[3] = {0, 0, 0};
 };
 
 struct DevRay {
@@ -64,19 +67,33 @@ RDU_HD bool mollerTrumbore(const DevRay & r, f3 v0, f3 e1, f3 e2, float mini, fl
 	return t > mini && t < maxi;
 }
 
-enum TraceMode : int { kClosest = 0, kAny = 1 };
+// How a batch is traversed.
+struct TraceOptions {
+	bool referenceOrder = false; // Rendu's exact depth-first order over the binary tree
+	bool wide = true;            // near-first order over the 4-wide collapse (else binary)
+	const uint32_t * perm = nullptr; // optional ray permutation: lane i handles ray perm[i]
+};
 
-// Launchers. rays: device rdu_ray array (float4 pairs); outputs are device pointers.
-// Returns hipError_t as int.
-int launchClosest(const DevBVH & bvh, const float4 * rays, size_t n, void * hits, bool referenceOrder,
+// Launchers. rays: device rdu_ray array (float4 pairs); outputs are device pointers, indexed by
+// the original ray index. Return hipError_t as int.
+int launchClosest(const DevBVH & bvh, const float4 * rays, size_t n, void * hits, const TraceOptions & opt,
 	int * gstack, uint64_t * stats, hipStream_t stream);
-int launchAny(const DevBVH & bvh, const float4 * rays, size_t n, uint8_t * occluded, bool referenceOrder,
+int launchAny(const DevBVH & bvh, const float4 * rays, size_t n, uint8_t * occluded, const TraceOptions & opt,
 	int * gstack, uint64_t * stats, hipStream_t stream);
 
-// Number of stack entries the kernels keep in LDS per ray; deeper trees spill to `gstack`,
-// which then needs (maxDepth + 2 - kLdsStack) ints per ray slot of the launch.
+// Number of stack entries the kernels keep in LDS per ray; deeper stacks spill to `gstack`,
+// which then needs globalStackInts() ints for the launch.
 constexpr int kLdsStack = 32;
 int ldsStackDepth();
-size_t globalStackInts(const DevBVH & bvh, size_t n);
+// Worst-case stack entries of a traversal: binary tree depth + 2, or 3 per 4-wide level + 2.
+int stackNeed(const DevBVH & bvh, const TraceOptions & opt);
+size_t globalStackInts(const DevBVH & bvh, const TraceOptions & opt, size_t n);
+
+// Ray reordering for incoherent batches: keys = Morton code of the origin cell in the scene box
+// (kOrderBits per axis), sorted with the ray index. perm receives the ray indices in key order.
+constexpr int kOrderBits = 7;
+size_t sortTempBytes(size_t n);
+int launchRayOrder(const DevBVH & bvh, const float4 * rays, size_t n, uint32_t * keys, uint32_t * keysAlt,
+	uint32_t * perm, uint32_t * permAlt, void * temp, size_t tempBytes, hipStream_t stream);
 
 } // namespace rdu

@@ -50,6 +50,10 @@ def make_rays(origins, directions, mini=1e-4, maxi=1e8) -> np.ndarray:
     return rays
 
 
+def _flags(reference_order: bool, incoherent: bool) -> int:
+    return (_lib.RDU_REFERENCE_ORDER if reference_order else 0) | (_lib.RDU_INCOHERENT if incoherent else 0)
+
+
 class Raycaster:
     """Raycaster (Raycaster.hpp:10). Not copyable, like the reference (Raycaster.hpp:100-113)."""
 
@@ -122,26 +126,27 @@ class Raycaster:
         return (w * a[i0] + u * a[i1]) + v * a[i2]
 
     # -- batch API ----------------------------------------------------------------------------------
-    def intersect_batch(self, rays: np.ndarray, reference_order: bool = False) -> np.ndarray:
+    def intersect_batch(self, rays: np.ndarray, reference_order: bool = False, incoherent: bool = False) -> np.ndarray:
         """Closest hit of every rdu_ray record (host arrays, synchronous)."""
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
         hits = np.empty(rays.shape[0], dtype=HIT_DTYPE)
-        flags = _lib.RDU_REFERENCE_ORDER if reference_order else 0
+        flags = _flags(reference_order, incoherent)
         check(lib.rdu_raycaster_intersect(self._h, vptr(rays), rays.shape[0], vptr(hits), flags, None))
         return hits
 
-    def intersect_any_batch(self, rays: np.ndarray, reference_order: bool = False) -> np.ndarray:
+    def intersect_any_batch(self, rays: np.ndarray, reference_order: bool = False,
+                            incoherent: bool = False) -> np.ndarray:
         """Any-hit occlusion of every rdu_ray record, as a bool array."""
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
         occ = np.empty(rays.shape[0], dtype=np.uint8)
-        flags = _lib.RDU_REFERENCE_ORDER if reference_order else 0
+        flags = _flags(reference_order, incoherent)
         check(lib.rdu_raycaster_intersect_any(self._h, vptr(rays), rays.shape[0], vptr(occ), flags, None))
         return occ.astype(bool)
 
     def intersect_device(self, rays_ptr: int, n: int, out_ptr: int, stream: int = 0, any_hit: bool = False,
-                         reference_order: bool = False) -> None:
+                         reference_order: bool = False, incoherent: bool = False) -> None:
         """Asynchronous query on device buffers (raw device addresses) on a HIP stream handle."""
-        flags = _lib.RDU_DEVICE_POINTERS | (_lib.RDU_REFERENCE_ORDER if reference_order else 0)
+        flags = _lib.RDU_DEVICE_POINTERS | _flags(reference_order, incoherent)
         fn = lib.rdu_raycaster_intersect_any if any_hit else lib.rdu_raycaster_intersect
         check(fn(self._h, ctypes.c_void_p(rays_ptr), n, ctypes.c_void_p(out_ptr), flags,
                  ctypes.c_void_p(stream) if stream else None))

@@ -111,6 +111,19 @@ def test_near_first_closest(mixed, mixed_rays):
     compare_near_first(got, ref.intersect(mixed_rays, "closest"))
 
 
+@pytest.mark.parametrize("layout", ["wide", "binary"])
+@pytest.mark.parametrize("incoherent", [False, True])
+def test_near_first_layouts_and_ordering(mixed, mixed_rays, monkeypatch, layout, incoherent):
+    ours, ref = mixed
+    if layout == "binary":
+        monkeypatch.setenv("RDU_TRACE_LAYOUT", "binary")
+    log_case(f"near-first layout={layout} incoherent={incoherent}")
+    exp = ref.intersect(mixed_rays, "closest")
+    compare_near_first(ours.intersect_batch(mixed_rays, incoherent=incoherent), exp)
+    occ = ours.intersect_any_batch(mixed_rays, incoherent=incoherent)
+    assert np.array_equal(occ, exp["hit"].astype(bool))
+
+
 @pytest.mark.parametrize("reference_order", [False, True])
 def test_any_hit_exact(mixed, mixed_rays, reference_order):
     ours, ref = mixed
@@ -152,6 +165,8 @@ def test_spill_path_bit_exact(mixed, mixed_rays, monkeypatch):
     compare_near_first(got, ref.intersect(mixed_rays[:20000], "closest"))
     occ = ours.intersect_any_batch(mixed_rays[:20000])
     assert np.array_equal(occ, ref.intersect(mixed_rays[:20000], "any")["hit"].astype(bool))
+    monkeypatch.setenv("RDU_TRACE_LAYOUT", "binary")
+    compare_near_first(ours.intersect_batch(mixed_rays), ref.intersect(mixed_rays, "closest"))
 
 
 def test_many_roots_and_tiny_meshes():
