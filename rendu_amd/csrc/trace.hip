@@ -332,6 +332,19 @@ __global__ __launch_bounds__(kBlock) void traceKernel(DevBVH bvh, const float4 *
 	}
 }
 
+template<bool ANY, bool SPILL, bool STATS>
+__global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
+	const uint32_t * __restrict__ perm, rdu_hit * __restrict__ hits, uint8_t * __restrict__ occ,
+	int * __restrict__ gstack, int lds, uint32_t share, uint64_t * stats);
+
+struct PersistentGrid {
+	unsigned blocks = 0;
+	uint32_t share = 0;
+};
+
+template<bool ANY, bool SPILL, bool STATS>
+PersistentGrid persistentGrid(size_t n);
+
 template<bool ANY>
 int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits, uint8_t * occ, const TraceOptions & opt,
 	int * gstack, uint64_t * stats, hipStream_t stream) {
@@ -353,6 +366,25 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 		RDU_LAUNCH(R, W, true, ST); \
 	else                           \
 		RDU_LAUNCH(R, W, false, ST)
+	const char * kern = std::getenv("RDU_TRACE_KERNEL");
+	const bool persistent = wide && !(kern && kern[0] == 's'); // RDU_TRACE_KERNEL=simple: one ray per lane
+	if(persistent) {
+		if(n > 0xffffffffull)
+			return int(hipErrorInvalidValue);
+#define RDU_PERSIST(S, ST)                                                                                       \
+	{                                                                                                            \
+		const PersistentGrid g = persistentGrid<ANY, S, ST>(n);                                                  \
+		hipLaunchKernelGGL((tracePersistent<ANY, S, ST>), dim3(g.blocks), dim3(kBlock), 0, stream, bvh, rays,      \
+			uint32_t(n), perm, hits, occ, gstack, lds, g.share, stats);                                          \
+	}
+		if(stats) {
+			if(spill) RDU_PERSIST(true, true) else RDU_PERSIST(false, true)
+		} else {
+			if(spill) RDU_PERSIST(true, false) else RDU_PERSIST(false, false)
+		}
+#undef RDU_PERSIST
+		return int(hipGetLastError());
+	}
 	if(stats) {
 		if(opt.referenceOrder) {
 			RDU_BY_SPILL(true, false, true);
@@ -373,6 +405,239 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 #undef RDU_BY_SPILL
 #undef RDU_LAUNCH
 	return int(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------------
+// Persistent near-first traversal of the 4-wide tree with lane refill.
+//
+// The one-ray-per-lane kernel above makes every wavefront wait for its slowest ray, and each
+// loop trip can pay two dependent memory round trips (a node fetch, then a triangle fetch by
+// other lanes). Here every lane is a small state machine that advances by exactly one step per
+// trip — one wide node, one triangle, or one root box — so a trip issues at most one fetch per
+// lane and waits once; lanes whose ray is finished take the next ray of their wavefront's share
+// of the batch on the next trip, so the wavefront stays full until its share is exhausted.
+// Wavefront w owns rays [w * share, (w + 1) * share) of the (optionally permuted) batch; it
+// exits when its share is done, so every wave of the grid terminates.
+constexpr int kNeedRoot = 0x7ffffffe; // state: test the next root box
+
+template<bool ANY, bool SPILL, bool STATS>
+__global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
+	const uint32_t * __restrict__ perm, rdu_hit * __restrict__ hits, uint8_t * __restrict__ occ,
+	int * __restrict__ gstack, int lds, uint32_t share, uint64_t * stats) {
+	__shared__ int ls[kWideLds][kBlock];
+	const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
+	const uint32_t wave = gtid >> 6;
+	const uint64_t begin64 = uint64_t(wave) * share;
+	uint32_t cursor = begin64 < n ? uint32_t(begin64) : n;
+	const uint32_t end = begin64 + share < n ? uint32_t(begin64 + share) : n;
+	LaneStack st{ls, gstack, size_t(gridDim.x) * kBlock, gtid, lds};
+	uint64_t nodeVisits = 0, triTests = 0;
+
+	bool has = false;
+	uint32_t ri = 0;
+	DevRay r = makeRay(mk(0, 0, 0), mk(0, 0, 1), 0.0f, 0.0f);
+	bool clean = true;
+	int c = 0, root = 0, kk = 0;
+	float maxi = 0.0f, bt = FLT_MAX, bu = 0.0f, bv = 0.0f;
+	int best = -1;
+	while(true) {
+		// Refill idle lanes from this wavefront's share (wave-uniform cursor, no atomics).
+		const uint64_t idle = __ballot(!has);
+		if(idle) {
+			if(!has) {
+				const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(idle >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0u));
+				const uint32_t k = cursor + rank;
+				if(k < end) {
+					ri = perm ? perm[k] : k;
+					const float4 ra = rays[2 * size_t(ri)];
+					const float4 rb = rays[2 * size_t(ri) + 1];
+					r = makeRay(xyz(ra), xyz(rb), ra.w, rb.w);
+					clean = finite3(r.inv) && finite3(r.pos);
+					maxi = r.maxi;
+					best = -1;
+					bt = FLT_MAX;
+					bu = bv = 0.0f;
+					st.sp = 0;
+					kk = 0;
+					root = int(bvh.nRoots);
+					c = kNeedRoot;
+					has = true;
+				}
+			}
+			const uint32_t take = uint32_t(__popcll(idle));
+			cursor = (end - cursor > take) ? cursor + take : end;
+		}
+		if(!__any(has))
+			break;
+
+		// Fetch: one record per lane.
+		float4 a0 = make_float4(0, 0, 0, 0), a1 = a0, a2 = a0, a3 = a0, a4 = a0, a5 = a0, a6 = a0;
+		if(has) {
+			if(c == kNeedRoot) {
+				const int rr = root - 1;
+				a0 = bvh.roots[2 * rr];
+				a1 = bvh.roots[2 * rr + 1];
+			} else if(c >= 0) {
+				const float4 * nd = bvh.wide + 8 * size_t(c);
+				a0 = nd[0];
+				a1 = nd[1];
+				a2 = nd[2];
+				a3 = nd[3];
+				a4 = nd[4];
+				a5 = nd[5];
+				a6 = nd[6];
+			} else {
+				const int ti = ((~c) >> 1) + kk;
+				a0 = bvh.tris[3 * ti + 0];
+				a1 = bvh.tris[3 * ti + 1];
+				a2 = bvh.tris[3 * ti + 2];
+			}
+		}
+		if(!has)
+			continue;
+		// Process.
+		const bool exact = !__all(clean);
+		bool next = false; // pop, or go to the next root, or finish
+		if(c == kNeedRoot) {
+			--root;
+			float e;
+			const bool hitRoot = exact ? slab6<true>(r, a0.x, a0.y, a0.z, a1.x, a1.y, a1.z, r.mini, maxi, e)
+									   : slab6<false>(r, a0.x, a0.y, a0.z, a1.x, a1.y, a1.z, r.mini, maxi, e);
+			if(hitRoot) {
+				c = __float_as_int(a1.w);
+				st.sp = 0;
+				kk = 0;
+			} else if(root == 0) {
+				next = true;
+			}
+		} else if(c >= 0) {
+			if(STATS)
+				++nodeVisits;
+			int c0 = __float_as_int(a6.x), c1 = __float_as_int(a6.y), c2 = __float_as_int(a6.z), c3 = __float_as_int(a6.w);
+			float e0, e1, e2 = 0.0f, e3 = 0.0f;
+			bool h0, h1, h2, h3;
+			if(exact) {
+				h0 = slab6<true>(r, a0.x, a2.x, a4.x, a1.x, a3.x, a5.x, r.mini, maxi, e0);
+				h1 = slab6<true>(r, a0.y, a2.y, a4.y, a1.y, a3.y, a5.y, r.mini, maxi, e1);
+				h2 = (c2 != kWideEmpty) && slab6<true>(r, a0.z, a2.z, a4.z, a1.z, a3.z, a5.z, r.mini, maxi, e2);
+				h3 = (c3 != kWideEmpty) && slab6<true>(r, a0.w, a2.w, a4.w, a1.w, a3.w, a5.w, r.mini, maxi, e3);
+			} else {
+				h0 = slab6<false>(r, a0.x, a2.x, a4.x, a1.x, a3.x, a5.x, r.mini, maxi, e0);
+				h1 = slab6<false>(r, a0.y, a2.y, a4.y, a1.y, a3.y, a5.y, r.mini, maxi, e1);
+				h2 = (c2 != kWideEmpty) && slab6<false>(r, a0.z, a2.z, a4.z, a1.z, a3.z, a5.z, r.mini, maxi, e2);
+				h3 = (c3 != kWideEmpty) && slab6<false>(r, a0.w, a2.w, a4.w, a1.w, a3.w, a5.w, r.mini, maxi, e3);
+			}
+			const int nh = int(h0) + int(h1) + int(h2) + int(h3);
+			if(nh == 0) {
+				next = true;
+			} else {
+				e0 = h0 ? __builtin_fminf(e0, FLT_MAX) : __builtin_inff();
+				e1 = h1 ? __builtin_fminf(e1, FLT_MAX) : __builtin_inff();
+				e2 = h2 ? __builtin_fminf(e2, FLT_MAX) : __builtin_inff();
+				e3 = h3 ? __builtin_fminf(e3, FLT_MAX) : __builtin_inff();
+				cswap(e0, c0, e1, c1);
+				cswap(e2, c2, e3, c3);
+				cswap(e0, c0, e2, c2);
+				cswap(e1, c1, e3, c3);
+				cswap(e1, c1, e2, c2);
+				if(nh > 3)
+					st.push<SPILL>(c3);
+				if(nh > 2)
+					st.push<SPILL>(c2);
+				if(nh > 1)
+					st.push<SPILL>(c1);
+				c = c0;
+				kk = 0;
+			}
+		} else {
+			if(STATS)
+				++triTests;
+			const int code = ~c;
+			const int ti = (code >> 1) + kk;
+			float tt, uu, vv;
+			if(mollerTrumbore(r, xyz(a0), mk(a0.w, a1.x, a1.y), mk(a1.z, a1.w, a2.x), r.mini, maxi, tt, uu, vv)) {
+				best = ti;
+				bt = tt;
+				bu = uu;
+				bv = vv;
+				maxi = tt;
+			}
+			if(ANY && best >= 0) {
+				st.sp = 0;
+				root = 0;
+				next = true;
+			} else if(kk < (code & 1)) {
+				++kk;
+			} else {
+				next = true;
+			}
+		}
+		if(next) {
+			if(st.sp > 0) {
+				c = st.pop<SPILL>();
+				kk = 0;
+			} else if(root > 0) {
+				c = kNeedRoot;
+			} else {
+				// Done: write the result at the ray's own index.
+				if(ANY) {
+					occ[ri] = best >= 0 ? 1 : 0;
+				} else {
+					rdu_hit h;
+					if(best >= 0) {
+						const float4 t2 = bvh.tris[3 * best + 2];
+						h.t = bt;
+						h.u = bu;
+						h.v = bv;
+						h.hit = 1u;
+						h.mesh_id = __float_as_uint(t2.y);
+						h.local_id = __float_as_uint(t2.z);
+					} else {
+						h.t = FLT_MAX;
+						h.u = 0.0f;
+						h.v = 0.0f;
+						h.hit = 0u;
+						h.mesh_id = 0u;
+						h.local_id = 0u;
+					}
+					hits[ri] = h;
+				}
+				has = false;
+			}
+		}
+	}
+	if(STATS) {
+		const uint64_t a = waveSum(nodeVisits);
+		const uint64_t b = waveSum(triTests);
+		if((threadIdx.x & 63) == 0) {
+			atomicAdd(reinterpret_cast<unsigned long long *>(stats), (unsigned long long)a);
+			atomicAdd(reinterpret_cast<unsigned long long *>(stats + 1), (unsigned long long)b);
+		}
+	}
+}
+
+template<bool ANY, bool SPILL, bool STATS>
+PersistentGrid persistentGrid(size_t n) {
+	static int blocksPerCu = -1, cus = -1;
+	if(blocksPerCu < 0) {
+		int dev = 0;
+		(void)hipGetDevice(&dev);
+		(void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+		(void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocksPerCu, tracePersistent<ANY, SPILL, STATS>, kBlock, 0);
+		if(blocksPerCu < 1)
+			blocksPerCu = 1;
+		if(cus < 1)
+			cus = 1;
+	}
+	PersistentGrid g;
+	const size_t waves = size_t(blocksPerCu) * size_t(cus) * (kBlock / 64);
+	// Shares of at least 64 rays; no more waves than the batch needs.
+	size_t share = (n + waves - 1) / waves;
+	share = share < 64 ? 64 : share;
+	const size_t neededWaves = (n + share - 1) / share;
+	g.blocks = unsigned((neededWaves * 64 + kBlock - 1) / kBlock);
+	g.share = uint32_t(share);
+	return g;
 }
 
 // Morton code of the origin's cell in the scene box, kOrderBits per axis.

@@ -111,12 +111,14 @@ def test_near_first_closest(mixed, mixed_rays):
     compare_near_first(got, ref.intersect(mixed_rays, "closest"))
 
 
-@pytest.mark.parametrize("layout", ["wide", "binary"])
+@pytest.mark.parametrize("layout", ["wide", "wide-simple", "binary"])
 @pytest.mark.parametrize("incoherent", [False, True])
 def test_near_first_layouts_and_ordering(mixed, mixed_rays, monkeypatch, layout, incoherent):
     ours, ref = mixed
     if layout == "binary":
         monkeypatch.setenv("RDU_TRACE_LAYOUT", "binary")
+    if layout == "wide-simple":
+        monkeypatch.setenv("RDU_TRACE_KERNEL", "simple")
     log_case(f"near-first layout={layout} incoherent={incoherent}")
     exp = ref.intersect(mixed_rays, "closest")
     compare_near_first(ours.intersect_batch(mixed_rays, incoherent=incoherent), exp)
