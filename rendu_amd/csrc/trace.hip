@@ -335,12 +335,21 @@ __global__ __launch_bounds__(kBlock) void traceKernel(DevBVH bvh, const float4 *
 template<bool ANY, bool SPILL, bool STATS>
 __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
 	const uint32_t * __restrict__ perm, rdu_hit * __restrict__ hits, uint8_t * __restrict__ occ,
-	int * __restrict__ gstack, int lds, uint32_t share, uint64_t * stats);
+	int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint64_t * stats);
+
+// MI355X (gfx950) has 8 XCDs; workgroups are dispatched to them round-robin.
+constexpr uint32_t kXcds = 8;
 
 struct PersistentGrid {
 	unsigned blocks = 0;
-	uint32_t share = 0;
 };
+
+// Idle lanes needed before a wavefront refills (RDU_REFILL_MIN overrides, 1..64).
+uint32_t refillMin() {
+	const char * v = std::getenv("RDU_REFILL_MIN");
+	const int k = v ? std::atoi(v) : 16;
+	return (k >= 1 && k <= 64) ? uint32_t(k) : 16u;
+}
 
 template<bool ANY, bool SPILL, bool STATS>
 PersistentGrid persistentGrid(size_t n);
@@ -375,7 +384,7 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 	{                                                                                                            \
 		const PersistentGrid g = persistentGrid<ANY, S, ST>(n);                                                  \
 		hipLaunchKernelGGL((tracePersistent<ANY, S, ST>), dim3(g.blocks), dim3(kBlock), 0, stream, bvh, rays,      \
-			uint32_t(n), perm, hits, occ, gstack, lds, g.share, stats);                                          \
+			uint32_t(n), perm, hits, occ, gstack, lds, kXcds, refillMin(), stats);                               \
 	}
 		if(stats) {
 			if(spill) RDU_PERSIST(true, true) else RDU_PERSIST(false, true)
@@ -414,40 +423,80 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 // loop trip can pay two dependent memory round trips (a node fetch, then a triangle fetch by
 // other lanes). Here every lane is a small state machine that advances by exactly one step per
 // trip — one wide node, one triangle, or one root box — so a trip issues at most one fetch per
-// lane and waits once; lanes whose ray is finished take the next ray of their wavefront's share
-// of the batch on the next trip, so the wavefront stays full until its share is exhausted.
-// Wavefront w owns rays [w * share, (w + 1) * share) of the (optionally permuted) batch; it
-// exits when its share is done, so every wave of the grid terminates.
+// lane and waits once. A lane whose ray is finished keeps its result until the wavefront
+// refills: once at least `refillMin` lanes are idle (or all are), the idle lanes write their
+// results and take the next rays of the wavefront's sequence together, so ray set-up and result
+// stores run for many lanes at a time instead of one lane per trip.
+//
+// Ray assignment follows the XCD layout: workgroups are dispatched round-robin over the `xcds`
+// dies, so die k runs blocks k, k + xcds, ...; die k owns the k-th contiguous slice of the
+// (optionally Morton-sorted) batch, and its W wavefronts take that slice in interleaved batches
+// of 64 (wavefront v takes batches v, v + W, v + 2W, ...). At any moment a die's wavefronts
+// work on one window of about 64 W neighbouring rays, which keeps the part of the tree they
+// touch small enough for that die's own L2. Every wavefront exits when its sequence is done.
 constexpr int kNeedRoot = 0x7ffffffe; // state: test the next root box
 
 template<bool ANY, bool SPILL, bool STATS>
 __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
 	const uint32_t * __restrict__ perm, rdu_hit * __restrict__ hits, uint8_t * __restrict__ occ,
-	int * __restrict__ gstack, int lds, uint32_t share, uint64_t * stats) {
+	int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint64_t * stats) {
 	__shared__ int ls[kWideLds][kBlock];
 	const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
-	const uint32_t wave = gtid >> 6;
-	const uint64_t begin64 = uint64_t(wave) * share;
-	uint32_t cursor = begin64 < n ? uint32_t(begin64) : n;
-	const uint32_t end = begin64 + share < n ? uint32_t(begin64 + share) : n;
+	const uint32_t die = blockIdx.x % xcds;
+	const uint64_t W = uint64_t(gridDim.x / xcds) * (kBlock / 64);
+	const uint64_t v = uint64_t(blockIdx.x / xcds) * (kBlock / 64) + (threadIdx.x >> 6);
+	const uint64_t sliceBegin = uint64_t(n) * die / xcds;
+	const uint64_t sliceEnd = uint64_t(n) * (die + 1) / xcds;
+	uint64_t cursor = 0; // position in this wavefront's sequence (wave-uniform)
 	LaneStack st{ls, gstack, size_t(gridDim.x) * kBlock, gtid, lds};
 	uint64_t nodeVisits = 0, triTests = 0;
 
-	bool has = false;
+	bool has = false, pend = false;
 	uint32_t ri = 0;
 	DevRay r = makeRay(mk(0, 0, 0), mk(0, 0, 1), 0.0f, 0.0f);
 	bool clean = true;
 	int c = 0, root = 0, kk = 0;
 	float maxi = 0.0f, bt = FLT_MAX, bu = 0.0f, bv = 0.0f;
 	int best = -1;
+	auto writeResult = [&]() {
+		if(ANY) {
+			occ[ri] = best >= 0 ? 1 : 0;
+		} else {
+			rdu_hit h;
+			if(best >= 0) {
+				const float4 t2 = bvh.tris[3 * best + 2];
+				h.t = bt;
+				h.u = bu;
+				h.v = bv;
+				h.hit = 1u;
+				h.mesh_id = __float_as_uint(t2.y);
+				h.local_id = __float_as_uint(t2.z);
+			} else {
+				h.t = FLT_MAX;
+				h.u = 0.0f;
+				h.v = 0.0f;
+				h.hit = 0u;
+				h.mesh_id = 0u;
+				h.local_id = 0u;
+			}
+			hits[ri] = h;
+		}
+	};
 	while(true) {
-		// Refill idle lanes from this wavefront's share (wave-uniform cursor, no atomics).
+		// Refill idle lanes from this wavefront's sequence (wave-uniform cursor, no atomics).
 		const uint64_t idle = __ballot(!has);
-		if(idle) {
+		const uint32_t nIdle = uint32_t(__popcll(idle));
+		if(nIdle >= refillMin || nIdle == 64u) {
 			if(!has) {
+				if(pend) {
+					writeResult();
+					pend = false;
+				}
 				const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(idle >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0u));
-				const uint32_t k = cursor + rank;
-				if(k < end) {
+				const uint64_t j = cursor + rank;
+				const uint64_t kpos = sliceBegin + (((j >> 6) * W + v) << 6) + (j & 63);
+				if(kpos < sliceEnd) {
+					const uint32_t k = uint32_t(kpos);
 					ri = perm ? perm[k] : k;
 					const float4 ra = rays[2 * size_t(ri)];
 					const float4 rb = rays[2 * size_t(ri) + 1];
@@ -464,8 +513,7 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 					has = true;
 				}
 			}
-			const uint32_t take = uint32_t(__popcll(idle));
-			cursor = (end - cursor > take) ? cursor + take : end;
+			cursor += nIdle;
 		}
 		if(!__any(has))
 			break;
@@ -579,33 +627,14 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 			} else if(root > 0) {
 				c = kNeedRoot;
 			} else {
-				// Done: write the result at the ray's own index.
-				if(ANY) {
-					occ[ri] = best >= 0 ? 1 : 0;
-				} else {
-					rdu_hit h;
-					if(best >= 0) {
-						const float4 t2 = bvh.tris[3 * best + 2];
-						h.t = bt;
-						h.u = bu;
-						h.v = bv;
-						h.hit = 1u;
-						h.mesh_id = __float_as_uint(t2.y);
-						h.local_id = __float_as_uint(t2.z);
-					} else {
-						h.t = FLT_MAX;
-						h.u = 0.0f;
-						h.v = 0.0f;
-						h.hit = 0u;
-						h.mesh_id = 0u;
-						h.local_id = 0u;
-					}
-					hits[ri] = h;
-				}
+				// Done: the result is written at the ray's own index at the next refill.
 				has = false;
+				pend = true;
 			}
 		}
 	}
+	if(pend)
+		writeResult();
 	if(STATS) {
 		const uint64_t a = waveSum(nodeVisits);
 		const uint64_t b = waveSum(triTests);
@@ -630,13 +659,13 @@ PersistentGrid persistentGrid(size_t n) {
 			cus = 1;
 	}
 	PersistentGrid g;
-	const size_t waves = size_t(blocksPerCu) * size_t(cus) * (kBlock / 64);
-	// Shares of at least 64 rays; no more waves than the batch needs.
-	size_t share = (n + waves - 1) / waves;
-	share = share < 64 ? 64 : share;
-	const size_t neededWaves = (n + share - 1) / share;
-	g.blocks = unsigned((neededWaves * 64 + kBlock - 1) / kBlock);
-	g.share = uint32_t(share);
+	// One resident grid, trimmed for small batches (about one 64-ray batch per wavefront),
+	// always a whole number of blocks per die.
+	size_t blocks = size_t(blocksPerCu) * size_t(cus);
+	const size_t needed = (n + kBlock - 1) / kBlock;
+	blocks = needed < blocks ? needed : blocks;
+	blocks = ((blocks + kXcds - 1) / kXcds) * kXcds;
+	g.blocks = unsigned(blocks);
 	return g;
 }
 
