@@ -36,7 +36,7 @@ namespace rdu {
 namespace {
 
 constexpr int kBlock = 128;
-constexpr int kWideLds = 24;
+constexpr int kWideLds = 16;
 
 __device__ __forceinline__ f3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
 
@@ -182,6 +182,20 @@ __device__ __forceinline__ void traverseBinary(const DevBVH & bvh, const DevRay 
 			return;
 		c = st.pop<SPILL>();
 	}
+}
+
+// fp16 value not above e (round towards -inf; beyond the fp16 range: the largest finite half).
+__device__ __forceinline__ _Float16 roundDownHalf(float e) {
+	const float big = 65504.0f;
+	const float ec = e > big ? big : e;
+	_Float16 h = _Float16(ec);
+	if(float(h) > ec) {
+		// one fp16 ulp down (ec >= 0 here in practice; the bit step works for both signs)
+		unsigned short b = __builtin_bit_cast(unsigned short, h);
+		b = (b & 0x8000u) ? (unsigned short)(b + 1u) : (b == 0 ? (unsigned short)0x8001u : (unsigned short)(b - 1u));
+		h = __builtin_bit_cast(_Float16, b);
+	}
+	return h;
 }
 
 __device__ __forceinline__ void cswap(float & a, int & ca, float & b, int & cb) {
@@ -335,7 +349,7 @@ __global__ __launch_bounds__(kBlock) void traceKernel(DevBVH bvh, const float4 *
 template<bool ANY, bool SPILL, bool STATS>
 __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
 	const uint32_t * __restrict__ perm, rdu_hit * __restrict__ hits, uint8_t * __restrict__ occ,
-	int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint64_t * stats);
+	int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint32_t chunk, uint64_t * stats);
 
 // MI355X (gfx950) has 8 XCDs; workgroups are dispatched to them round-robin.
 constexpr uint32_t kXcds = 8;
@@ -383,8 +397,13 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 #define RDU_PERSIST(S, ST)                                                                                       \
 	{                                                                                                            \
 		const PersistentGrid g = persistentGrid<ANY, S, ST>(n);                                                  \
+		const uint32_t waves = (g.blocks / kXcds) * (kBlock / 64);                                               \
+		const uint64_t slice = (uint64_t(n) + kXcds - 1) / kXcds;                                                \
+		/* sorted batches: interleaved 64-ray chunks; input order: one contiguous run per wavefront */          \
+		uint64_t chunk = perm ? 64 : (slice + waves - 1) / (waves ? waves : 1);                                 \
+		chunk = chunk < 64 ? 64 : chunk;                                                                        \
 		hipLaunchKernelGGL((tracePersistent<ANY, S, ST>), dim3(g.blocks), dim3(kBlock), 0, stream, bvh, rays,      \
-			uint32_t(n), perm, hits, occ, gstack, lds, kXcds, refillMin(), stats);                               \
+			uint32_t(n), perm, hits, occ, gstack, lds, kXcds, refillMin(), uint32_t(chunk), stats);              \
 	}
 		if(stats) {
 			if(spill) RDU_PERSIST(true, true) else RDU_PERSIST(false, true)
@@ -436,11 +455,16 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 // touch small enough for that die's own L2. Every wavefront exits when its sequence is done.
 constexpr int kNeedRoot = 0x7ffffffe; // state: test the next root box
 
+// Closest-hit queries also keep each pushed child's slab entry in LDS (fp16 rounded towards -inf,
+// so never above the true entry): a popped child whose entry already exceeds the current maxi
+// is dropped without being fetched, exactly as its own box test would drop it (the test fails
+// once entry > maxi). Levels spilled to global memory carry no entry and are always fetched.
 template<bool ANY, bool SPILL, bool STATS>
 __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
 	const uint32_t * __restrict__ perm, rdu_hit * __restrict__ hits, uint8_t * __restrict__ occ,
-	int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint64_t * stats) {
+	int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint32_t chunk, uint64_t * stats) {
 	__shared__ int ls[kWideLds][kBlock];
+	__shared__ _Float16 le[ANY ? 1 : kWideLds][kBlock];
 	const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
 	const uint32_t die = blockIdx.x % xcds;
 	const uint64_t W = uint64_t(gridDim.x / xcds) * (kBlock / 64);
@@ -494,7 +518,7 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 				}
 				const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(idle >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0u));
 				const uint64_t j = cursor + rank;
-				const uint64_t kpos = sliceBegin + (((j >> 6) * W + v) << 6) + (j & 63);
+				const uint64_t kpos = sliceBegin + ((j / chunk) * W + v) * chunk + (j % chunk);
 				if(kpos < sliceEnd) {
 					const uint32_t k = uint32_t(kpos);
 					ri = perm ? perm[k] : k;
@@ -588,12 +612,17 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 				cswap(e0, c0, e2, c2);
 				cswap(e1, c1, e3, c3);
 				cswap(e1, c1, e2, c2);
+				auto pushE = [&](int code, float e) {
+					if(!ANY && (!SPILL || st.sp < st.lds))
+						le[st.sp][threadIdx.x] = roundDownHalf(e);
+					st.push<SPILL>(code);
+				};
 				if(nh > 3)
-					st.push<SPILL>(c3);
+					pushE(c3, e3);
 				if(nh > 2)
-					st.push<SPILL>(c2);
+					pushE(c2, e2);
 				if(nh > 1)
-					st.push<SPILL>(c1);
+					pushE(c1, e1);
 				c = c0;
 				kk = 0;
 			}
@@ -619,6 +648,11 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 			} else {
 				next = true;
 			}
+		}
+		if(next && !ANY) {
+			// Drop stacked children whose entry is already beyond the closest hit.
+			while(st.sp > 0 && st.sp <= st.lds && float(le[st.sp - 1][threadIdx.x]) > maxi)
+				--st.sp;
 		}
 		if(next) {
 			if(st.sp > 0) {
