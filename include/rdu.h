@@ -123,6 +123,130 @@ int rdu_raycaster_export_triangles(const rdu_raycaster* rc, uint32_t* mesh_id, u
 int rdu_raycaster_trace_stats(const rdu_raycaster* rc, const rdu_ray* rays, size_t n, int any_hit, uint32_t flags,
 	uint64_t* node_visits, uint64_t* tri_tests);
 
+/* ------------------------------------------------------------------------------------------------
+ * Path tracer: Rendu's PathTracer (src/apps/pathtracer/PathTracer.hpp:10-76) over a scene
+ * description that carries what PathTracer::render reads from Rendu's Scene
+ * (src/engine/scene/Scene.hpp:65-77): objects in order (object i is raycaster mesh i), their
+ * meshes, materials and textures, the analytic lights and the background.
+ * ------------------------------------------------------------------------------------------------ */
+
+/* A float RGBA image (Image with 4 components, row-major, y = 0 first row; Image.hpp:86-102).
+ * Constant-colour textures are 8x8 images of that colour (ResourcesManager.cpp:489-504). */
+typedef struct {
+	uint32_t width, height;
+	const float* rgba; /* width * height * 4 floats */
+} rdu_texture_desc;
+
+/* Mesh fields the path tracer reads (Mesh.hpp:126-134). normals/tangents/bitangents: n_vertices*3
+ * floats (required); texcoords: n_vertices*2 floats or NULL (then the object cannot use UVs). */
+typedef struct {
+	const float* positions;
+	const float* normals;
+	const float* tangents;
+	const float* bitangents;
+	const float* texcoords;
+	const uint32_t* indices;
+	size_t n_vertices, n_indices;
+} rdu_mesh_desc;
+
+/* Material::Type values used by the path tracer (Material.hpp:12-88): every non-emissive type
+ * is shaded as Lambert + GGX; EMISSIVE adds its texture-0 colour and stops the path. */
+#define RDU_MATERIAL_REGULAR 1
+#define RDU_MATERIAL_EMISSIVE 8
+
+typedef struct {
+	int32_t type;       /* RDU_MATERIAL_* */
+	int32_t masked;     /* Material::masked(): alpha cut-out through texture 0's alpha */
+	int32_t textures[3]; /* texture ids: 0 albedo+alpha, 1 normal map, 2 roughness/metal/AO */
+} rdu_material_desc;
+
+typedef struct {
+	uint32_t mesh;          /* mesh id from rdu_scene_add_mesh */
+	uint32_t material;      /* material id */
+	int32_t use_texcoords;  /* Object::useTexCoords() (Object.hpp:69) */
+	float model[16];        /* glm column-major model matrix (Object::model()) */
+} rdu_object_desc;
+
+#define RDU_LIGHT_POINT 0       /* PointLight::sample (PointLight.cpp:68-87) */
+#define RDU_LIGHT_DIRECTIONAL 1 /* DirectionalLight::sample (DirectionalLight.cpp:40-44) */
+#define RDU_LIGHT_SPOT 2        /* SpotLight::sample (SpotLight.cpp:52-81) */
+
+typedef struct {
+	int32_t type;
+	int32_t casts_shadow;
+	float intensity[3];
+	float position[3];  /* point, spot */
+	float direction[3]; /* directional, spot (the direction the light travels) */
+	float radius;       /* point, spot: radius of influence */
+	float angles[2];    /* spot: inner, outer half angles (radians) */
+} rdu_light_desc;
+
+#define RDU_BACKGROUND_COLOR 0      /* Scene::Background::COLOR */
+#define RDU_BACKGROUND_IMAGE 1      /* IMAGE: texture stretched over the screen, direct hits only */
+#define RDU_BACKGROUND_SKYBOX 2     /* SKYBOX: cubemap faces px, nx, py, ny, pz, nz */
+#define RDU_BACKGROUND_ATMOSPHERE 3 /* ATMOSPHERE: MaterialSky::eval with a 512x512 scattering table */
+#define RDU_BACKGROUND_LATLONG 4    /* extension: latitude-longitude environment map */
+
+typedef struct {
+	int32_t mode;
+	float color[3];     /* COLOR */
+	int32_t texture;    /* IMAGE, ATMOSPHERE (scattering table), LATLONG */
+	int32_t cube[6];    /* SKYBOX */
+	float sun_direction[3]; /* ATMOSPHERE */
+} rdu_background_desc;
+
+/* Camera::pose + projection (src/engine/input/Camera.cpp:10-18, 42-49). */
+typedef struct {
+	float position[3], center[3], up[3];
+	float fov;   /* vertical field of view, radians */
+	float ratio; /* width / height */
+} rdu_camera_desc;
+
+#define RDU_ESTIMATOR_REFERENCE 0 /* Rendu's estimator exactly (PathTracer.cpp:235-375) */
+#define RDU_ESTIMATOR_EXTENDED 1  /* adds Russian roulette from rr_start (extension) */
+
+typedef struct {
+	uint32_t seed;       /* key of the counter-based random stream */
+	int32_t estimator;   /* RDU_ESTIMATOR_* */
+	int32_t rr_start;    /* extended: first bounce subject to Russian roulette */
+	uint32_t max_paths;  /* paths in flight per pass (0 = library default) */
+} rdu_render_opts;
+
+typedef struct rdu_scene rdu_scene;
+
+int rdu_scene_create(rdu_scene** out);
+void rdu_scene_destroy(rdu_scene* scene);
+int rdu_scene_add_texture(rdu_scene* scene, const rdu_texture_desc* desc, int32_t* id);
+int rdu_scene_add_mesh(rdu_scene* scene, const rdu_mesh_desc* desc, int32_t* id);
+int rdu_scene_add_material(rdu_scene* scene, const rdu_material_desc* desc, int32_t* id);
+int rdu_scene_add_object(rdu_scene* scene, const rdu_object_desc* desc);
+int rdu_scene_add_light(rdu_scene* scene, const rdu_light_desc* desc);
+int rdu_scene_set_background(rdu_scene* scene, const rdu_background_desc* desc);
+
+/* Replaces PathTracer::PathTracer(scene) (PathTracer.cpp:11-21): adds every object's mesh with its
+ * model to a raycaster (mesh id = object index), builds Rendu's hierarchy and uploads scene data. */
+int rdu_scene_commit(rdu_scene* scene);
+
+/* The raycaster owned by a committed scene (PathTracer::raycaster(), PathTracer.hpp:29). */
+const rdu_raycaster* rdu_scene_raycaster(const rdu_scene* scene);
+
+/* Replaces PathTracer::render(camera, samples, depth, image) (PathTracer.cpp:159-299):
+ * image is the caller's host buffer of width*height*components floats (components >= 3), which
+ * like Rendu's Image is accumulated into (+=) and then divided by the sample count; the caller
+ * zeroes it first. samples is rounded to a power of two as Rendu does. opts may be NULL. */
+int rdu_pathtracer_render(rdu_scene* scene, const rdu_camera_desc* camera, size_t samples, size_t depth,
+	float* image, uint32_t width, uint32_t height, uint32_t components, const rdu_render_opts* opts);
+
+/* Timing and counters of the last render on this scene (no reference counterpart). */
+typedef struct {
+	double seconds;          /* wall time of the render call */
+	uint64_t paths;          /* pixel samples traced */
+	uint64_t extension_rays; /* closest-hit queries */
+	uint64_t shadow_rays;    /* visibility queries */
+	uint32_t samples;        /* spp after rounding */
+} rdu_render_stats;
+int rdu_scene_last_render_stats(const rdu_scene* scene, rdu_render_stats* out);
+
 /* Last error message of the calling thread ("" if none). */
 const char* rdu_last_error(void);
 

@@ -156,3 +156,111 @@ def ray_setup(d):
     dd, inv = np.zeros(3, np.float32), np.zeros(3, np.float32)
     lib.orc_ray_setup(_f(_v3(d)), _f(dd), _f(inv))
     return dd, inv
+
+
+# ---------------------------------------------------------------------------------------------------
+# Path tracer (pathtracer.cpp)
+# ---------------------------------------------------------------------------------------------------
+_pt_sig = {
+    "orc_pt_create": (c_void_p, []),
+    "orc_pt_destroy": (None, [c_void_p]),
+    "orc_pt_add_texture": (c_int, [c_void_p, c_void_p]),
+    "orc_pt_add_mesh": (c_int, [c_void_p, c_void_p]),
+    "orc_pt_add_material": (c_int, [c_void_p, c_void_p]),
+    "orc_pt_add_object": (c_int, [c_void_p, c_void_p]),
+    "orc_pt_add_light": (c_int, [c_void_p, c_void_p]),
+    "orc_pt_set_background": (c_int, [c_void_p, c_void_p]),
+    "orc_pt_commit": (c_int, [c_void_p]),
+    "orc_camera_shifts": (None, [c_void_p, POINTER(c_float), POINTER(c_float), POINTER(c_float)]),
+    "orc_pt_render": (c_int, [c_void_p, c_void_p, c_size_t, c_size_t, POINTER(c_float), ctypes.c_uint32,
+                              ctypes.c_uint32, ctypes.c_uint32, c_void_p, c_int, POINTER(c_uint64)]),
+    "orc_ggx_eval": (None, [POINTER(c_float)] * 2 + [c_float, c_float] + [POINTER(c_float)] * 2),
+    "orc_ggx_sample": (None, [POINTER(c_float)] * 2 + [c_float, c_float] + [POINTER(c_float)] * 3),
+    "orc_ggx_pdf": (None, [POINTER(c_float)] * 2 + [c_float, c_float] + [POINTER(c_float)] * 2),
+    "orc_sample_disk": (None, [c_float, c_float, POINTER(c_float)]),
+    "orc_texture_probe": (None, [c_void_p, c_int, c_int, c_float, c_float, c_float, POINTER(c_float)]),
+    "orc_philox": (None, [ctypes.c_uint32] * 6 + [POINTER(c_uint32)]),
+}
+for _name, (_res, _args) in _pt_sig.items():
+    _fn = getattr(lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+
+def philox4x32_10(counter, key):
+    out = np.zeros(4, np.uint32)
+    lib.orc_philox(*[int(c) for c in counter], *[int(k) for k in key], _u32(out))
+    return out
+
+
+def camera_shifts(cam_desc):
+    c, dx, dy = np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(3, np.float32)
+    lib.orc_camera_shifts(ctypes.byref(cam_desc), _f(c), _f(dx), _f(dy))
+    return c, dx, dy
+
+
+def ggx_eval(wo, base, rough, metal, wi):
+    out = np.zeros(3, np.float32)
+    lib.orc_ggx_eval(_f(_v3(wo)), _f(_v3(base)), rough, metal, _f(_v3(wi)), _f(out))
+    return out
+
+
+def ggx_sample(wo, base, rough, metal, u):
+    wi, w = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    uu = np.ascontiguousarray(np.asarray(u, np.float32).reshape(3))
+    lib.orc_ggx_sample(_f(_v3(wo)), _f(_v3(base)), rough, metal, _f(uu), _f(wi), _f(w))
+    return wi, w
+
+
+def ggx_pdf(wo, base, rough, metal, wi):
+    p = np.zeros(1, np.float32)
+    lib.orc_ggx_pdf(_f(_v3(wo)), _f(_v3(base)), rough, metal, _f(_v3(wi)), _f(p))
+    return float(p[0])
+
+
+def sample_disk(u0, u1):
+    out = np.zeros(2, np.float32)
+    lib.orc_sample_disk(u0, u1, _f(out))
+    return out
+
+
+class OraclePathTracer:
+    """The oracle's PathTracer over a rendu_amd.scene.Scene."""
+
+    def __init__(self, scene):
+        self.h = lib.orc_pt_create()
+        self._keep, tex, meshes, mats, objs, lights, bg = scene.descriptors()
+        for t in tex:
+            lib.orc_pt_add_texture(self.h, ctypes.byref(t))
+        for m in meshes:
+            lib.orc_pt_add_mesh(self.h, ctypes.byref(m))
+        for m in mats:
+            lib.orc_pt_add_material(self.h, ctypes.byref(m))
+        for o in objs:
+            lib.orc_pt_add_object(self.h, ctypes.byref(o))
+        for l in lights:
+            lib.orc_pt_add_light(self.h, ctypes.byref(l))
+        lib.orc_pt_set_background(self.h, ctypes.byref(bg))
+        if lib.orc_pt_commit(self.h):
+            raise RuntimeError("oracle scene commit failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_pt_destroy(self.h)
+            self.h = None
+
+    def render(self, cam_desc, samples, depth, image, opts, threads=0):
+        """image: (h, w, c) float32, accumulated into as Rendu does. Returns (spp, ext rays, shadow rays)."""
+        assert image.dtype == np.float32 and image.flags.c_contiguous
+        h, w, c = image.shape
+        cnt = np.zeros(2, np.uint64)
+        spp = lib.orc_pt_render(self.h, ctypes.byref(cam_desc), samples, depth, _f(image), w, h, c,
+                                ctypes.byref(opts), threads, _u64(cnt))
+        if spp < 0:
+            raise RuntimeError("oracle render refused its arguments")
+        return spp, int(cnt[0]), int(cnt[1])
+
+    def probe(self, tex, mode, x, y, z=0.0):
+        out = np.zeros(4, np.float32)
+        lib.orc_texture_probe(self.h, tex, mode, x, y, z, _f(out))
+        return out

@@ -69,6 +69,19 @@ def _load() -> ctypes.CDLL:
         "rdu_raycaster_export_triangles": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint32)]),
         "rdu_raycaster_trace_stats": (c_int, [c_void_p, c_void_p, c_size_t, c_int, c_uint32,
                                               POINTER(c_uint64), POINTER(c_uint64)]),
+        "rdu_scene_create": (c_int, [POINTER(c_void_p)]),
+        "rdu_scene_destroy": (None, [c_void_p]),
+        "rdu_scene_add_texture": (c_int, [c_void_p, c_void_p, POINTER(ctypes.c_int32)]),
+        "rdu_scene_add_mesh": (c_int, [c_void_p, c_void_p, POINTER(ctypes.c_int32)]),
+        "rdu_scene_add_material": (c_int, [c_void_p, c_void_p, POINTER(ctypes.c_int32)]),
+        "rdu_scene_add_object": (c_int, [c_void_p, c_void_p]),
+        "rdu_scene_add_light": (c_int, [c_void_p, c_void_p]),
+        "rdu_scene_set_background": (c_int, [c_void_p, c_void_p]),
+        "rdu_scene_commit": (c_int, [c_void_p]),
+        "rdu_scene_raycaster": (c_void_p, [c_void_p]),
+        "rdu_pathtracer_render": (c_int, [c_void_p, c_void_p, c_size_t, c_size_t, POINTER(c_float), c_uint32, c_uint32,
+                                          c_uint32, c_void_p]),
+        "rdu_scene_last_render_stats": (c_int, [c_void_p, c_void_p]),
         "rdu_last_error": (c_char_p, []),
         "rdu_version": (c_char_p, []),
     }

@@ -26,6 +26,7 @@ class Camera:
         self.position = np.array([0.0, 0.0, 1.0], F32)
         self.center = np.array([0.0, 0.0, 0.0], F32)
         self.up = np.array([0.0, 1.0, 0.0], F32)
+        self.up_in = np.array([0.0, 1.0, 0.0], F32)
         self.right = np.array([1.0, 0.0, 0.0], F32)
         self.fov = F32(1.3)  # Camera.hpp:172
         self.ratio = F32(1.0)  # Camera.hpp:173
@@ -34,6 +35,7 @@ class Camera:
         """Camera::pose (Camera.cpp:10-18)."""
         self.position = np.asarray(position, F32).copy()
         self.center = np.asarray(center, F32).copy()
+        self.up_in = np.asarray(up, F32).copy()  # the pose input, handed to the library as given
         u = _normalize(up)
         view = _normalize(self.center - self.position)
         self.right = _cross(view, u)

@@ -2,30 +2,13 @@
 // (src/engine/raycaster/Raycaster.hpp:10-157). Host-side bookkeeping only; all queries run on
 // the GPU through trace.hip. There is no CPU query path: without a HIP device the queries fail
 // with RDU_ERR_HIP.
-#include "capi_common.hpp"
-#include "bvh.hpp"
-#include "trace.hpp"
-#include "../../include/rdu.h"
+#include "raycaster_impl.hpp"
 
 #include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <mutex>
 #include <vector>
-
-struct rdu_raycaster {
-	rdu::HostScene scene;
-	rdu::DeviceLayout layout;
-	rdu::DevBVH dev;
-	rdu::DevBuffer nodes, tris, roots;
-	bool uploaded = false;
-	int device = -1;
-	// Staging for host-pointer queries and global stack spill; guarded by `lock`.
-	mutable std::mutex lock;
-	mutable rdu::DevBuffer stageIn, stageOut, gstack, stats;
-	mutable rdu::DevBuffer keys, keysAlt, perm, permAlt, sortTemp; // incoherent-batch ordering
-	rdu::DevBuffer wide;
-};
 
 using namespace rdu;
 
@@ -115,6 +98,7 @@ int query(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, void * out, 
 				"ray ordering")))
 			return st;
 		opt.perm = static_cast<const uint32_t *>(rc->perm.ptr);
+		opt.spatialOrder = true;
 	}
 	const int e = ANY ? launchAny(rc->dev, dRays, n, static_cast<uint8_t *>(dOut), opt, static_cast<int *>(rc->gstack.ptr),
 							  dstats, s)

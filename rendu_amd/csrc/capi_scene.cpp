@@ -1,0 +1,402 @@
+// C ABI of the path tracer (include/rdu.h): the drop-in boundary for Rendu's PathTracer
+// (src/apps/pathtracer/PathTracer.hpp:10-76). Host bookkeeping only; rendering runs in pt.hip.
+#include "pt.hpp"
+#include "raycaster_impl.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+struct rdu_scene {
+	struct Tex {
+		uint32_t w, h, offset;
+	};
+	struct MeshRec {
+		uint32_t vtx, idx, nv, ni;
+		bool hasUV;
+	};
+	std::vector<float> texels; // RGBA
+	std::vector<Tex> textures;
+	std::vector<float> positions, nrm, tan, bit, uv;
+	std::vector<uint32_t> indices;
+	std::vector<MeshRec> meshes;
+	std::vector<rdu_material_desc> materials;
+	std::vector<rdu_object_desc> objects;
+	std::vector<rdu_light_desc> lights;
+	rdu_background_desc bg{};
+	bool committed = false;
+	rdu_raycaster * rc = nullptr;
+	rdu::DevScene dev;
+	rdu::DevBuffer dTexels, dTextures, dNrm, dTan, dBit, dUv, dIdx, dMeshes, dObjects, dMaterials, dLights;
+	// Path buffers, grown on demand.
+	rdu::DevBuffer pRays, pHits, pAtten, pColor, pQ0, pQ1, pCounts, pTotals, pSRays, pSContrib, pSQ0, pSQ1, pOcc,
+		pSHits, pGstack, pImage;
+	size_t pathCap = 0;
+	rdu_render_stats last{};
+	~rdu_scene() {
+		if(rc)
+			rdu_raycaster_destroy(rc);
+	}
+};
+
+using namespace rdu;
+
+namespace {
+
+// glm inverse(mat3) (func_matrix.inl:269-291) of transpose(mat3(model)), column-major out.
+void inverseTransposeUpper(const float * m4, float * out) {
+	float m[3][3]; // m[c][r] = transpose(mat3(model))[c][r] = model[r][c]
+	for(int c = 0; c < 3; ++c)
+		for(int r = 0; r < 3; ++r)
+			m[c][r] = m4[4 * r + c];
+	const float ood = 1.0f / (+m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) - m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2]) +
+								 m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]));
+	float inv[3][3];
+	inv[0][0] = +(m[1][1] * m[2][2] - m[2][1] * m[1][2]) * ood;
+	inv[1][0] = -(m[1][0] * m[2][2] - m[2][0] * m[1][2]) * ood;
+	inv[2][0] = +(m[1][0] * m[2][1] - m[2][0] * m[1][1]) * ood;
+	inv[0][1] = -(m[0][1] * m[2][2] - m[2][1] * m[0][2]) * ood;
+	inv[1][1] = +(m[0][0] * m[2][2] - m[2][0] * m[0][2]) * ood;
+	inv[2][1] = -(m[0][0] * m[2][1] - m[2][0] * m[0][1]) * ood;
+	inv[0][2] = +(m[0][1] * m[1][2] - m[1][1] * m[0][2]) * ood;
+	inv[1][2] = -(m[0][0] * m[1][2] - m[1][0] * m[0][2]) * ood;
+	inv[2][2] = +(m[0][0] * m[1][1] - m[1][0] * m[0][1]) * ood;
+	for(int c = 0; c < 3; ++c)
+		for(int r = 0; r < 3; ++r)
+			out[3 * c + r] = inv[c][r];
+}
+
+bool texOk(const rdu_scene * s, int32_t t) { return t >= 0 && size_t(t) < s->textures.size(); }
+
+template<typename T>
+int uploadVec(DevBuffer & b, const std::vector<T> & v) {
+	return b.upload(v.data(), v.size() * sizeof(T));
+}
+
+int ensurePaths(rdu_scene * s, size_t P) {
+	if(P <= s->pathCap)
+		return RDU_OK;
+	int st;
+	if((st = s->pRays.reserve(P * 32)) || (st = s->pHits.reserve(P * sizeof(rdu_hit))) || (st = s->pAtten.reserve(P * 16)) ||
+		(st = s->pColor.reserve(P * 16)) || (st = s->pQ0.reserve(P * 4)) || (st = s->pQ1.reserve(P * 4)) ||
+		(st = s->pCounts.reserve(kCountSlots * 4)) || (st = s->pTotals.reserve(2 * 8)) || (st = s->pSRays.reserve(P * 32)) ||
+		(st = s->pSContrib.reserve(P * 16)) || (st = s->pSQ0.reserve(P * 4)) || (st = s->pSQ1.reserve(P * 4)) ||
+		(st = s->pOcc.reserve(P)) || (st = s->pSHits.reserve(P * sizeof(rdu_hit))))
+		return st;
+	TraceOptions o;
+	o.wide = true;
+	const size_t g = globalStackInts(s->rc->dev, o, P);
+	if(g && (st = s->pGstack.reserve(g * sizeof(int))))
+		return st;
+	s->pathCap = P;
+	return RDU_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int rdu_scene_create(rdu_scene ** out) {
+	clearError();
+	if(!out)
+		return fail(RDU_ERR_INVALID, "null output handle");
+	*out = new(std::nothrow) rdu_scene();
+	return *out ? RDU_OK : fail(RDU_ERR_ALLOC, "host allocation failed");
+}
+
+void rdu_scene_destroy(rdu_scene * s) { delete s; }
+
+int rdu_scene_add_texture(rdu_scene * s, const rdu_texture_desc * d, int32_t * id) {
+	clearError();
+	if(!s || !d || !id || (!d->rgba && d->width * d->height))
+		return fail(RDU_ERR_INVALID, "null argument");
+	if(s->committed)
+		return fail(RDU_ERR_STATE, "scene already committed");
+	if(d->width == 0 || d->height == 0)
+		return fail(RDU_ERR_INVALID, "empty texture");
+	rdu_scene::Tex t{d->width, d->height, uint32_t(s->texels.size() / 4)};
+	s->texels.insert(s->texels.end(), d->rgba, d->rgba + size_t(d->width) * d->height * 4);
+	s->textures.push_back(t);
+	*id = int32_t(s->textures.size() - 1);
+	return RDU_OK;
+}
+
+int rdu_scene_add_mesh(rdu_scene * s, const rdu_mesh_desc * d, int32_t * id) {
+	clearError();
+	if(!s || !d || !id)
+		return fail(RDU_ERR_INVALID, "null argument");
+	if(s->committed)
+		return fail(RDU_ERR_STATE, "scene already committed");
+	if(!d->positions || !d->normals || !d->tangents || !d->bitangents || !d->indices)
+		return fail(RDU_ERR_INVALID, "the path tracer needs positions, normals, tangent frames and indices");
+	if(d->n_indices % 3 != 0 || d->n_indices == 0)
+		return fail(RDU_ERR_INVALID, "index count must be a positive multiple of 3");
+	for(size_t i = 0; i < d->n_indices; ++i)
+		if(d->indices[i] >= d->n_vertices)
+			return fail(RDU_ERR_INVALID, "index out of range of the mesh vertices");
+	rdu_scene::MeshRec m{uint32_t(s->nrm.size() / 3), uint32_t(s->indices.size()), uint32_t(d->n_vertices),
+		uint32_t(d->n_indices), d->texcoords != nullptr};
+	const size_t nv = d->n_vertices;
+	s->positions.insert(s->positions.end(), d->positions, d->positions + 3 * nv);
+	s->nrm.insert(s->nrm.end(), d->normals, d->normals + 3 * nv);
+	s->tan.insert(s->tan.end(), d->tangents, d->tangents + 3 * nv);
+	s->bit.insert(s->bit.end(), d->bitangents, d->bitangents + 3 * nv);
+	if(d->texcoords)
+		s->uv.insert(s->uv.end(), d->texcoords, d->texcoords + 2 * nv);
+	else
+		s->uv.insert(s->uv.end(), 2 * nv, 0.0f);
+	s->indices.insert(s->indices.end(), d->indices, d->indices + d->n_indices);
+	s->meshes.push_back(m);
+	*id = int32_t(s->meshes.size() - 1);
+	return RDU_OK;
+}
+
+int rdu_scene_add_material(rdu_scene * s, const rdu_material_desc * d, int32_t * id) {
+	clearError();
+	if(!s || !d || !id)
+		return fail(RDU_ERR_INVALID, "null argument");
+	if(s->committed)
+		return fail(RDU_ERR_STATE, "scene already committed");
+	for(int k = 0; k < 3; ++k)
+		if(!texOk(s, d->textures[k]))
+			return fail(RDU_ERR_INVALID, "material texture id out of range");
+	s->materials.push_back(*d);
+	*id = int32_t(s->materials.size() - 1);
+	return RDU_OK;
+}
+
+int rdu_scene_add_object(rdu_scene * s, const rdu_object_desc * d) {
+	clearError();
+	if(!s || !d)
+		return fail(RDU_ERR_INVALID, "null argument");
+	if(s->committed)
+		return fail(RDU_ERR_STATE, "scene already committed");
+	if(d->mesh >= s->meshes.size() || d->material >= s->materials.size())
+		return fail(RDU_ERR_INVALID, "object mesh or material id out of range");
+	s->objects.push_back(*d);
+	return RDU_OK;
+}
+
+int rdu_scene_add_light(rdu_scene * s, const rdu_light_desc * d) {
+	clearError();
+	if(!s || !d)
+		return fail(RDU_ERR_INVALID, "null argument");
+	if(s->committed)
+		return fail(RDU_ERR_STATE, "scene already committed");
+	if(d->type < RDU_LIGHT_POINT || d->type > RDU_LIGHT_SPOT)
+		return fail(RDU_ERR_INVALID, "unknown light type");
+	s->lights.push_back(*d);
+	return RDU_OK;
+}
+
+int rdu_scene_set_background(rdu_scene * s, const rdu_background_desc * d) {
+	clearError();
+	if(!s || !d)
+		return fail(RDU_ERR_INVALID, "null argument");
+	if(s->committed)
+		return fail(RDU_ERR_STATE, "scene already committed");
+	s->bg = *d;
+	return RDU_OK;
+}
+
+int rdu_scene_commit(rdu_scene * s) {
+	clearError();
+	if(!s)
+		return fail(RDU_ERR_INVALID, "null scene");
+	if(s->committed)
+		return fail(RDU_ERR_STATE, "scene already committed");
+	if(s->objects.empty())
+		return fail(RDU_ERR_INVALID, "scene without objects");
+	const rdu_background_desc & b = s->bg;
+	if(b.mode < RDU_BACKGROUND_COLOR || b.mode > RDU_BACKGROUND_LATLONG)
+		return fail(RDU_ERR_INVALID, "unknown background mode");
+	if((b.mode == RDU_BACKGROUND_IMAGE || b.mode == RDU_BACKGROUND_ATMOSPHERE || b.mode == RDU_BACKGROUND_LATLONG) &&
+		!texOk(s, b.texture))
+		return fail(RDU_ERR_INVALID, "background texture id out of range");
+	if(b.mode == RDU_BACKGROUND_SKYBOX)
+		for(int k = 0; k < 6; ++k)
+			if(!texOk(s, b.cube[k]))
+				return fail(RDU_ERR_INVALID, "cubemap face id out of range");
+	int st = rdu_raycaster_create(&s->rc);
+	if(st)
+		return st;
+	// PathTracer::PathTracer: every object's mesh, with its model, in object order.
+	for(const rdu_object_desc & o : s->objects) {
+		const rdu_scene::MeshRec & m = s->meshes[o.mesh];
+		if((st = rdu_raycaster_add_mesh(s->rc, s->positions.data() + 3 * size_t(m.vtx), m.nv, s->indices.data() + m.idx, m.ni,
+				o.model)))
+			return st;
+	}
+	if((st = rdu_raycaster_update_hierarchy(s->rc)))
+		return st;
+	std::vector<DevTexture> tex;
+	for(const auto & t : s->textures)
+		tex.push_back({t.offset, int32_t(t.w), int32_t(t.h)});
+	std::vector<DevMesh> meshes;
+	for(const auto & m : s->meshes)
+		meshes.push_back({m.vtx, m.idx, m.hasUV ? 1 : 0});
+	std::vector<DevObject> objs;
+	for(const auto & o : s->objects) {
+		DevObject d;
+		d.mesh = o.mesh;
+		d.mat = o.material;
+		d.useUV = (o.use_texcoords && s->meshes[o.mesh].hasUV) ? 1 : 0; // Object.cpp:42
+		inverseTransposeUpper(o.model, d.invtp);
+		objs.push_back(d);
+	}
+	std::vector<DevMaterial> mats;
+	int anyMasked = 0;
+	for(const auto & m : s->materials) {
+		mats.push_back({m.type, m.masked, {m.textures[0], m.textures[1], m.textures[2]}});
+		anyMasked |= m.masked ? 1 : 0;
+	}
+	if((st = uploadVec(s->dTexels, s->texels)) || (st = uploadVec(s->dTextures, tex)) || (st = uploadVec(s->dNrm, s->nrm)) ||
+		(st = uploadVec(s->dTan, s->tan)) || (st = uploadVec(s->dBit, s->bit)) || (st = uploadVec(s->dUv, s->uv)) ||
+		(st = uploadVec(s->dIdx, s->indices)) || (st = uploadVec(s->dMeshes, meshes)) || (st = uploadVec(s->dObjects, objs)) ||
+		(st = uploadVec(s->dMaterials, mats)) || (st = uploadVec(s->dLights, s->lights)))
+		return st;
+	DevScene & d = s->dev;
+	d.texels = static_cast<const float4 *>(s->dTexels.ptr);
+	d.textures = static_cast<const DevTexture *>(s->dTextures.ptr);
+	d.nrm = static_cast<const float *>(s->dNrm.ptr);
+	d.tan = static_cast<const float *>(s->dTan.ptr);
+	d.bit = static_cast<const float *>(s->dBit.ptr);
+	d.uv = static_cast<const float *>(s->dUv.ptr);
+	d.idx = static_cast<const uint32_t *>(s->dIdx.ptr);
+	d.meshes = static_cast<const DevMesh *>(s->dMeshes.ptr);
+	d.objects = static_cast<const DevObject *>(s->dObjects.ptr);
+	d.materials = static_cast<const DevMaterial *>(s->dMaterials.ptr);
+	d.lights = static_cast<const rdu_light_desc *>(s->dLights.ptr);
+	d.nLights = uint32_t(s->lights.size());
+	d.anyMasked = anyMasked;
+	d.bg = s->bg;
+	s->committed = true;
+	return RDU_OK;
+}
+
+const rdu_raycaster * rdu_scene_raycaster(const rdu_scene * s) { return s ? s->rc : nullptr; }
+
+int rdu_pathtracer_render(rdu_scene * s, const rdu_camera_desc * cam, size_t samples, size_t depth, float * image,
+	uint32_t width, uint32_t height, uint32_t components, const rdu_render_opts * opts) {
+	clearError();
+	const auto t0 = std::chrono::steady_clock::now();
+	if(!s)
+		return fail(RDU_ERR_INVALID, "[PathTracer] No scene available.");
+	if(!s->committed)
+		return fail(RDU_ERR_STATE, "scene not committed");
+	if(!cam || !image)
+		return fail(RDU_ERR_INVALID, "null camera or image");
+	if(components < 3)
+		return fail(RDU_ERR_INVALID, "[PathTracer] Expected a RGB image.");
+	if(width == 0 || height == 0 || samples == 0 || depth == 0)
+		return fail(RDU_ERR_INVALID, "empty image, zero samples or zero depth");
+	// PathTracer.cpp:169-173: round to a power of two.
+	const size_t spp = size_t(std::pow(2.0, std::round(std::log2(float(samples)))));
+	rdu_render_opts opt{};
+	if(opts)
+		opt = *opts;
+	PtOptions po;
+	po.depth = uint32_t(depth);
+	po.estimator = opt.estimator;
+	po.rrStart = opt.rr_start;
+	// Camera::pose + Camera::pixelShifts (Camera.cpp:10-18, 42-49).
+	const f3 eye = mk(cam->position[0], cam->position[1], cam->position[2]);
+	const f3 center = mk(cam->center[0], cam->center[1], cam->center[2]);
+	f3 up = normalize(mk(cam->up[0], cam->up[1], cam->up[2]));
+	const f3 viewDir = normalize(center - eye);
+	const f3 right = cross(viewDir, up);
+	up = cross(right, viewDir);
+	const float heightScale = std::tan(0.5f * cam->fov);
+	const float widthScale = cam->ratio * heightScale;
+	const float imageDist = length(center - eye);
+	PtFrame f;
+	f.width = width;
+	f.height = height;
+	f.seed = opt.seed;
+	f.eye = eye;
+	f.corner = center + imageDist * ((-widthScale) * right + heightScale * up);
+	f.dx = ((2.0f * widthScale) * imageDist) * right;
+	f.dy = ((-2.0f * heightScale) * imageDist) * up;
+	// getSampleGrid (PathTracer.cpp:57-69).
+	const int k = int(std::floor(std::log2(double(spp))));
+	int cx = int(std::pow(2, k / 2)), cy = cx;
+	if(k % 2 == 1) {
+		cx = int(std::pow(2, (k + 1) / 2));
+		cy = int(std::pow(2, (k - 1) / 2));
+	}
+	f.cellX = cx;
+	f.cellW = 1.0f / float(cx);
+	f.cellH = 1.0f / float(cy);
+
+	const size_t pixels = size_t(width) * height;
+	const size_t maxPaths = opt.max_paths ? opt.max_paths : (size_t(1) << 22);
+	if(pixels > (size_t(1) << 31))
+		return fail(RDU_ERR_UNSUPPORTED, "image larger than 2^31 pixels");
+	size_t perPass = maxPaths / pixels;
+	perPass = perPass < 1 ? 1 : (perPass > spp ? spp : perPass);
+	int st = setDevice(s->rc->device);
+	if(st)
+		return st;
+	if((st = ensurePaths(s, pixels * perPass)))
+		return st;
+	const size_t imgBytes = pixels * components * sizeof(float);
+	if((st = s->pImage.reserve(imgBytes)))
+		return st;
+	hipStream_t stream = nullptr;
+	if((st = hipCheck(hipMemcpyAsync(s->pImage.ptr, image, imgBytes, hipMemcpyHostToDevice, stream), "image upload")))
+		return st;
+	if((st = hipCheck(hipMemsetAsync(s->pTotals.ptr, 0, 16, stream), "hipMemsetAsync")))
+		return st;
+	PtPaths P;
+	P.rays = static_cast<float4 *>(s->pRays.ptr);
+	P.hits = static_cast<rdu_hit *>(s->pHits.ptr);
+	P.atten = static_cast<float4 *>(s->pAtten.ptr);
+	P.color = static_cast<float4 *>(s->pColor.ptr);
+	P.queue[0] = static_cast<uint32_t *>(s->pQ0.ptr);
+	P.queue[1] = static_cast<uint32_t *>(s->pQ1.ptr);
+	P.counts = static_cast<uint32_t *>(s->pCounts.ptr);
+	P.totals = static_cast<unsigned long long *>(s->pTotals.ptr);
+	P.shadowRays = static_cast<float4 *>(s->pSRays.ptr);
+	P.shadowContrib = static_cast<float4 *>(s->pSContrib.ptr);
+	P.shadowQueue[0] = static_cast<uint32_t *>(s->pSQ0.ptr);
+	P.shadowQueue[1] = static_cast<uint32_t *>(s->pSQ1.ptr);
+	P.occluded = static_cast<uint8_t *>(s->pOcc.ptr);
+	P.shadowHits = static_cast<rdu_hit *>(s->pSHits.ptr);
+	PtScratch scratch;
+	scratch.gstack = static_cast<int *>(s->pGstack.ptr);
+	for(size_t base = 0; base < spp; base += perPass) {
+		const size_t n = (spp - base) < perPass ? (spp - base) : perPass;
+		if((st = hipCheck(hipError_t(ptRenderPass(s->dev, s->rc->dev, f, P, uint32_t(base), uint32_t(n), po,
+								  static_cast<float *>(s->pImage.ptr), components, scratch, stream)),
+				"path tracing pass")))
+			return st;
+	}
+	if((st = hipCheck(hipError_t(ptNormalizeImage(static_cast<float *>(s->pImage.ptr), uint32_t(pixels), components,
+							  uint32_t(spp), stream)),
+			"normalise")))
+		return st;
+	if((st = hipCheck(hipMemcpyAsync(image, s->pImage.ptr, imgBytes, hipMemcpyDeviceToHost, stream), "image download")))
+		return st;
+	unsigned long long totals[2] = {0, 0};
+	if((st = hipCheck(hipMemcpyAsync(totals, s->pTotals.ptr, 16, hipMemcpyDeviceToHost, stream), "stats download")) ||
+		(st = hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize")))
+		return st;
+	const auto t1 = std::chrono::steady_clock::now();
+	s->last.seconds = std::chrono::duration<double>(t1 - t0).count();
+	s->last.paths = uint64_t(pixels) * spp;
+	s->last.extension_rays = totals[0];
+	s->last.shadow_rays = totals[1];
+	s->last.samples = uint32_t(spp);
+	return RDU_OK;
+}
+
+int rdu_scene_last_render_stats(const rdu_scene * s, rdu_render_stats * out) {
+	clearError();
+	if(!s || !out)
+		return fail(RDU_ERR_INVALID, "null argument");
+	*out = s->last;
+	return RDU_OK;
+}
+
+} // extern "C"

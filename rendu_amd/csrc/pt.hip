@@ -1,0 +1,354 @@
+// Wavefront path tracer for gfx950: Rendu's PathTracer::render (src/apps/pathtracer/PathTracer.cpp:
+// 159-299) split into kernels over queues of live paths.
+//
+// Per pass (a batch of pixel samples, path p = s * pixels + pixel):
+//   generate   camera rays (stratified jitter, PathTracer.cpp:57-84, 191-198); queue = all paths
+//   for each bounce b:
+//     extend   closest hit of every queued path (persistent traversal kernel, trace.hip)
+//     shade    background / emissive / masked cut-out; local frame; NEE light sample (shadow ray
+//              queued with its would-be contribution); BSDF sample; the surviving paths are
+//              appended to the next queue
+//     shadow   any hit (or, with masked materials, Rendu's closest-hit walk through cut-outs) of
+//              the queued shadow rays; visible ones add their contribution to the path
+//   finalize   clamp min(c, 5) per path; the pixel sums add the pass's samples in sample order, so
+//              every pixel accumulates exactly like Rendu's loop over sid (then / spp).
+//
+// Queues are compacted with a wavefront ballot + mbcnt prefix and one atomic per wavefront.
+// Counts live on the device: the extend/shadow traversals read them there, so a bounce issues no
+// host synchronisation.
+#include "pt.hpp"
+
+namespace rdu {
+
+namespace {
+
+constexpr int kPtBlock = 256;
+
+// Append `p` to a queue for every lane whose `want` is set: one atomic per wavefront.
+__device__ __forceinline__ void appendQueue(bool want, uint32_t p, uint32_t * queue, uint32_t * count) {
+	const uint64_t m = __ballot(want);
+	if(!m)
+		return;
+	const uint32_t lane = __lane_id();
+	const uint32_t leader = uint32_t(__ffsll((unsigned long long)m) - 1);
+	uint32_t base = 0;
+	if(lane == leader)
+		base = atomicAdd(count, uint32_t(__popcll(m)));
+	base = __shfl(base, int(leader), 64);
+	if(want) {
+		const uint32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+		queue[base + rank] = p;
+	}
+}
+
+__device__ __forceinline__ void putRay(float4 * rays, uint32_t p, f3 o, f3 d, float tmin, float tmax) {
+	rays[2 * size_t(p)] = make_float4(o.x, o.y, o.z, tmin);
+	rays[2 * size_t(p) + 1] = make_float4(d.x, d.y, d.z, tmax);
+}
+
+// Screen position of sample sid of pixel (x, y): getSamplePosition (PathTracer.cpp:71-84).
+__device__ __forceinline__ f2 samplePos(const PtFrame & f, uint32_t x, uint32_t y, uint32_t sid) {
+	float u[4];
+	draw4(f.seed, y * f.width + x, sid, 0, u);
+	const int sidy = int(sid) / f.cellX;
+	const int sidx = int(sid) % f.cellX;
+	const float gx = float(sidx) + u[0], gy = float(sidy) + u[1];
+	const f2 screen = {float(x) + gx * f.cellW, float(y) + gy * f.cellH};
+	return {screen.x / float(f.width), screen.y / float(f.height)};
+}
+
+__global__ __launch_bounds__(kPtBlock) void ptGenerate(PtFrame f, PtPaths P, uint32_t sampleBase, uint32_t nPaths) {
+	const uint32_t p = blockIdx.x * kPtBlock + threadIdx.x;
+	if(p >= nPaths)
+		return;
+	const uint32_t pixels = f.width * f.height;
+	const uint32_t s = p / pixels, pix = p % pixels;
+	const uint32_t x = pix % f.width, y = pix / f.width;
+	const f2 ndc = samplePos(f, x, y, sampleBase + s);
+	const f3 eye = f.eye;
+	const f3 worldPos = f.corner + ndc.x * f.dx + ndc.y * f.dy;
+	const f3 dir = normalize(worldPos - eye);
+	putRay(P.rays, p, eye, dir, 0.0001f, 1e8f);
+	P.atten[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+	P.color[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	P.queue[0][p] = p;
+}
+
+__global__ void ptResetCounts(uint32_t * counts, uint32_t active) {
+	if(threadIdx.x == 0) {
+		counts[kCountActive] = active;
+		counts[kCountNext] = 0;
+		counts[kCountShadow] = 0;
+		counts[kCountShadowNext] = 0;
+	}
+}
+
+// After a bounce: next queue becomes the active one; totals for the statistics.
+__global__ void ptAdvance(uint32_t * counts, unsigned long long * totals) {
+	if(threadIdx.x == 0) {
+		totals[0] += counts[kCountActive];
+		totals[1] += counts[kCountShadow];
+		counts[kCountActive] = counts[kCountNext];
+		counts[kCountNext] = 0;
+		counts[kCountShadow] = 0;
+		counts[kCountShadowNext] = 0;
+	}
+}
+
+__global__ void ptShadowSwap(uint32_t * counts, unsigned long long * totals) {
+	if(threadIdx.x == 0) {
+		totals[1] += counts[kCountShadow];
+		counts[kCountShadow] = counts[kCountShadowNext];
+		counts[kCountShadowNext] = 0;
+	}
+}
+
+// One bounce of PathTracer::render's loop body (PathTracer.cpp:202-286) for every queued path.
+__global__ __launch_bounds__(kPtBlock) void ptShade(DevScene S, PtFrame f, PtPaths P, const uint32_t * queueIn,
+	uint32_t * queueOut, uint32_t * counts, uint32_t bounce, uint32_t sampleBase, PtOptions opt) {
+	const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
+	const uint32_t n = counts[kCountActive];
+	bool cont = false, shadow = false;
+	uint32_t p = 0;
+	if(i < n) {
+		p = queueIn[i];
+		const uint32_t pixels = f.width * f.height;
+		const uint32_t s = p / pixels, pix = p % pixels;
+		const uint32_t sid = sampleBase + s;
+		const float4 ro = P.rays[2 * size_t(p)];
+		const float4 rd = P.rays[2 * size_t(p) + 1];
+		const f3 rayPos = {ro.x, ro.y, ro.z};
+		const f3 rayDir = {rd.x, rd.y, rd.z};
+		const rdu_hit hit = P.hits[p];
+		const float4 a4 = P.atten[p];
+		f3 atten = {a4.x, a4.y, a4.z};
+		float4 c4 = P.color[p];
+		f3 color = {c4.x, c4.y, c4.z};
+		if(!hit.hit) {
+			const f2 ndc = samplePos(f, pix % f.width, pix / f.width, sid);
+			color = color + atten * evalBackground(S, rayDir, rayPos, ndc, bounce == 0);
+		} else {
+			const DevObject obj = S.objects[hit.mesh_id];
+			const DevMaterial mat = S.materials[obj.mat];
+			const f3 pt = rayPos + hit.t * rayDir;
+			const float hw = 1.0f - hit.u - hit.v; // Hit::w (Raycaster.cpp:10-12)
+			const HitVerts hv = hitVerts(S, obj, hit.local_id);
+			const f2 uv = obj.useUV ? interp2(S.uv, hv.i0, hv.i1, hv.i2, hw, hit.u, hit.v) : f2{0.5f, 0.5f};
+			const f4v bCol = rgbal(S, mat.tex[0], uv.x, uv.y);
+			if(mat.masked && bCol.w < 0.01f) {
+				// Cut-out: continue from the hit point in the same direction; this consumes a bounce.
+				if(bounce + 1 < opt.depth) {
+					putRay(P.rays, p, pt, rayDir, 0.0001f, 1e8f);
+					cont = true;
+				}
+			} else if(mat.type == RDU_MATERIAL_EMISSIVE) {
+				color = color + atten * mk(bCol.x, bCol.y, bCol.z);
+			} else {
+				const M3 tbn = buildLocalFrame(S, obj, mat, hv, hw, hit.u, hit.v, rayDir, uv);
+				const f3 wo = normalize(mulT(tbn, -rayDir));
+				const f3 baseColor = {powf(bCol.x, 2.2f), powf(bCol.y, 2.2f), powf(bCol.z, 2.2f)};
+				const f4v rmao = rgbal(S, mat.tex[2], uv.x, uv.y);
+				float u[4];
+				draw4(f.seed, pix, sid, 1 + bounce, u);
+				if(S.nLights > 0) {
+					uint32_t lid = uint32_t(u[0] * float(S.nLights));
+					lid = lid < S.nLights ? lid : S.nLights - 1;
+					const rdu_light_desc & light = S.lights[lid];
+					const f3 pShift = pt + 0.001f * tbn.c[2];
+					float maxDist, falloff;
+					const f3 direction = lightSample(light, pShift, maxDist, falloff);
+					if(falloff > 0.0f) {
+						const f3 lwi = normalize(mulT(tbn, direction));
+						const f3 evalLight = ggxEval(wo, baseColor, rmao.x, rmao.y, lwi);
+						const float lightPdf = 1.0f / float(S.nLights);
+						const f3 illum = falloff * evalLight * mk(light.intensity[0], light.intensity[1], light.intensity[2]) / lightPdf;
+						const f3 contrib = atten * illum;
+						// checkVisibility (PathTracer.cpp:124-157) starts only while maxDist > 0.
+						if(light.casts_shadow && maxDist > 0.0f) {
+							putRay(P.shadowRays, p, pShift, direction, 0.001f, maxDist);
+							P.shadowContrib[p] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+							shadow = true;
+						} else {
+							color = color + contrib;
+						}
+					}
+				}
+				f3 wi;
+				const f3 eval = ggxSampleAndEval(wo, baseColor, rmao.x, rmao.y, u + 1, wi);
+				const f3 nextDir = normalize(mul(tbn, wi));
+				atten = atten * eval;
+				bool alive = bounce + 1 < opt.depth;
+				if(alive && opt.estimator == RDU_ESTIMATOR_EXTENDED && int(bounce) + 1 >= opt.rrStart) {
+					const float pmax = gmax(atten.x, gmax(atten.y, atten.z));
+					const float ps = gmin(pmax, 0.95f);
+					float rr[4];
+					draw4(f.seed, pix, sid, 0x10000u + bounce, rr);
+					if(!(rr[0] < ps))
+						alive = false;
+					else
+						atten = atten / ps;
+				}
+				// A path whose throughput is exactly zero can add nothing more: stop tracing it.
+				if(alive && !(atten.x == 0.0f && atten.y == 0.0f && atten.z == 0.0f)) {
+					putRay(P.rays, p, pt, normalize(nextDir), 0.0001f, 1e8f);
+					P.atten[p] = make_float4(atten.x, atten.y, atten.z, 0.0f);
+					cont = true;
+				}
+			}
+		}
+		P.color[p] = make_float4(color.x, color.y, color.z, 0.0f);
+	}
+	appendQueue(shadow, p, P.shadowQueue[0], counts + kCountShadow);
+	appendQueue(cont, p, queueOut, counts + kCountNext);
+}
+
+// Visible shadow rays add their contribution (any-hit result per path).
+__global__ __launch_bounds__(kPtBlock) void ptShadowApply(PtPaths P, const uint32_t * counts) {
+	const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
+	if(i >= counts[kCountShadow])
+		return;
+	const uint32_t p = P.shadowQueue[0][i];
+	if(P.occluded[p])
+		return;
+	const float4 c = P.color[p], d = P.shadowContrib[p];
+	P.color[p] = make_float4(c.x + d.x, c.y + d.y, c.z + d.z, 0.0f);
+}
+
+// Rendu's shadow walk through alpha cut-outs (PathTracer.cpp:124-157), one closest hit per step:
+// no hit -> visible; an unmasked (or UV-less) occluder -> occluded; a transparent texel ->
+// advance past it and query again.
+__global__ __launch_bounds__(kPtBlock) void ptShadowWalk(DevScene S, PtPaths P, const uint32_t * queueIn,
+	uint32_t * queueOut, uint32_t * counts) {
+	const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
+	bool again = false;
+	uint32_t p = 0;
+	if(i < counts[kCountShadow]) {
+		p = queueIn[i];
+		const rdu_hit h = P.shadowHits[p];
+		bool visible = false;
+		if(!h.hit) {
+			visible = true;
+		} else {
+			const DevObject obj = S.objects[h.mesh_id];
+			const DevMaterial mat = S.materials[obj.mat];
+			if(mat.masked && obj.useUV) {
+				const float hw = 1.0f - h.u - h.v;
+				const HitVerts hv = hitVerts(S, obj, h.local_id);
+				const f2 luv = interp2(S.uv, hv.i0, hv.i1, hv.i2, hw, h.u, h.v);
+				const float alpha = rgbal(S, mat.tex[0], luv.x, luv.y).w;
+				if(alpha < 0.01f) {
+					const float4 ro = P.shadowRays[2 * size_t(p)];
+					const float4 rd = P.shadowRays[2 * size_t(p) + 1];
+					const float maxDist = rd.w - h.t;
+					const f3 lpos = mk(ro.x, ro.y, ro.z) + h.t * mk(rd.x, rd.y, rd.z);
+					if(maxDist > 0.0f) {
+						putRay(P.shadowRays, p, lpos, mk(rd.x, rd.y, rd.z), 0.001f, maxDist);
+						again = true;
+					} else {
+						visible = true;
+					}
+				}
+			}
+		}
+		if(visible) {
+			const float4 c = P.color[p], d = P.shadowContrib[p];
+			P.color[p] = make_float4(c.x + d.x, c.y + d.y, c.z + d.z, 0.0f);
+		}
+	}
+	appendQueue(again, p, queueOut, counts + kCountShadowNext);
+}
+
+// image.rgb += min(color, 5) over the pass's samples in sample order (PathTracer.cpp:288).
+__global__ __launch_bounds__(kPtBlock) void ptAccumulate(PtFrame f, PtPaths P, float * image, uint32_t components,
+	uint32_t samplesInPass) {
+	const uint32_t pix = blockIdx.x * kPtBlock + threadIdx.x;
+	const uint32_t pixels = f.width * f.height;
+	if(pix >= pixels)
+		return;
+	float * px = image + size_t(pix) * components;
+	float r = px[0], g = px[1], b = px[2];
+	for(uint32_t s = 0; s < samplesInPass; ++s) {
+		const float4 c = P.color[size_t(s) * pixels + pix];
+		r += gmin(c.x, 5.0f);
+		g += gmin(c.y, 5.0f);
+		b += gmin(c.z, 5.0f);
+	}
+	px[0] = r;
+	px[1] = g;
+	px[2] = b;
+}
+
+// color = render.rgb / float(samples) (PathTracer.cpp:291-292).
+__global__ __launch_bounds__(kPtBlock) void ptNormalize(float * image, uint32_t pixels, uint32_t components, float spp) {
+	const uint32_t pix = blockIdx.x * kPtBlock + threadIdx.x;
+	if(pix >= pixels)
+		return;
+	float * px = image + size_t(pix) * components;
+	px[0] = px[0] / spp;
+	px[1] = px[1] / spp;
+	px[2] = px[2] / spp;
+}
+
+inline unsigned blocksFor(size_t n) { return unsigned((n + kPtBlock - 1) / kPtBlock); }
+
+} // namespace
+
+int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, const PtPaths & P, uint32_t sampleBase,
+	uint32_t samplesInPass, const PtOptions & opt, float * dImage, uint32_t components, PtScratch & scratch,
+	hipStream_t stream) {
+	const uint32_t pixels = f.width * f.height;
+	const uint32_t nPaths = pixels * samplesInPass;
+	hipLaunchKernelGGL(ptResetCounts, dim3(1), dim3(64), 0, stream, P.counts, nPaths);
+	hipLaunchKernelGGL(ptGenerate, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, f, P, sampleBase, nPaths);
+	int q = 0;
+	for(uint32_t b = 0; b < opt.depth; ++b) {
+		TraceOptions to;
+		to.wide = true;
+		to.perm = P.queue[q];
+		to.countDev = P.counts + kCountActive;
+		int e = launchClosest(bvh, P.rays, nPaths, P.hits, to, scratch.gstack, nullptr, stream);
+		if(e)
+			return e;
+		hipLaunchKernelGGL(ptShade, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, S, f, P, P.queue[q], P.queue[1 - q],
+			P.counts, b, sampleBase, opt);
+		if(S.nLights > 0) {
+			if(!S.anyMasked) {
+				TraceOptions so;
+				so.wide = true;
+				so.perm = P.shadowQueue[0];
+				so.countDev = P.counts + kCountShadow;
+				e = launchAny(bvh, P.shadowRays, nPaths, P.occluded, so, scratch.gstack, nullptr, stream);
+				if(e)
+					return e;
+				hipLaunchKernelGGL(ptShadowApply, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, P, P.counts);
+			} else {
+				// Cut-out walk: bounded by the number of masked layers a ray can cross; each step
+				// is a closest-hit query on the remaining shadow rays.
+				for(int layer = 0; layer < opt.maxShadowLayers; ++layer) {
+					TraceOptions so;
+					so.wide = true;
+					so.perm = P.shadowQueue[layer & 1];
+					so.countDev = P.counts + kCountShadow;
+					e = launchClosest(bvh, P.shadowRays, nPaths, P.shadowHits, so, scratch.gstack, nullptr, stream);
+					if(e)
+						return e;
+					hipLaunchKernelGGL(ptShadowWalk, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, S, P,
+						P.shadowQueue[layer & 1], P.shadowQueue[1 - (layer & 1)], P.counts);
+					hipLaunchKernelGGL(ptShadowSwap, dim3(1), dim3(64), 0, stream, P.counts, P.totals);
+				}
+			}
+		}
+		hipLaunchKernelGGL(ptAdvance, dim3(1), dim3(64), 0, stream, P.counts, P.totals);
+		q = 1 - q;
+	}
+	hipLaunchKernelGGL(ptAccumulate, dim3(blocksFor(pixels)), dim3(kPtBlock), 0, stream, f, P, dImage, components,
+		samplesInPass);
+	return int(hipGetLastError());
+}
+
+int ptNormalizeImage(float * dImage, uint32_t pixels, uint32_t components, uint32_t spp, hipStream_t stream) {
+	hipLaunchKernelGGL(ptNormalize, dim3(blocksFor(pixels)), dim3(kPtBlock), 0, stream, dImage, pixels, components, float(spp));
+	return int(hipGetLastError());
+}
+
+} // namespace rdu

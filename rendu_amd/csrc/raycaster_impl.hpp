@@ -1,0 +1,21 @@
+// The raycaster handle behind include/rdu.h, shared by the C ABI translation units.
+#pragma once
+#include "bvh.hpp"
+#include "capi_common.hpp"
+#include "trace.hpp"
+#include "../../include/rdu.h"
+
+#include <mutex>
+
+struct rdu_raycaster {
+	rdu::HostScene scene;
+	rdu::DeviceLayout layout;
+	rdu::DevBVH dev;
+	rdu::DevBuffer nodes, tris, roots, wide;
+	bool uploaded = false;
+	int device = -1;
+	// Staging for host-pointer queries and global stack spill; guarded by `lock`.
+	mutable std::mutex lock;
+	mutable rdu::DevBuffer stageIn, stageOut, gstack, stats;
+	mutable rdu::DevBuffer keys, keysAlt, perm, permAlt, sortTemp; // incoherent-batch ordering
+};

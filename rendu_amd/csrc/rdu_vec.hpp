@@ -50,12 +50,9 @@ RDU_HD f3 cross(f3 x, f3 y) {
 }
 RDU_HD float comp(f3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
 
-#if defined(__HIP_DEVICE_COMPILE__)
-// Correctly rounded square root (IEEE), matching SSE sqrtss on the host.
-__device__ __forceinline__ float sqrt_rn(float x) { return __builtin_sqrtf(x); }
-#else
-__host__ inline float sqrt_rn(float x) { return __builtin_sqrtf(x); }
-#endif
+// Correctly rounded square root (IEEE): sqrtss on the host; on gfx950 the correctly rounded
+// lowering selected by -fhip-fp32-correctly-rounded-divide-sqrt.
+RDU_HD float sqrt_rn(float x) { return __builtin_sqrtf(x); }
 
 RDU_HD f3 normalize(f3 v) {
 	const float inv = 1.0f / sqrt_rn(dot(v, v));

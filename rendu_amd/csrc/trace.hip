@@ -289,12 +289,14 @@ __device__ __forceinline__ bool finite3(f3 v) {
 
 template<bool ANY, bool REF, bool WIDE, bool SPILL, bool STATS>
 __global__ __launch_bounds__(kBlock) void traceKernel(DevBVH bvh, const float4 * __restrict__ rays, size_t n,
-	const uint32_t * __restrict__ perm, rdu_hit * __restrict__ hits, uint8_t * __restrict__ occ,
-	int * __restrict__ gstack, int lds, uint64_t * stats) {
+	const uint32_t * __restrict__ perm, const uint32_t * __restrict__ countDev, rdu_hit * __restrict__ hits,
+	uint8_t * __restrict__ occ, int * __restrict__ gstack, int lds, uint64_t * stats) {
 	__shared__ int ls[WIDE ? kWideLds : kLdsStack][kBlock];
 	const size_t gid = size_t(blockIdx.x) * kBlock + threadIdx.x;
 	LaneStack st{ls, gstack, size_t(gridDim.x) * kBlock, gid, lds};
 	TraceState s;
+	if(countDev)
+		n = size_t(*countDev) < n ? size_t(*countDev) : n;
 	const bool active = gid < n;
 	size_t ri = gid;
 	DevRay r;
@@ -348,8 +350,9 @@ __global__ __launch_bounds__(kBlock) void traceKernel(DevBVH bvh, const float4 *
 
 template<bool ANY, bool SPILL, bool STATS>
 __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
-	const uint32_t * __restrict__ perm, rdu_hit * __restrict__ hits, uint8_t * __restrict__ occ,
-	int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint32_t chunk, uint64_t * stats);
+	const uint32_t * __restrict__ perm, const uint32_t * __restrict__ countDev, rdu_hit * __restrict__ hits,
+	uint8_t * __restrict__ occ, int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint32_t chunk,
+	uint64_t * stats);
 
 // MI355X (gfx950) has 8 XCDs; workgroups are dispatched to them round-robin.
 constexpr uint32_t kXcds = 8;
@@ -382,8 +385,8 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 	const dim3 block(kBlock);
 	const uint32_t * perm = opt.perm;
 #define RDU_LAUNCH(R, W, S, ST)                                                                               \
-	hipLaunchKernelGGL((traceKernel<ANY, R, W, S, ST>), grid, block, 0, stream, bvh, rays, n, perm, hits, occ, gstack, \
-		lds, stats)
+	hipLaunchKernelGGL((traceKernel<ANY, R, W, S, ST>), grid, block, 0, stream, bvh, rays, n, perm, opt.countDev, hits, \
+		occ, gstack, lds, stats)
 #define RDU_BY_SPILL(R, W, ST)     \
 	if(spill)                      \
 		RDU_LAUNCH(R, W, true, ST); \
@@ -400,10 +403,10 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 		const uint32_t waves = (g.blocks / kXcds) * (kBlock / 64);                                               \
 		const uint64_t slice = (uint64_t(n) + kXcds - 1) / kXcds;                                                \
 		/* sorted batches: interleaved 64-ray chunks; input order: one contiguous run per wavefront */          \
-		uint64_t chunk = perm ? 64 : (slice + waves - 1) / (waves ? waves : 1);                                 \
+		uint64_t chunk = opt.spatialOrder ? 64 : (slice + waves - 1) / (waves ? waves : 1);                     \
 		chunk = chunk < 64 ? 64 : chunk;                                                                        \
 		hipLaunchKernelGGL((tracePersistent<ANY, S, ST>), dim3(g.blocks), dim3(kBlock), 0, stream, bvh, rays,      \
-			uint32_t(n), perm, hits, occ, gstack, lds, kXcds, refillMin(), uint32_t(chunk), stats);              \
+			uint32_t(n), perm, opt.countDev, hits, occ, gstack, lds, kXcds, refillMin(), uint32_t(chunk), stats); \
 	}
 		if(stats) {
 			if(spill) RDU_PERSIST(true, true) else RDU_PERSIST(false, true)
@@ -461,10 +464,13 @@ constexpr int kNeedRoot = 0x7ffffffe; // state: test the next root box
 // once entry > maxi). Levels spilled to global memory carry no entry and are always fetched.
 template<bool ANY, bool SPILL, bool STATS>
 __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
-	const uint32_t * __restrict__ perm, rdu_hit * __restrict__ hits, uint8_t * __restrict__ occ,
-	int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint32_t chunk, uint64_t * stats) {
+	const uint32_t * __restrict__ perm, const uint32_t * __restrict__ countDev, rdu_hit * __restrict__ hits,
+	uint8_t * __restrict__ occ, int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint32_t chunk,
+	uint64_t * stats) {
 	__shared__ int ls[kWideLds][kBlock];
 	__shared__ _Float16 le[ANY ? 1 : kWideLds][kBlock];
+	if(countDev)
+		n = *countDev < n ? *countDev : n;
 	const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
 	const uint32_t die = blockIdx.x % xcds;
 	const uint64_t W = uint64_t(gridDim.x / xcds) * (kBlock / 64);

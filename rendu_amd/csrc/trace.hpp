@@ -72,6 +72,8 @@ struct TraceOptions {
 	bool referenceOrder = false; // Rendu's exact depth-first order over the binary tree
 	bool wide = true;            // near-first order over the 4-wide collapse (else binary)
 	const uint32_t * perm = nullptr; // optional ray permutation: lane i handles ray perm[i]
+	const uint32_t * countDev = nullptr; // optional device count: only the first *countDev entries
+	bool spatialOrder = false; // perm is a spatial (Morton) order: interleave wavefronts finely
 };
 
 // Launchers. rays: device rdu_ray array (float4 pairs); outputs are device pointers, indexed by

@@ -1,0 +1,84 @@
+"""Python mirror of Rendu's PathTracer (src/apps/pathtracer/PathTracer.hpp:10-76) over librdu.
+
+PathTracer(scene) builds the raycaster over the scene's objects (PathTracer.cpp:11-21);
+render(camera, samples, depth, image) fills a caller-owned float image (PathTracer.cpp:159-299):
+like Rendu's Image it is accumulated into and divided by the sample count, so pass zeros.
+Rendering runs entirely in the library's wavefront kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import scene as S
+from ._lib import check, fptr, lib
+
+
+class _BorrowedRaycaster:
+    """The raycaster a committed scene owns (PathTracer::raycaster()), usable for queries."""
+
+    def __init__(self, handle, owner):
+        from .raycaster import Raycaster
+        self._owner = owner  # keeps the scene alive
+        rc = Raycaster.__new__(Raycaster)
+        rc._h = ctypes.c_void_p(handle)
+        rc._borrowed = True
+        self.rc = rc
+
+    def __getattr__(self, name):
+        return getattr(self.rc, name)
+
+    def __del__(self):
+        # Never destroy: the scene owns the handle.
+        if hasattr(self, "rc"):
+            self.rc._h = None
+
+
+class PathTracer:
+    def __init__(self, scene: S.Scene):
+        h = ctypes.c_void_p()
+        check(lib.rdu_scene_create(ctypes.byref(h)))
+        self._h = h
+        keep, tex, meshes, mats, objs, lights, bg = scene.descriptors()
+        i = ctypes.c_int32()
+        for t in tex:
+            check(lib.rdu_scene_add_texture(h, ctypes.byref(t), ctypes.byref(i)))
+        for m in meshes:
+            check(lib.rdu_scene_add_mesh(h, ctypes.byref(m), ctypes.byref(i)))
+        for m in mats:
+            check(lib.rdu_scene_add_material(h, ctypes.byref(m), ctypes.byref(i)))
+        for o in objs:
+            check(lib.rdu_scene_add_object(h, ctypes.byref(o)))
+        for l in lights:
+            check(lib.rdu_scene_add_light(h, ctypes.byref(l)))
+        check(lib.rdu_scene_set_background(h, ctypes.byref(bg)))
+        check(lib.rdu_scene_commit(h))
+        del keep
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.rdu_scene_destroy(h)
+            self._h = None
+
+    def raycaster(self):
+        """PathTracer::raycaster() (PathTracer.hpp:29)."""
+        return _BorrowedRaycaster(lib.rdu_scene_raycaster(self._h), self)
+
+    def render(self, camera, samples: int, depth: int, image: np.ndarray, seed: int = 0,
+               estimator: int = S.ESTIMATOR_REFERENCE, rr_start: int = 3, max_paths: int = 0) -> dict:
+        """PathTracer::render(camera, samples, depth, image). image: (h, w, c >= 3) float32."""
+        if image.dtype != np.float32 or not image.flags.c_contiguous or image.ndim != 3:
+            raise ValueError("image must be a C-contiguous (height, width, components) float32 array")
+        h, w, c = image.shape
+        opts = S.RenderOpts(seed, estimator, rr_start, max_paths)
+        cam = S.camera_desc(camera)
+        check(lib.rdu_pathtracer_render(self._h, ctypes.byref(cam), samples, depth, fptr(image), w, h, c,
+                                        ctypes.byref(opts)))
+        return self.last_stats()
+
+    def last_stats(self) -> dict:
+        st = S.RenderStats()
+        check(lib.rdu_scene_last_render_stats(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in S.RenderStats._fields_}

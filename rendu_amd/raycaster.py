@@ -64,9 +64,9 @@ class Raycaster:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
+        if h is not None and h.value and not getattr(self, "_borrowed", False):
             lib.rdu_raycaster_destroy(h)
-            self._h = None
+        self._h = None
 
     def __copy__(self):
         raise TypeError("Raycaster is not copyable (Raycaster.hpp:100-113)")

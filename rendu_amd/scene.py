@@ -1,0 +1,176 @@
+"""Scene description for the path tracer: what Rendu's PathTracer reads from its Scene.
+
+Mirrors src/engine/scene/{Scene,Object,Material}.hpp and the lights restricted to the fields
+PathTracer::render uses, with ctypes structures identical to include/rdu.h so the same
+description feeds the library (and, in tests, the oracle).
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import c_float, c_int32, c_size_t, c_uint32, c_void_p, POINTER
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from .workloads import MeshData
+
+F32 = np.float32
+
+MATERIAL_REGULAR = 1
+MATERIAL_EMISSIVE = 8
+LIGHT_POINT, LIGHT_DIRECTIONAL, LIGHT_SPOT = 0, 1, 2
+BG_COLOR, BG_IMAGE, BG_SKYBOX, BG_ATMOSPHERE, BG_LATLONG = 0, 1, 2, 3, 4
+ESTIMATOR_REFERENCE, ESTIMATOR_EXTENDED = 0, 1
+
+
+class TextureDesc(ctypes.Structure):
+    _fields_ = [("width", c_uint32), ("height", c_uint32), ("rgba", POINTER(c_float))]
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [("positions", POINTER(c_float)), ("normals", POINTER(c_float)), ("tangents", POINTER(c_float)),
+                ("bitangents", POINTER(c_float)), ("texcoords", POINTER(c_float)), ("indices", POINTER(c_uint32)),
+                ("n_vertices", c_size_t), ("n_indices", c_size_t)]
+
+
+class MaterialDesc(ctypes.Structure):
+    _fields_ = [("type", c_int32), ("masked", c_int32), ("textures", c_int32 * 3)]
+
+
+class ObjectDesc(ctypes.Structure):
+    _fields_ = [("mesh", c_uint32), ("material", c_uint32), ("use_texcoords", c_int32), ("model", c_float * 16)]
+
+
+class LightDesc(ctypes.Structure):
+    _fields_ = [("type", c_int32), ("casts_shadow", c_int32), ("intensity", c_float * 3), ("position", c_float * 3),
+                ("direction", c_float * 3), ("radius", c_float), ("angles", c_float * 2)]
+
+
+class BackgroundDesc(ctypes.Structure):
+    _fields_ = [("mode", c_int32), ("color", c_float * 3), ("texture", c_int32), ("cube", c_int32 * 6),
+                ("sun_direction", c_float * 3)]
+
+
+class CameraDesc(ctypes.Structure):
+    _fields_ = [("position", c_float * 3), ("center", c_float * 3), ("up", c_float * 3), ("fov", c_float),
+                ("ratio", c_float)]
+
+
+class RenderOpts(ctypes.Structure):
+    _fields_ = [("seed", c_uint32), ("estimator", c_int32), ("rr_start", c_int32), ("max_paths", c_uint32)]
+
+
+class RenderStats(ctypes.Structure):
+    _fields_ = [("seconds", ctypes.c_double), ("paths", ctypes.c_uint64), ("extension_rays", ctypes.c_uint64),
+                ("shadow_rays", ctypes.c_uint64), ("samples", c_uint32)]
+
+
+def _fp(a):
+    return a.ctypes.data_as(POINTER(c_float))
+
+
+def constant_texture(rgba) -> np.ndarray:
+    """An 8x8 constant image, as Rendu builds for a colour given inline (ResourcesManager.cpp:489-504)."""
+    c = np.asarray(list(rgba) + [1.0] * (4 - len(rgba)), F32)[:4]
+    return np.ascontiguousarray(np.broadcast_to(c, (8, 8, 4)).astype(F32))
+
+
+@dataclass
+class Material:
+    type: int = MATERIAL_REGULAR
+    textures: Sequence[int] = (0, 0, 0)  # albedo(+alpha), normal map, roughness/metal/AO
+    masked: bool = False
+
+
+@dataclass
+class Object:
+    mesh: int
+    material: int
+    model: np.ndarray = field(default_factory=lambda: np.eye(4, dtype=F32))
+    use_texcoords: bool = False
+
+
+@dataclass
+class Light:
+    type: int = LIGHT_POINT
+    intensity: Sequence[float] = (1.0, 1.0, 1.0)
+    position: Sequence[float] = (0.0, 0.0, 0.0)
+    direction: Sequence[float] = (0.0, -1.0, 0.0)
+    radius: float = 1.0
+    angles: Sequence[float] = (0.5, 0.7)
+    casts_shadow: bool = True
+
+
+@dataclass
+class Background:
+    mode: int = BG_COLOR
+    color: Sequence[float] = (0.0, 0.0, 0.0)
+    texture: int = -1
+    cube: Sequence[int] = (-1, -1, -1, -1, -1, -1)
+    sun_direction: Sequence[float] = (0.0, 1.0, 0.0)
+
+
+@dataclass
+class Scene:
+    """Objects (in order: object i is raycaster mesh i), meshes, materials, textures, lights, background."""
+    textures: List[np.ndarray] = field(default_factory=list)   # (h, w, 4) float32
+    meshes: List[MeshData] = field(default_factory=list)
+    materials: List[Material] = field(default_factory=list)
+    objects: List[Object] = field(default_factory=list)
+    lights: List[Light] = field(default_factory=list)
+    background: Background = field(default_factory=Background)
+
+    def add_texture(self, rgba: np.ndarray) -> int:
+        self.textures.append(np.ascontiguousarray(np.asarray(rgba, F32)))
+        return len(self.textures) - 1
+
+    def add_mesh(self, mesh: MeshData) -> int:
+        self.meshes.append(mesh)
+        return len(self.meshes) - 1
+
+    def add_material(self, mat: Material) -> int:
+        self.materials.append(mat)
+        return len(self.materials) - 1
+
+    # -- ctypes views (arrays kept alive by the returned holder) -----------------------------------
+    def descriptors(self):
+        keep = []
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(np.asarray(a, dt))
+            keep.append(a)
+            return a
+
+        tex = []
+        for t in self.textures:
+            a = arr(t, F32)
+            tex.append(TextureDesc(a.shape[1], a.shape[0], _fp(a)))
+        meshes = []
+        for m in self.meshes:
+            pos = arr(m.positions, F32).reshape(-1, 3)
+            nv = pos.shape[0]
+            nrm = arr(m.normals if m.normals is not None else np.tile([0, 0, 1], (nv, 1)), F32)
+            tan = arr(m.tangents if m.tangents is not None else np.tile([1, 0, 0], (nv, 1)), F32)
+            bit = arr(m.bitangents if m.bitangents is not None else np.tile([0, 1, 0], (nv, 1)), F32)
+            uv = arr(m.texcoords, F32) if m.texcoords is not None else None
+            idx = arr(m.indices, np.uint32)
+            meshes.append(MeshDesc(_fp(pos), _fp(nrm), _fp(tan), _fp(bit), _fp(uv) if uv is not None else None,
+                                   idx.ctypes.data_as(POINTER(c_uint32)), nv, idx.shape[0]))
+        mats = [MaterialDesc(m.type, int(m.masked), (c_int32 * 3)(*m.textures)) for m in self.materials]
+        objs = []
+        for o in self.objects:
+            col = np.asarray(o.model, F32).reshape(4, 4).T.reshape(-1)  # glm column-major
+            objs.append(ObjectDesc(o.mesh, o.material, int(o.use_texcoords), (c_float * 16)(*col)))
+        lights = [LightDesc(l.type, int(l.casts_shadow), (c_float * 3)(*l.intensity), (c_float * 3)(*l.position),
+                            (c_float * 3)(*l.direction), l.radius, (c_float * 2)(*l.angles)) for l in self.lights]
+        b = self.background
+        bg = BackgroundDesc(b.mode, (c_float * 3)(*b.color), b.texture, (c_int32 * 6)(*b.cube),
+                            (c_float * 3)(*b.sun_direction))
+        return keep, tex, meshes, mats, objs, lights, bg
+
+
+def camera_desc(camera) -> CameraDesc:
+    """From a rendu_amd.Camera (pose inputs + projection)."""
+    return CameraDesc((c_float * 3)(*camera.position), (c_float * 3)(*camera.center), (c_float * 3)(*camera.up_in),
+                      float(camera.fov), float(camera.ratio))
