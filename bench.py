@@ -1,14 +1,16 @@
 """Ray-cast benchmark: incoherent closest-hit MRays/s on one MI355X.
 
-Workload: 1,000,000-triangle soup (centroids U[0,1]^3, edge exp(-5 + 0.5 z), random rotation),
-2^23 incoherent rays (origins U[0,1]^3, directions uniform on the sphere) with Rendu's default
-interval (1e-4, 1e8). One step = one closest-hit pass (rdu_raycaster_intersect, near-first
-order) over the 2^23 incoherent rays, inputs and outputs resident in HBM. Coherent primaries
-(2^23, pinhole 4096x2048) and any-hit rates are reported alongside.
+Workload (the ray-cast microbenchmark): a 1,000,000-triangle soup (centroids U[0,1]^3, edge
+exp(-5 + 0.5 z), random rotation) with 2^23 incoherent rays (origins U[0,1]^3, directions uniform on
+the sphere) and 2^23 coherent pinhole primaries (4096 x 2048), Rendu's default interval (1e-4, 1e8).
+One step = one closest-hit pass (rdu_raycaster_intersect, near-first order, RDU_INCOHERENT hint so
+the library orders the rays itself inside the timed region) over the 2^23 incoherent rays, rays
+and hits resident in HBM. The refill threshold is autotuned on the same batch during warm-up.
 
-Prints one JSON line. The CPU baseline is the oracle's restatement of Rendu's Raycaster
-(same tree, same traversal, forParallel's max(hw-1, 1) thread policy) on a bounded sample of the
-same incoherent rays, on the host cores.
+Alongside: coherent closest hit, any hit, Rendu's exact visiting order, and the small path-tracing
+workload (Cornell box + 3 icospheres, 256x256, 16 spp, depth 5) in samples/s. The CPU baseline is
+the oracle's restatement of Rendu's Raycaster (same tree, same traversal, forParallel's thread
+policy) on a bounded sample of the same incoherent rays, on the host cores.
 """
 from __future__ import annotations
 
@@ -33,16 +35,15 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 21, help="rays in the CPU baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 23, help="rays in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--reference-order", action="store_true", help="time Rendu's exact visiting order")
     ap.add_argument("--main-only", action="store_true", help="only the timed incoherent closest-hit passes")
     return ap.parse_args()
 
 
-def load_traffic(tag: str):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if one exists."""
-    path = os.path.join(ROOT, "profiles", f"{tag}_pmc.json")
+def load_traffic():
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", "trace_closest_soup1m_pmc.json")
     if not os.path.exists(path):
         return None
     try:
@@ -55,8 +56,8 @@ def load_traffic(tag: str):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every box in this setup has a single MI355X; "
-              "the multi-GPU sharding design is described in DESIGN.md", file=sys.stderr)
+        print("bench.py measures one GPU: every box in this setup has a single MI355X; how the path "
+              "would shard over several GPUs is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -84,48 +85,72 @@ def main():
     d_occ = torch.empty(N_HALF, dtype=torch.uint8, device="cuda")
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
-    ref = bool(args.reference_order)
 
-    def closest(d):
-        rc.intersect_device(d.data_ptr(), N_HALF, d_hits.data_ptr(), sh, reference_order=ref)
+    def closest(d, incoh=True, ref=False):
+        rc.intersect_device(d.data_ptr(), N_HALF, d_hits.data_ptr(), sh, reference_order=ref, incoherent=incoh)
 
-    def anyhit(d):
-        rc.intersect_device(d.data_ptr(), N_HALF, d_occ.data_ptr(), sh, any_hit=True, reference_order=ref)
+    def anyhit(d, incoh=True):
+        rc.intersect_device(d.data_ptr(), N_HALF, d_occ.data_ptr(), sh, any_hit=True, incoherent=incoh)
 
-    def timed(fn, d, steps, warmup):
+    def timed(fn, steps, warmup):
         for _ in range(warmup):
-            fn(d)
+            fn()
         torch.cuda.synchronize()
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
         w0 = time.perf_counter()
         for a, b in ev:
             a.record(stream)
-            fn(d)
+            fn()
             b.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - w0
-        kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
-        return wall, kernel_ms
+        return wall, float(np.mean([a.elapsed_time(b) for a, b in ev]))
 
-    wall, k_ms = timed(closest, d_inc, args.steps, args.warmup)
+    tune = rc.autotune(rays_ptr=d_inc.data_ptr(), n=N_HALF, incoherent=True, stream=sh)
+    wall, k_ms = timed(lambda: closest(d_inc), args.steps, args.warmup)
     value = args.steps * N_HALF / wall / 1e6
+    if args.main_only:  # profiling runs: the timed incoherent closest-hit passes only
+        print(json.dumps({"value": round(value, 2), "kernel_ms": round(k_ms, 4), "tune": tune}), flush=True)
+        return
     hits = d_hits.cpu().numpy().view(np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("hit", "<u4"),
                                                 ("mesh_id", "<u4"), ("local_id", "<u4")]))
     hit_rate = float((hits["hit"] == 1).mean())
-    if args.main_only:  # profiling runs: the timed incoherent closest-hit passes only
-        print(json.dumps({"value": round(value, 2), "kernel_ms": round(k_ms, 4)}), flush=True)
-        return
+
     steps2 = max(3, args.steps // 2)
-    wall_c, kc_ms = timed(closest, d_coh, steps2, 1)
-    wall_ai, kai_ms = timed(anyhit, d_inc, steps2, 1)
-    wall_ac, kac_ms = timed(anyhit, d_coh, steps2, 1)
+    rate = lambda w, k: round(k * N_HALF / w / 1e6, 2)
+    tune_c = rc.autotune(rays_ptr=d_coh.data_ptr(), n=N_HALF, incoherent=False, stream=sh)
+    extra = {
+        "coherent_closest_mrays": rate(timed(lambda: closest(d_coh, incoh=False), steps2, 1)[0], steps2),
+        "incoherent_closest_input_order_mrays": rate(timed(lambda: closest(d_inc, incoh=False), steps2, 1)[0], steps2),
+        "incoherent_any_mrays": rate(timed(lambda: anyhit(d_inc), steps2, 1)[0], steps2),
+        "coherent_any_mrays": rate(timed(lambda: anyhit(d_coh, incoh=False), steps2, 1)[0], steps2),
+        "incoherent_closest_reference_order_mrays": rate(timed(lambda: closest(d_inc, ref=True), 2, 1)[0], 2),
+    }
+    sub = np.ascontiguousarray(incoherent[: 1 << 20])
+    nv, tt = rc.trace_stats(sub)
+    extra["near_first_wide_nodes_per_ray"] = round(nv / sub.shape[0], 2)
+    extra["near_first_triangle_tests_per_ray"] = round(tt / sub.shape[0], 2)
 
-    # Algorithmic bytes per launch: each ray read once (32 B), each hit written once (24 B), and
-    # the scene (internal nodes 64 B, triangles 48 B, roots 32 B) read once per launch.
-    scene_bytes = info["device_bytes"]
-    alg_bytes = N_HALF * (32 + 24) + scene_bytes
+    # Small path-tracing workload on the GPU (samples/s).
+    from rendu_amd import scenes
+    from rendu_amd.pathtracer import PathTracer
+    scn, cam_of = scenes.cornell_spheres(seed=3, subdivisions=4)
+    pt = PathTracer(scn)
+    img = np.zeros((256, 256, 3), np.float32)
+    pt.render(cam_of(256, 256), 16, 5, img, seed=1)
+    best = None
+    for rep in range(3):
+        img[:] = 0
+        st = pt.render(cam_of(256, 256), 16, 5, img, seed=1 + rep)
+        best = st if best is None or st["seconds"] < best["seconds"] else best
+    extra["cornell_256_16spp_depth5_msamples_per_s"] = round(best["paths"] / best["seconds"] / 1e6, 2)
+    extra["cornell_256_16spp_depth5_ms"] = round(best["seconds"] * 1e3, 3)
+    extra["cornell_rays_per_sample"] = round((best["extension_rays"] + best["shadow_rays"]) / best["paths"], 3)
+
+    # Algorithmic bytes per launch: each ray read once (32 B), each hit written once (24 B), the
+    # ray order (key + index written and read back, 16 B), and the traversed scene arrays once.
+    alg_bytes = N_HALF * (32 + 24 + 16) + info["wide_bytes"]
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
-
     out = {
         "metric": "incoherent MRays/s at 1 GPU (closest hit, 1M-triangle soup)",
         "value": round(value, 2),
@@ -145,7 +170,9 @@ def main():
             "rays_per_step": N_HALF,
             "bvh_nodes": info["nodes"],
             "bvh_depth": info["max_depth"],
-            "visiting_order": "reference" if ref else "near-first",
+            "wide_nodes": info["wide_nodes"],
+            "visiting_order": "near-first, 4-wide, rays Morton-ordered in the timed region",
+            "refill_min": tune["refill_min"],
         },
         "roofline": {
             "bound": "hbm",
@@ -153,17 +180,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": load_traffic("trace_closest_soup1m"),
+            "traffic": load_traffic(),
         },
-        "extra": {
-            "kernel_ms_incoherent_closest": round(k_ms, 4),
-            "hit_rate_incoherent": round(hit_rate, 4),
-            "coherent_closest_mrays": round(steps2 * N_HALF / wall_c / 1e6, 2),
-            "incoherent_any_mrays": round(steps2 * N_HALF / wall_ai / 1e6, 2),
-            "coherent_any_mrays": round(steps2 * N_HALF / wall_ac / 1e6, 2),
-            "bvh_build_s": round(build_s, 3),
-            "input_generation_s": round(gen_s, 3),
-        },
+        "extra": dict(extra, kernel_ms_per_step=round(k_ms, 4), hit_rate_incoherent=round(hit_rate, 4),
+                      autotune_incoherent=tune, autotune_coherent=tune_c, bvh_build_s=round(build_s, 3),
+                      input_generation_s=round(gen_s, 3)),
     }
 
     if not args.no_cpu:

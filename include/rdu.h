@@ -247,6 +247,20 @@ typedef struct {
 } rdu_render_stats;
 int rdu_scene_last_render_stats(const rdu_scene* scene, rdu_render_stats* out);
 
+/* Autotuner (no reference counterpart). Times the near-first traversal on the caller's batch for
+ * each candidate of the wavefront refill threshold (idle lanes a persistent wavefront waits for
+ * before fetching new rays) and keeps the fastest for later queries of the same class
+ * (RDU_INCOHERENT or not, closest or any hit) on this raycaster. flags as for the batch queries;
+ * the batch is only read. Not thread-safe against concurrent queries on the same raycaster. */
+typedef struct {
+	uint32_t refill_min; /* chosen threshold */
+	float best_ms;       /* mean time of one batch with it */
+	float default_ms;    /* mean time with the untuned default */
+	uint32_t candidates; /* thresholds tried */
+} rdu_tune_report;
+int rdu_raycaster_autotune(rdu_raycaster* rc, const rdu_ray* rays, size_t n, int any_hit, uint32_t flags, void* stream,
+	rdu_tune_report* report);
+
 /* Last error message of the calling thread ("" if none). */
 const char* rdu_last_error(void);
 

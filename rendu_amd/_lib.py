@@ -48,6 +48,10 @@ class _Info(ctypes.Structure):
                 ("wide_bytes", c_uint64)]
 
 
+class TuneReport(ctypes.Structure):
+    _fields_ = [("refill_min", c_uint32), ("best_ms", c_float), ("default_ms", c_float), ("candidates", c_uint32)]
+
+
 def _load() -> ctypes.CDLL:
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
@@ -69,6 +73,7 @@ def _load() -> ctypes.CDLL:
         "rdu_raycaster_export_triangles": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint32)]),
         "rdu_raycaster_trace_stats": (c_int, [c_void_p, c_void_p, c_size_t, c_int, c_uint32,
                                               POINTER(c_uint64), POINTER(c_uint64)]),
+        "rdu_raycaster_autotune": (c_int, [c_void_p, c_void_p, c_size_t, c_int, c_uint32, c_void_p, c_void_p]),
         "rdu_scene_create": (c_int, [POINTER(c_void_p)]),
         "rdu_scene_destroy": (None, [c_void_p]),
         "rdu_scene_add_texture": (c_int, [c_void_p, c_void_p, POINTER(ctypes.c_int32)]),

@@ -189,73 +189,118 @@ void HostScene::flatten(DeviceLayout & out) const {
 		o[9] = fromBits(t.meshId);
 		o[10] = fromBits(t.localId);
 	}
-	// 4-wide collapse, depth-first preorder numbering.
+	// Top level over the mesh roots (near-first queries only): Rendu tests every root box in turn
+	// (Raycaster.cpp:149-155); here the roots are grouped under a binary tree of exact box merges,
+	// split at the median root centroid on the largest axis. A group's box contains its members'
+	// boxes, so it culls nothing that a member's own test would keep.
+	std::vector<HNode> ext(nodes);
+	uint64_t top = 0;
+	if(meshCount > 1) {
+		struct TWork {
+			uint64_t id, begin, count;
+		};
+		std::vector<uint64_t> ids(meshCount);
+		for(uint32_t r = 0; r < meshCount; ++r)
+			ids[r] = r;
+		std::vector<TWork> tw;
+		ext.emplace_back();
+		top = ext.size() - 1;
+		tw.push_back({top, 0, meshCount});
+		while(!tw.empty()) {
+			const TWork w = tw.back();
+			tw.pop_back();
+			HBox box = nodes[ids[w.begin]].box;
+			for(uint64_t k = 1; k < w.count; ++k)
+				mergeInto(box, nodes[ids[w.begin + k]].box);
+			ext[w.id].box = box;
+			ext[w.id].leaf = false;
+			const f3 sz = box.mx - box.mn;
+			const int axis = (sz.x >= sz.y && sz.x >= sz.z) ? 0 : (sz.y >= sz.z ? 1 : 2);
+			const uint64_t half = w.count / 2;
+			auto cen = [&](uint64_t id) { return comp(nodes[id].box.mn, axis) + comp(nodes[id].box.mx, axis); };
+			std::nth_element(ids.begin() + w.begin, ids.begin() + w.begin + half, ids.begin() + w.begin + w.count,
+				[&](uint64_t x, uint64_t y) { return cen(x) < cen(y); });
+			uint64_t kid[2];
+			const uint64_t beg[2] = {w.begin, w.begin + half}, cnt[2] = {half, w.count - half};
+			for(int k = 0; k < 2; ++k) {
+				if(cnt[k] == 1) {
+					kid[k] = ids[beg[k]];
+				} else {
+					ext.emplace_back();
+					kid[k] = ext.size() - 1;
+					tw.push_back({kid[k], beg[k], cnt[k]});
+				}
+			}
+			ext[w.id].left = kid[0];
+			ext[w.id].right = kid[1];
+		}
+	}
+	out.wideTopBox = ext[top].box;
+
+	// 4-wide collapse from the single top node, depth-first preorder numbering.
 	out.wide.clear();
 	out.wideCount = 0;
 	out.wideDepth = 0;
 	auto area = [&](uint64_t id) {
-		const f3 s = nodes[id].box.mx - nodes[id].box.mn;
+		const f3 s = ext[id].box.mx - ext[id].box.mn;
 		return s.x * s.y + s.y * s.z + s.z * s.x;
 	};
-	std::vector<int32_t> wideRoot(meshCount, 0);
 	struct WItem {
-		uint64_t id; // internal node of Rendu's tree heading this 4-wide node
+		uint64_t id; // internal node heading this 4-wide node
 		uint32_t slot; // wide index assigned to it
 		uint32_t depth;
 	};
 	std::vector<WItem> wst;
-	for(uint32_t r = 0; r < meshCount; ++r) {
-		if(nodes[r].leaf) {
-			wideRoot[r] = leafCode(nodes[r].left, nodes[r].right);
-			continue;
-		}
-		wideRoot[r] = int32_t(out.wideCount++);
+	if(ext[top].leaf) {
+		out.wideTopCode = leafCode(ext[top].left, ext[top].right);
+	} else {
+		out.wideTopCode = int32_t(out.wideCount++);
 		out.wide.resize(size_t(out.wideCount) * 32, 0.0f);
-		wst.push_back({r, uint32_t(wideRoot[r]), 1});
-		while(!wst.empty()) {
-			const WItem it = wst.back();
-			wst.pop_back();
-			out.wideDepth = std::max(out.wideDepth, it.depth);
-			std::vector<uint64_t> kids = {nodes[it.id].left, nodes[it.id].right};
-			while(kids.size() < 4) {
-				int best = -1;
-				float bestArea = -1.0f;
-				for(size_t k = 0; k < kids.size(); ++k)
-					if(!nodes[kids[k]].leaf && area(kids[k]) > bestArea) {
-						best = int(k);
-						bestArea = area(kids[k]);
-					}
-				if(best < 0)
-					break;
-				const uint64_t e = kids[size_t(best)];
-				kids[size_t(best)] = nodes[e].left;
-				kids.insert(kids.begin() + best + 1, nodes[e].right);
-			}
-			int32_t codes[4] = {kWideEmpty, kWideEmpty, kWideEmpty, kWideEmpty};
-			float * o = out.wide.data() + size_t(it.slot) * 32;
-			std::vector<WItem> children;
-			for(size_t k = 0; k < kids.size(); ++k) {
-				const HNode & c = nodes[kids[k]];
-				o[0 + k] = c.box.mn.x;
-				o[4 + k] = c.box.mx.x;
-				o[8 + k] = c.box.mn.y;
-				o[12 + k] = c.box.mx.y;
-				o[16 + k] = c.box.mn.z;
-				o[20 + k] = c.box.mx.z;
-				if(c.leaf) {
-					codes[k] = leafCode(c.left, c.right);
-				} else {
-					codes[k] = int32_t(out.wideCount++);
-					children.push_back({kids[k], uint32_t(codes[k]), it.depth + 1});
+		wst.push_back({top, uint32_t(out.wideTopCode), 1});
+	}
+	while(!wst.empty()) {
+		const WItem it = wst.back();
+		wst.pop_back();
+		out.wideDepth = std::max(out.wideDepth, it.depth);
+		std::vector<uint64_t> kids = {ext[it.id].left, ext[it.id].right};
+		while(kids.size() < 4) {
+			int best = -1;
+			float bestArea = -1.0f;
+			for(size_t k = 0; k < kids.size(); ++k)
+				if(!ext[kids[k]].leaf && area(kids[k]) > bestArea) {
+					best = int(k);
+					bestArea = area(kids[k]);
 				}
-			}
-			out.wide.resize(size_t(out.wideCount) * 32, 0.0f);
-			o = out.wide.data() + size_t(it.slot) * 32;
-			for(int k = 0; k < 4; ++k)
-				o[24 + k] = fromBits(uint32_t(codes[k]));
-			for(auto c = children.rbegin(); c != children.rend(); ++c)
-				wst.push_back(*c);
+			if(best < 0)
+				break;
+			const uint64_t e = kids[size_t(best)];
+			kids[size_t(best)] = ext[e].left;
+			kids.insert(kids.begin() + best + 1, ext[e].right);
 		}
+		int32_t codes[4] = {kWideEmpty, kWideEmpty, kWideEmpty, kWideEmpty};
+		float * o = out.wide.data() + size_t(it.slot) * 32;
+		std::vector<WItem> children;
+		for(size_t k = 0; k < kids.size(); ++k) {
+			const HNode & c = ext[kids[k]];
+			o[0 + k] = c.box.mn.x;
+			o[4 + k] = c.box.mx.x;
+			o[8 + k] = c.box.mn.y;
+			o[12 + k] = c.box.mx.y;
+			o[16 + k] = c.box.mn.z;
+			o[20 + k] = c.box.mx.z;
+			if(c.leaf) {
+				codes[k] = leafCode(c.left, c.right);
+			} else {
+				codes[k] = int32_t(out.wideCount++);
+				children.push_back({kids[k], uint32_t(codes[k]), it.depth + 1});
+			}
+		}
+		out.wide.resize(size_t(out.wideCount) * 32, 0.0f);
+		o = out.wide.data() + size_t(it.slot) * 32;
+		for(int k = 0; k < 4; ++k)
+			o[24 + k] = fromBits(uint32_t(codes[k]));
+		for(auto c = children.rbegin(); c != children.rend(); ++c)
+			wst.push_back(*c);
 	}
 
 	out.roots.assign(size_t(meshCount) * 8, 0.0f);
@@ -269,7 +314,6 @@ void HostScene::flatten(DeviceLayout & out) const {
 		o[4] = n.box.mx.x;
 		o[5] = n.box.mx.y;
 		o[6] = n.box.mx.z;
-		o[7] = fromBits(uint32_t(wideRoot[r]));
 		if(r == 0) {
 			out.sceneMin = n.box.mn;
 			out.sceneMax = n.box.mx;

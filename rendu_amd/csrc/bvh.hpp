@@ -54,6 +54,8 @@ struct DeviceLayout {
 	uint32_t maxDepth = 0;
 	uint32_t wideCount = 0;
 	uint32_t wideDepth = 0;
+	HBox wideTopBox{{0, 0, 0}, {0, 0, 0}}; // the single root of the 4-wide tree (top level over meshes)
+	int32_t wideTopCode = 0;
 	f3 sceneMin{0, 0, 0}, sceneMax{0, 0, 0};
 };
 

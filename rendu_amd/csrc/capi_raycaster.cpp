@@ -57,6 +57,7 @@ int query(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, void * out, 
 	if(n > (size_t(1) << 31))
 		return fail(RDU_ERR_UNSUPPORTED, "batch larger than 2^31 rays");
 	TraceOptions opt = traceOptions(flags);
+	opt.refillMin = rc->tunedRefill[(flags & RDU_INCOHERENT) ? 1 : 0][ANY ? 1 : 0];
 	const size_t outBytes = ANY ? n : n * sizeof(rdu_hit);
 	std::lock_guard<std::mutex> guard(rc->lock);
 	if((st = setDevice(rc->device)))
@@ -192,6 +193,11 @@ int rdu_raycaster_update_hierarchy(rdu_raycaster * rc) {
 	const float l3[3] = {lo.x, lo.y, lo.z}, h3[3] = {hi.x, hi.y, hi.z};
 	std::memcpy(rc->dev.sceneMin, l3, sizeof(l3));
 	std::memcpy(rc->dev.sceneMax, h3, sizeof(h3));
+	const HBox & tb = rc->layout.wideTopBox;
+	const float tl[3] = {tb.mn.x, tb.mn.y, tb.mn.z}, th[3] = {tb.mx.x, tb.mx.y, tb.mx.z};
+	std::memcpy(rc->dev.wideTopMin, tl, sizeof(tl));
+	std::memcpy(rc->dev.wideTopMax, th, sizeof(th));
+	rc->dev.wideTopCode = rc->layout.wideTopCode;
 	rc->uploaded = true;
 	return RDU_OK;
 }
@@ -289,6 +295,81 @@ int rdu_raycaster_export_triangles(const rdu_raycaster * rc, uint32_t * mesh_id,
 	for(size_t i = 0; i < rc->scene.tris.size(); ++i) {
 		mesh_id[i] = rc->scene.tris[i].meshId;
 		local_id[i] = rc->scene.tris[i].localId;
+	}
+	return RDU_OK;
+}
+
+int rdu_raycaster_autotune(rdu_raycaster * rc, const rdu_ray * rays, size_t n, int any_hit, uint32_t flags, void * stream,
+	rdu_tune_report * report) {
+	clearError();
+	int st = checkBuilt(rc);
+	if(st)
+		return st;
+	if(!rays || n == 0)
+		return fail(RDU_ERR_INVALID, "autotune needs a non-empty batch");
+	if(flags & RDU_REFERENCE_ORDER)
+		return fail(RDU_ERR_INVALID, "the reference visiting order has nothing to tune");
+	const bool device = (flags & RDU_DEVICE_POINTERS) != 0;
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	// Device copies of the batch so that every candidate is timed on the GPU work alone.
+	DevBuffer inBuf, outBuf;
+	const rdu_ray * dRays = rays;
+	if(!device) {
+		if((st = inBuf.upload(rays, n * sizeof(rdu_ray))))
+			return st;
+		dRays = static_cast<const rdu_ray *>(inBuf.ptr);
+	}
+	if((st = outBuf.reserve(any_hit ? n : n * sizeof(rdu_hit))))
+		return st;
+	const uint32_t qflags = (flags & ~RDU_DEVICE_POINTERS) | RDU_DEVICE_POINTERS;
+	const uint32_t cls = (flags & RDU_INCOHERENT) ? 1 : 0;
+	const uint32_t candidates[] = {4, 8, 16, 24, 32, 40, 48, 64};
+	hipEvent_t e0, e1;
+	if((st = hipCheck(hipEventCreate(&e0), "hipEventCreate")) || (st = hipCheck(hipEventCreate(&e1), "hipEventCreate")))
+		return st;
+	float bestMs = 0.0f, defaultMs = 0.0f;
+	uint32_t best = 0;
+	for(uint32_t k = 0; k <= sizeof(candidates) / sizeof(candidates[0]); ++k) {
+		// k == 0: the untuned default; then each candidate.
+		rc->tunedRefill[cls][any_hit ? 1 : 0] = k == 0 ? 0 : candidates[k - 1];
+		float ms = 0.0f;
+		for(int rep = 0; rep < 3 && !st; ++rep) {
+			st = any_hit ? query<true>(rc, dRays, n, outBuf.ptr, qflags, s, nullptr)
+						 : query<false>(rc, dRays, n, outBuf.ptr, qflags, s, nullptr); // warm-up / first rep
+			if(st || rep == 0)
+				continue;
+			if((st = hipCheck(hipEventRecord(e0, s), "hipEventRecord")))
+				break;
+			st = any_hit ? query<true>(rc, dRays, n, outBuf.ptr, qflags, s, nullptr)
+						 : query<false>(rc, dRays, n, outBuf.ptr, qflags, s, nullptr);
+			if(!st)
+				st = hipCheck(hipEventRecord(e1, s), "hipEventRecord");
+			if(!st)
+				st = hipCheck(hipEventSynchronize(e1), "hipEventSynchronize");
+			float t = 0.0f;
+			if(!st)
+				st = hipCheck(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+			ms += t;
+		}
+		if(st)
+			break;
+		if(k == 0)
+			defaultMs = ms;
+		else if(best == 0 || ms < bestMs) {
+			bestMs = ms;
+			best = candidates[k - 1];
+		}
+	}
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	rc->tunedRefill[cls][any_hit ? 1 : 0] = st ? 0 : best;
+	if(st)
+		return st;
+	if(report) {
+		report->refill_min = best;
+		report->best_ms = bestMs / 2.0f;
+		report->default_ms = defaultMs / 2.0f;
+		report->candidates = uint32_t(sizeof(candidates) / sizeof(candidates[0]));
 	}
 	return RDU_OK;
 }

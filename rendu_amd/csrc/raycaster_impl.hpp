@@ -18,4 +18,6 @@ struct rdu_raycaster {
 	mutable std::mutex lock;
 	mutable rdu::DevBuffer stageIn, stageOut, gstack, stats;
 	mutable rdu::DevBuffer keys, keysAlt, perm, permAlt, sortTemp; // incoherent-batch ordering
+	// Autotuned refill threshold per [incoherent][any-hit] class (0 = not tuned).
+	uint32_t tunedRefill[2][2] = {{0, 0}, {0, 0}};
 };

@@ -268,16 +268,21 @@ __device__ __forceinline__ void traceRay(const DevBVH & bvh, const DevRay & r, L
 	const float maxi0 = r.maxi;
 	// Rendu pushes every root whose box passes (tested with the query's maxi), then pops the last
 	// one first: visiting roots from M-1 down to 0 is the same order.
+	if(WIDE) {
+		// One root: the top of the 4-wide tree.
+		float e;
+		if(slab6<EXACT>(r, bvh.wideTopMin[0], bvh.wideTopMin[1], bvh.wideTopMin[2], bvh.wideTopMax[0], bvh.wideTopMax[1],
+			   bvh.wideTopMax[2], mini, s.maxi, e))
+			traverseWide<ANY, SPILL, EXACT, STATS>(bvh, r, bvh.wideTopCode, mini, st, s);
+		return;
+	}
 	for(int ri = int(bvh.nRoots) - 1; ri >= 0; --ri) {
 		const float4 rmin = bvh.roots[2 * ri];
 		const float4 rmax = bvh.roots[2 * ri + 1];
 		float e;
 		if(!slab6<EXACT>(r, rmin.x, rmin.y, rmin.z, rmax.x, rmax.y, rmax.z, mini, REF ? maxi0 : s.maxi, e))
 			continue;
-		if(WIDE)
-			traverseWide<ANY, SPILL, EXACT, STATS>(bvh, r, __float_as_int(rmax.w), mini, st, s);
-		else
-			traverseBinary<ANY, REF, SPILL, EXACT, STATS>(bvh, r, __float_as_int(rmin.w), mini, st, s);
+		traverseBinary<ANY, REF, SPILL, EXACT, STATS>(bvh, r, __float_as_int(rmin.w), mini, st, s);
 		if(ANY && s.found)
 			return;
 	}
@@ -362,10 +367,18 @@ struct PersistentGrid {
 };
 
 // Idle lanes needed before a wavefront refills (RDU_REFILL_MIN overrides, 1..64).
-uint32_t refillMin() {
+// Refill threshold of a launch: RDU_REFILL_MIN (1..64) when set, else the batch's tuned value,
+// else 32 for spatially sorted (incoherent) batches and 16 for input order.
+uint32_t refillMin(const TraceOptions & opt) {
 	const char * v = std::getenv("RDU_REFILL_MIN");
-	const int k = v ? std::atoi(v) : 16;
-	return (k >= 1 && k <= 64) ? uint32_t(k) : 16u;
+	if(v) {
+		const int k = std::atoi(v);
+		if(k >= 1 && k <= 64)
+			return uint32_t(k);
+	}
+	if(opt.refillMin >= 1 && opt.refillMin <= 64)
+		return opt.refillMin;
+	return opt.spatialOrder ? 32u : 16u;
 }
 
 template<bool ANY, bool SPILL, bool STATS>
@@ -406,7 +419,7 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 		uint64_t chunk = opt.spatialOrder ? 64 : (slice + waves - 1) / (waves ? waves : 1);                     \
 		chunk = chunk < 64 ? 64 : chunk;                                                                        \
 		hipLaunchKernelGGL((tracePersistent<ANY, S, ST>), dim3(g.blocks), dim3(kBlock), 0, stream, bvh, rays,      \
-			uint32_t(n), perm, opt.countDev, hits, occ, gstack, lds, kXcds, refillMin(), uint32_t(chunk), stats); \
+			uint32_t(n), perm, opt.countDev, hits, occ, gstack, lds, kXcds, refillMin(opt), uint32_t(chunk), stats); \
 	}
 		if(stats) {
 			if(spill) RDU_PERSIST(true, true) else RDU_PERSIST(false, true)
@@ -456,7 +469,6 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 // of 64 (wavefront v takes batches v, v + W, v + 2W, ...). At any moment a die's wavefronts
 // work on one window of about 64 W neighbouring rays, which keeps the part of the tree they
 // touch small enough for that die's own L2. Every wavefront exits when its sequence is done.
-constexpr int kNeedRoot = 0x7ffffffe; // state: test the next root box
 
 // Closest-hit queries also keep each pushed child's slab entry in LDS (fp16 rounded towards -inf,
 // so never above the true entry): a popped child whose entry already exceeds the current maxi
@@ -485,7 +497,7 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 	uint32_t ri = 0;
 	DevRay r = makeRay(mk(0, 0, 0), mk(0, 0, 1), 0.0f, 0.0f);
 	bool clean = true;
-	int c = 0, root = 0, kk = 0;
+	int c = 0, kk = 0;
 	float maxi = 0.0f, bt = FLT_MAX, bu = 0.0f, bv = 0.0f;
 	int best = -1;
 	auto writeResult = [&]() {
@@ -538,24 +550,31 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 					bu = bv = 0.0f;
 					st.sp = 0;
 					kk = 0;
-					root = int(bvh.nRoots);
-					c = kNeedRoot;
-					has = true;
+					// The top box of the 4-wide tree (select semantics: correct for any ray).
+					float e;
+					if(slab6<true>(r, bvh.wideTopMin[0], bvh.wideTopMin[1], bvh.wideTopMin[2], bvh.wideTopMax[0],
+						   bvh.wideTopMax[1], bvh.wideTopMax[2], r.mini, maxi, e)) {
+						c = bvh.wideTopCode;
+						has = true;
+					} else {
+						pend = true; // a miss, written at the next refill
+					}
 				}
 			}
 			cursor += nIdle;
 		}
-		if(!__any(has))
-			break;
+		if(!__any(has)) {
+			// Leave only when this wavefront's sequence is exhausted too (positions only grow).
+			const uint64_t next = sliceBegin + ((cursor / chunk) * W + v) * chunk + (cursor % chunk);
+			if(next >= sliceEnd)
+				break;
+			continue;
+		}
 
 		// Fetch: one record per lane.
 		float4 a0 = make_float4(0, 0, 0, 0), a1 = a0, a2 = a0, a3 = a0, a4 = a0, a5 = a0, a6 = a0;
 		if(has) {
-			if(c == kNeedRoot) {
-				const int rr = root - 1;
-				a0 = bvh.roots[2 * rr];
-				a1 = bvh.roots[2 * rr + 1];
-			} else if(c >= 0) {
+			if(c >= 0) {
 				const float4 * nd = bvh.wide + 8 * size_t(c);
 				a0 = nd[0];
 				a1 = nd[1];
@@ -576,19 +595,7 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 		// Process.
 		const bool exact = !__all(clean);
 		bool next = false; // pop, or go to the next root, or finish
-		if(c == kNeedRoot) {
-			--root;
-			float e;
-			const bool hitRoot = exact ? slab6<true>(r, a0.x, a0.y, a0.z, a1.x, a1.y, a1.z, r.mini, maxi, e)
-									   : slab6<false>(r, a0.x, a0.y, a0.z, a1.x, a1.y, a1.z, r.mini, maxi, e);
-			if(hitRoot) {
-				c = __float_as_int(a1.w);
-				st.sp = 0;
-				kk = 0;
-			} else if(root == 0) {
-				next = true;
-			}
-		} else if(c >= 0) {
+		if(c >= 0) {
 			if(STATS)
 				++nodeVisits;
 			int c0 = __float_as_int(a6.x), c1 = __float_as_int(a6.y), c2 = __float_as_int(a6.z), c3 = __float_as_int(a6.w);
@@ -647,7 +654,6 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 			}
 			if(ANY && best >= 0) {
 				st.sp = 0;
-				root = 0;
 				next = true;
 			} else if(kk < (code & 1)) {
 				++kk;
@@ -664,8 +670,6 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 			if(st.sp > 0) {
 				c = st.pop<SPILL>();
 				kk = 0;
-			} else if(root > 0) {
-				c = kNeedRoot;
 			} else {
 				// Done: the result is written at the ray's own index at the next refill.
 				has = false;

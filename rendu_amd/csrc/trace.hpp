@@ -17,6 +17,9 @@ struct DevBVH {
 	uint32_t maxDepth = 0;
 	uint32_t wideDepth = 0;
 	float sceneMin[3] = {0, 0, 0}, sceneMax[3] = {0, 0, 0};
+	// Single root of the 4-wide tree: a top level over the mesh roots when there are several.
+	float wideTopMin[3] = {0, 0, 0}, wideTopMax[3] = {0, 0, 0};
+	int32_t wideTopCode = 0;
 };
 
 struct DevRay {
@@ -74,6 +77,7 @@ struct TraceOptions {
 	const uint32_t * perm = nullptr; // optional ray permutation: lane i handles ray perm[i]
 	const uint32_t * countDev = nullptr; // optional device count: only the first *countDev entries
 	bool spatialOrder = false; // perm is a spatial (Morton) order: interleave wavefronts finely
+	uint32_t refillMin = 0;    // idle lanes before a wavefront refills (0 = default for the order)
 };
 
 // Launchers. rays: device rdu_ray array (float4 pairs); outputs are device pointers, indexed by

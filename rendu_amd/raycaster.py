@@ -151,6 +151,22 @@ class Raycaster:
         check(fn(self._h, ctypes.c_void_p(rays_ptr), n, ctypes.c_void_p(out_ptr), flags,
                  ctypes.c_void_p(stream) if stream else None))
 
+    def autotune(self, rays=None, n: int = 0, rays_ptr: int = 0, any_hit: bool = False, incoherent: bool = False,
+                 stream: int = 0) -> dict:
+        """Pick the refill threshold for this class of batch by timing candidates (rdu_raycaster_autotune).
+        Pass host `rays` (rdu_ray records) or a device pointer `rays_ptr` with its count `n`."""
+        rep = _lib.TuneReport()
+        flags = _flags(False, incoherent)
+        if rays is not None:
+            rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+            ptr, count = vptr(rays), rays.shape[0]
+        else:
+            ptr, count = ctypes.c_void_p(rays_ptr), n
+            flags |= _lib.RDU_DEVICE_POINTERS
+        check(lib.rdu_raycaster_autotune(self._h, ptr, count, int(any_hit), flags,
+                                         ctypes.c_void_p(stream) if stream else None, ctypes.byref(rep)))
+        return {k: getattr(rep, k) for k, _ in _lib.TuneReport._fields_}
+
     def trace_stats(self, rays: np.ndarray, any_hit: bool = False, reference_order: bool = False):
         """(internal-node expansions, triangle tests) summed over a batch, counted on the GPU."""
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)

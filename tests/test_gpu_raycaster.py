@@ -181,6 +181,18 @@ def test_many_roots_and_tiny_meshes():
     compare_near_first(ours.intersect_batch(rays), ref.intersect(rays, "closest"))
 
 
+def test_autotune_keeps_results(mixed, mixed_rays):
+    ours, ref = mixed
+    log_case("autotune incoherent closest + any, then compare")
+    rep = ours.autotune(mixed_rays, incoherent=True)
+    assert rep["refill_min"] in (4, 8, 16, 24, 32, 40, 48, 64) and rep["best_ms"] > 0
+    rep = ours.autotune(mixed_rays, any_hit=True, incoherent=True)
+    assert rep["refill_min"] > 0
+    exp = ref.intersect(mixed_rays, "closest")
+    compare_near_first(ours.intersect_batch(mixed_rays, incoherent=True), exp)
+    assert np.array_equal(ours.intersect_any_batch(mixed_rays, incoherent=True), exp["hit"].astype(bool))
+
+
 def test_empty_batch(mixed):
     ours, _ = mixed
     assert ours.intersect_batch(np.zeros(0, _lib.RAY_DTYPE)).shape == (0,)
