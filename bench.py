@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1 << 23, help="rays in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--main-only", action="store_true", help="only the timed incoherent closest-hit passes")
+    ap.add_argument("--refill", type=int, default=0,
+                    help="fixed refill threshold instead of autotuning (profiling runs: one kernel configuration)")
     return ap.parse_args()
 
 
@@ -106,7 +108,11 @@ def main():
         wall = time.perf_counter() - w0
         return wall, float(np.mean([a.elapsed_time(b) for a, b in ev]))
 
-    tune = rc.autotune(rays_ptr=d_inc.data_ptr(), n=N_HALF, incoherent=True, stream=sh)
+    if args.refill:
+        os.environ["RDU_REFILL_MIN"] = str(args.refill)
+        tune = {"refill_min": args.refill, "fixed": True}
+    else:
+        tune = rc.autotune(rays_ptr=d_inc.data_ptr(), n=N_HALF, incoherent=True, stream=sh)
     wall, k_ms = timed(lambda: closest(d_inc), args.steps, args.warmup)
     value = args.steps * N_HALF / wall / 1e6
     if args.main_only:  # profiling runs: the timed incoherent closest-hit passes only
