@@ -59,7 +59,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X; how the path "
-              "would shard over several GPUs is described in DESIGN.md", file=sys.stderr)
+              "would shard is a design note of the repository, not a measurement", file=sys.stderr)
         sys.exit(2)
 
     import torch
