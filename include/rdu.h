@@ -261,6 +261,37 @@ typedef struct {
 int rdu_raycaster_autotune(rdu_raycaster* rc, const rdu_ray* rays, size_t n, int any_hit, uint32_t flags, void* stream,
 	rdu_tune_report* report);
 
+/* Kernel timing (no reference counterpart; for measurement). While enabled, each batch query
+ * records a pair of HIP events on its stream around the traversal kernel alone (not the ray
+ * ordering or copies), for up to 4096 launches. rdu_raycaster_kernel_time waits for the recorded
+ * launches, returns their summed duration and count, and starts a new count. */
+int rdu_raycaster_set_kernel_timing(rdu_raycaster* rc, int enable);
+int rdu_raycaster_kernel_time(rdu_raycaster* rc, double* total_ms, uint64_t* launches);
+
+/* ------------------------------------------------------------------------------------------------
+ * Learned radiance cache (no Rendu counterpart; an extension of the path tracer): a 16-level hash
+ * grid (2^19 entries x 2 fp16 features per level) over the scene box, followed by a 5 x 64 ReLU MLP
+ * to RGB radiance, trained online with bf16 MFMA kernels accumulating in fp32 (Adam, relative L2).
+ * ------------------------------------------------------------------------------------------------ */
+typedef struct rdu_rcache rdu_rcache;
+#define RDU_RCACHE_MLP_PARAMS 19456      /* W0 64x32, W1..W4 64x64, W5 16x64 (rows >= 3 zero) */
+#define RDU_RCACHE_TABLE_PARAMS 16777216 /* 16 levels x 2^19 entries x 2 features */
+
+int rdu_rcache_create(rdu_rcache** out, const float scene_min[3], const float scene_max[3], uint32_t seed);
+void rdu_rcache_destroy(rdu_rcache* cache);
+int rdu_rcache_set_learning_rate(rdu_rcache* cache, float lr);
+/* positions: n x 4 floats (xyz, pad); radiance: n x 4 floats out. flags: RDU_DEVICE_POINTERS. */
+int rdu_rcache_infer(rdu_rcache* cache, const float* positions, size_t n, float* radiance, uint32_t flags, void* stream);
+/* steps optimisation steps on one batch of (position, target radiance) pairs (n x 4 floats each). */
+int rdu_rcache_train(rdu_rcache* cache, const float* positions, const float* targets, size_t n, uint32_t steps,
+	uint32_t flags, void* stream, float* mean_loss);
+/* Gradients of the mean relative L2 loss for one batch, without an update (host arrays):
+ * mlp_grad RDU_RCACHE_MLP_PARAMS floats, table_grad RDU_RCACHE_TABLE_PARAMS floats or NULL. */
+int rdu_rcache_gradients(rdu_rcache* cache, const float* positions, const float* targets, size_t n, float* mlp_grad,
+	float* table_grad);
+int rdu_rcache_get_params(const rdu_rcache* cache, float* mlp, float* tables, int32_t* level_resolutions);
+int rdu_rcache_set_params(rdu_rcache* cache, const float* mlp, const float* tables);
+
 /* Last error message of the calling thread ("" if none). */
 const char* rdu_last_error(void);
 

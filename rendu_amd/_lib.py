@@ -74,6 +74,8 @@ def _load() -> ctypes.CDLL:
         "rdu_raycaster_trace_stats": (c_int, [c_void_p, c_void_p, c_size_t, c_int, c_uint32,
                                               POINTER(c_uint64), POINTER(c_uint64)]),
         "rdu_raycaster_autotune": (c_int, [c_void_p, c_void_p, c_size_t, c_int, c_uint32, c_void_p, c_void_p]),
+        "rdu_raycaster_set_kernel_timing": (c_int, [c_void_p, c_int]),
+        "rdu_raycaster_kernel_time": (c_int, [c_void_p, POINTER(ctypes.c_double), POINTER(c_uint64)]),
         "rdu_scene_create": (c_int, [POINTER(c_void_p)]),
         "rdu_scene_destroy": (None, [c_void_p]),
         "rdu_scene_add_texture": (c_int, [c_void_p, c_void_p, POINTER(ctypes.c_int32)]),
@@ -87,6 +89,14 @@ def _load() -> ctypes.CDLL:
         "rdu_pathtracer_render": (c_int, [c_void_p, c_void_p, c_size_t, c_size_t, POINTER(c_float), c_uint32, c_uint32,
                                           c_uint32, c_void_p]),
         "rdu_scene_last_render_stats": (c_int, [c_void_p, c_void_p]),
+        "rdu_rcache_create": (c_int, [POINTER(c_void_p), fp, fp, c_uint32]),
+        "rdu_rcache_destroy": (None, [c_void_p]),
+        "rdu_rcache_set_learning_rate": (c_int, [c_void_p, c_float]),
+        "rdu_rcache_infer": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_uint32, c_void_p]),
+        "rdu_rcache_train": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_uint32, c_uint32, c_void_p, fp]),
+        "rdu_rcache_gradients": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, fp, fp]),
+        "rdu_rcache_get_params": (c_int, [c_void_p, fp, fp, POINTER(ctypes.c_int32)]),
+        "rdu_rcache_set_params": (c_int, [c_void_p, fp, fp]),
         "rdu_last_error": (c_char_p, []),
         "rdu_version": (c_char_p, []),
     }

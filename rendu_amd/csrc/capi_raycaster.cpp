@@ -101,11 +101,34 @@ int query(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, void * out, 
 		opt.perm = static_cast<const uint32_t *>(rc->perm.ptr);
 		opt.spatialOrder = true;
 	}
+	hipEvent_t tStart = nullptr, tStop = nullptr;
+	if(rc->timing && rc->timedLaunches < rdu_raycaster::kMaxTimed) {
+		if(rc->timingEvents.size() < 2 * (rc->timedLaunches + 1)) {
+			hipEvent_t a, b;
+			if((st = hipCheck(hipEventCreate(&a), "hipEventCreate")))
+				return st;
+			if((st = hipCheck(hipEventCreate(&b), "hipEventCreate"))) {
+				(void)hipEventDestroy(a);
+				return st;
+			}
+			rc->timingEvents.push_back(a);
+			rc->timingEvents.push_back(b);
+		}
+		tStart = rc->timingEvents[2 * rc->timedLaunches];
+		tStop = rc->timingEvents[2 * rc->timedLaunches + 1];
+		if((st = hipCheck(hipEventRecord(tStart, s), "hipEventRecord")))
+			return st;
+	}
 	const int e = ANY ? launchAny(rc->dev, dRays, n, static_cast<uint8_t *>(dOut), opt, static_cast<int *>(rc->gstack.ptr),
 							  dstats, s)
 					  : launchClosest(rc->dev, dRays, n, dOut, opt, static_cast<int *>(rc->gstack.ptr), dstats, s);
 	if((st = hipCheck(hipError_t(e), "trace kernel launch")))
 		return st;
+	if(tStop) {
+		if((st = hipCheck(hipEventRecord(tStop, s), "hipEventRecord")))
+			return st;
+		rc->timedLaunches += 1;
+	}
 	if(!device) {
 		if((st = hipCheck(hipMemcpyAsync(out, dOut, outBytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync D2H")))
 			return st;
@@ -371,6 +394,39 @@ int rdu_raycaster_autotune(rdu_raycaster * rc, const rdu_ray * rays, size_t n, i
 		report->default_ms = defaultMs / 2.0f;
 		report->candidates = uint32_t(sizeof(candidates) / sizeof(candidates[0]));
 	}
+	return RDU_OK;
+}
+
+int rdu_raycaster_set_kernel_timing(rdu_raycaster * rc, int enable) {
+	clearError();
+	if(!rc)
+		return fail(RDU_ERR_INVALID, "null raycaster");
+	std::lock_guard<std::mutex> guard(rc->lock);
+	rc->timing = enable != 0;
+	rc->timedLaunches = 0;
+	return RDU_OK;
+}
+
+int rdu_raycaster_kernel_time(rdu_raycaster * rc, double * total_ms, uint64_t * launches) {
+	clearError();
+	if(!rc || !total_ms || !launches)
+		return fail(RDU_ERR_INVALID, "null argument");
+	std::lock_guard<std::mutex> guard(rc->lock);
+	int st = setDevice(rc->device);
+	if(st)
+		return st;
+	double sum = 0.0;
+	for(size_t i = 0; i < rc->timedLaunches; ++i) {
+		float ms = 0.0f;
+		if((st = hipCheck(hipEventSynchronize(rc->timingEvents[2 * i + 1]), "hipEventSynchronize")) ||
+			(st = hipCheck(hipEventElapsedTime(&ms, rc->timingEvents[2 * i], rc->timingEvents[2 * i + 1]),
+				 "hipEventElapsedTime")))
+			return st;
+		sum += ms;
+	}
+	*total_ms = sum;
+	*launches = rc->timedLaunches;
+	rc->timedLaunches = 0;
 	return RDU_OK;
 }
 

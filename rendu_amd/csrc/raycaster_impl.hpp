@@ -6,6 +6,7 @@
 #include "../../include/rdu.h"
 
 #include <mutex>
+#include <vector>
 
 struct rdu_raycaster {
 	rdu::HostScene scene;
@@ -20,4 +21,14 @@ struct rdu_raycaster {
 	mutable rdu::DevBuffer keys, keysAlt, perm, permAlt, sortTemp; // incoherent-batch ordering
 	// Autotuned refill threshold per [incoherent][any-hit] class (0 = not tuned).
 	uint32_t tunedRefill[2][2] = {{0, 0}, {0, 0}};
+	// Optional timing of the traversal kernel alone: event pairs recorded on the query's stream
+	// around each traversal launch, read back and summed by rdu_raycaster_kernel_time.
+	static constexpr size_t kMaxTimed = 4096;
+	bool timing = false;
+	mutable std::vector<hipEvent_t> timingEvents; // 2 per launch
+	mutable size_t timedLaunches = 0;
+	~rdu_raycaster() {
+		for(hipEvent_t e : timingEvents)
+			(void)hipEventDestroy(e);
+	}
 };

@@ -1,0 +1,419 @@
+// Radiance cache kernels for gfx950: hash-grid encoding, fused 5 x 64 MLP forward / backward on
+// bf16 MFMA with fp32 accumulation, split-K weight-gradient GEMMs and Adam.
+//
+// MFMA mapping (v_mfma_f32_16x16x16_bf16, one wavefront): D[16x16] += A[16x16] B[16x16] with
+//   A[m][k] in lane m + 16 * (k / 4), element k % 4;   B[k][n] in lane n + 16 * (k / 4), element k % 4;
+//   D[m][n] in lane n + 16 * (m / 4), element m % 4.
+// A wavefront evaluates 16 samples at a time with layers computed transposed, Y^T = W X^T:
+// A = weight tile (m = output feature, k = input feature), B = activations (k = feature,
+// n = sample). The D layout of output tile t (lane = sample + 16 g, element e: feature 16 t + 4 g + e)
+// is exactly the B layout of input chunk t of the next layer, so activations stay in registers
+// between layers. The backward pass uses the same structure with transposed weight tiles.
+#include "rcache.hpp"
+
+namespace rdu {
+
+namespace {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4v4 __attribute__((ext_vector_type(4)));
+
+constexpr int kRcBlock = 256; // 4 wavefronts
+constexpr int kOffW[6] = {0, kRcW0, kRcW0 + kRcWh, kRcW0 + 2 * kRcWh, kRcW0 + 3 * kRcWh, kRcW0 + 4 * kRcWh};
+constexpr int kInDim[6] = {kRcIn, kRcWidth, kRcWidth, kRcWidth, kRcWidth, kRcWidth};
+constexpr int kOutDim[6] = {kRcWidth, kRcWidth, kRcWidth, kRcWidth, kRcWidth, kRcOutPad};
+// Row offsets of the per-sample buffers (feature-major).
+constexpr int kXRow[6] = {0, 32, 96, 160, 224, 288}; // X0..X5
+constexpr int kGRow[6] = {0, 64, 128, 192, 256, 320}; // G1..G5, GY
+
+__device__ __forceinline__ uint16_t f2bf(float x) {
+	uint32_t u = __float_as_uint(x);
+	if((u & 0x7fffffffu) > 0x7f800000u)
+		return 0x7fc0;
+	u += 0x7fffu + ((u >> 16) & 1u); // round to nearest even
+	return uint16_t(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+__device__ __forceinline__ float h2f(uint16_t h) { return float(__builtin_bit_cast(_Float16, h)); }
+__device__ __forceinline__ uint16_t f2h(float x) { return __builtin_bit_cast(uint16_t, _Float16(x)); }
+
+__device__ __forceinline__ s4 pack4(const float v[4]) {
+	s4 r;
+	r.x = short(f2bf(v[0]));
+	r.y = short(f2bf(v[1]));
+	r.z = short(f2bf(v[2]));
+	r.w = short(f2bf(v[3]));
+	return r;
+}
+__device__ __forceinline__ float lane4(const s4 & v, int e) {
+	const short s = e == 0 ? v.x : (e == 1 ? v.y : (e == 2 ? v.z : v.w));
+	return bf2f(uint16_t(s));
+}
+
+__device__ __forceinline__ f4v4 mfma(s4 a, s4 b, f4v4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+
+// A operand: W[mt*16 + m][kt*16 + 4g .. +3] (row-major [out][in], in = ld).
+__device__ __forceinline__ s4 loadA(const uint16_t * W, int ld, int mt, int kt, int lane) {
+	const int m = mt * 16 + (lane & 15), k = kt * 16 + 4 * (lane >> 4);
+	const uint16_t * p = W + m * ld + k;
+	s4 r;
+	r.x = short(p[0]);
+	r.y = short(p[1]);
+	r.z = short(p[2]);
+	r.w = short(p[3]);
+	return r;
+}
+// A operand of W^T: A[m][k] = W[k][m], m = input feature (tile mt), k = output feature (tile kt).
+__device__ __forceinline__ s4 loadAT(const uint16_t * W, int ld, int mt, int kt, int lane) {
+	const int m = mt * 16 + (lane & 15), k = kt * 16 + 4 * (lane >> 4);
+	s4 r;
+	r.x = short(W[(k + 0) * ld + m]);
+	r.y = short(W[(k + 1) * ld + m]);
+	r.z = short(W[(k + 2) * ld + m]);
+	r.w = short(W[(k + 3) * ld + m]);
+	return r;
+}
+
+// Hash-grid features of one level at normalised position p (trilinear over 8 corners).
+__device__ __forceinline__ void cornerSetup(const RcGrid & grid, int level, const float p[3], int c, uint32_t & idx,
+	float & w) {
+	const int res = grid.res[level];
+	int cell[3];
+	float fr[3];
+	for(int a = 0; a < 3; ++a) {
+		const float x = p[a] * float(res);
+		int ci = int(floorf(x));
+		ci = ci < 0 ? 0 : (ci > res - 1 ? res - 1 : ci);
+		cell[a] = ci;
+		fr[a] = x - float(ci);
+	}
+	const int dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
+	const uint32_t cx = uint32_t(cell[0] + dx), cy = uint32_t(cell[1] + dy), cz = uint32_t(cell[2] + dz);
+	const float wx = dx ? fr[0] : 1.0f - fr[0];
+	const float wy = dy ? fr[1] : 1.0f - fr[1];
+	const float wz = dz ? fr[2] : 1.0f - fr[2];
+	w = (wx * wy) * wz;
+	if(grid.dense[level]) {
+		const uint32_t r1 = uint32_t(res + 1);
+		idx = cx + cy * r1 + cz * r1 * r1;
+	} else {
+		idx = (cx * 1u ^ cy * 2654435761u ^ cz * 805459861u) & (kRcTable - 1u);
+	}
+}
+
+__device__ __forceinline__ void encodeLevel(const RcGrid & grid, const uint16_t * t16, int level, const float p[3],
+	float & f0, float & f1) {
+	f0 = 0.0f;
+	f1 = 0.0f;
+	const uint16_t * tl = t16 + size_t(level) * kRcTable * kRcFeatures;
+	for(int c = 0; c < 8; ++c) {
+		uint32_t idx;
+		float w;
+		cornerSetup(grid, level, p, c, idx, w);
+		const uint32_t pair = reinterpret_cast<const uint32_t *>(tl)[idx]; // 2 fp16 features
+		f0 += w * h2f(uint16_t(pair & 0xffffu));
+		f1 += w * h2f(uint16_t(pair >> 16));
+	}
+}
+
+__device__ __forceinline__ void scatterLevel(const RcGrid & grid, float * gt, int level, const float p[3], float g0, float g1) {
+	float * gl = gt + size_t(level) * kRcTable * kRcFeatures;
+	for(int c = 0; c < 8; ++c) {
+		uint32_t idx;
+		float w;
+		cornerSetup(grid, level, p, c, idx, w);
+		atomicAdd(gl + 2 * idx, w * g0);
+		atomicAdd(gl + 2 * idx + 1, w * g1);
+	}
+}
+
+__device__ __forceinline__ void normalisePos(const RcGrid & grid, float4 q, float p[3]) {
+	const float v[3] = {q.x, q.y, q.z};
+	for(int a = 0; a < 3; ++a) {
+		const float t = (v[a] - grid.lo[a]) * grid.inv[a];
+		p[a] = t > 0.0f ? (t < 1.0f ? t : 1.0f) : 0.0f; // NaN -> 0
+	}
+}
+
+__device__ __forceinline__ void loadWeights(uint16_t * W, const uint16_t * w16) {
+	for(int i = threadIdx.x; i < kRcMlpParams / 4; i += kRcBlock)
+		reinterpret_cast<uint2 *>(W)[i] = reinterpret_cast<const uint2 *>(w16)[i];
+}
+
+// Fused per-sample pass. TRAIN: forward, relative-L2 loss gradient, backward to the encoding
+// (scattered into the table gradients), activations and output gradients stored for the
+// weight-gradient GEMMs. Otherwise: forward only, radiance written to `out`.
+template<bool TRAIN>
+__global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t * __restrict__ w16,
+	const uint16_t * __restrict__ t16, const float4 * __restrict__ pos, const float4 * __restrict__ targets,
+	const uint32_t * __restrict__ countDev, uint32_t n, float4 * __restrict__ out, RcTrainBuffers tb,
+	float * __restrict__ gt, float lossScale) {
+	__shared__ uint16_t W[kRcMlpParams];
+	loadWeights(W, w16);
+	__syncthreads();
+	if(countDev)
+		n = *countDev < n ? *countDev : n;
+	const int lane = threadIdx.x & 63;
+	const int q = lane & 15, g = lane >> 4;
+	const uint32_t waveGlobal = blockIdx.x * (kRcBlock / 64) + (threadIdx.x >> 6);
+	const uint32_t waves = gridDim.x * (kRcBlock / 64);
+	const uint32_t nPad = (n + 15u) & ~15u;
+	for(uint32_t base = waveGlobal * 16u; base < nPad; base += waves * 16u) {
+		const uint32_t idx = base + uint32_t(q);
+		const bool valid = idx < n;
+		float p[3] = {0.0f, 0.0f, 0.0f};
+		if(valid)
+			normalisePos(grid, pos[idx], p);
+		// Encoding: this lane computes levels 2g, 2g+1 (chunk 0) and 8+2g, 9+2g (chunk 1).
+		s4 X[6][4];
+		{
+			float e[4];
+			encodeLevel(grid, t16, 2 * g, p, e[0], e[1]);
+			encodeLevel(grid, t16, 2 * g + 1, p, e[2], e[3]);
+			X[0][0] = pack4(e);
+			encodeLevel(grid, t16, 8 + 2 * g, p, e[0], e[1]);
+			encodeLevel(grid, t16, 9 + 2 * g, p, e[2], e[3]);
+			X[0][1] = pack4(e);
+		}
+		// Hidden layers 0..4 (ReLU).
+#pragma unroll
+		for(int l = 0; l < 5; ++l) {
+			const uint16_t * Wl = W + kOffW[l];
+			const int kts = kInDim[l] / 16;
+#pragma unroll
+			for(int mt = 0; mt < 4; ++mt) {
+				f4v4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+				for(int kt = 0; kt < kts; ++kt)
+					acc = mfma(loadA(Wl, kInDim[l], mt, kt, lane), X[l][kt], acc);
+				const float r[4] = {acc.x > 0.0f ? acc.x : 0.0f, acc.y > 0.0f ? acc.y : 0.0f, acc.z > 0.0f ? acc.z : 0.0f,
+					acc.w > 0.0f ? acc.w : 0.0f};
+				X[l + 1][mt] = pack4(r);
+			}
+		}
+		// Output layer (rows 0..2 used).
+		f4v4 Y = {0.0f, 0.0f, 0.0f, 0.0f};
+		for(int kt = 0; kt < 4; ++kt)
+			Y = mfma(loadA(W + kOffW[5], kRcWidth, 0, kt, lane), X[5][kt], Y);
+		if(!TRAIN) {
+			if(valid && g == 0)
+				out[idx] = make_float4(Y.x, Y.y, Y.z, 0.0f);
+			continue;
+		}
+		// Relative L2 loss gradient on the three outputs: d/dy (y - t)^2 / (sg(y)^2 + 0.01).
+		float gy[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+		if(valid && g == 0) {
+			const float4 t = targets[idx];
+			const float y3[3] = {Y.x, Y.y, Y.z}, t3[3] = {t.x, t.y, t.z};
+			float loss = 0.0f;
+			for(int j = 0; j < 3; ++j) {
+				const float den = y3[j] * y3[j] + 0.01f;
+				const float d = y3[j] - t3[j];
+				gy[j] = 2.0f * d / den * lossScale;
+				loss += d * d / den;
+			}
+			tb.loss[idx] = loss / 3.0f;
+		}
+		s4 G[6][4]; // G[l][t]: gradient w.r.t. the pre-activation of layer l's output, tile t
+		G[5][0] = pack4(gy);
+		// Back through the output layer: dX5 = W5^T GY, then through each hidden layer.
+#pragma unroll
+		for(int l = 5; l >= 1; --l) {
+			const uint16_t * Wl = W + kOffW[l];
+			const int kts = kOutDim[l] / 16; // K of the transposed product
+#pragma unroll
+			for(int mt = 0; mt < 4; ++mt) {
+				f4v4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+				for(int kt = 0; kt < kts; ++kt)
+					acc = mfma(loadAT(Wl, kInDim[l], mt, kt, lane), G[l][kt], acc);
+				// ReLU derivative: X[l][mt] > 0 (post-activation of layer l - 1).
+				float r[4];
+				const float a4[4] = {acc.x, acc.y, acc.z, acc.w};
+				for(int e = 0; e < 4; ++e)
+					r[e] = lane4(X[l][mt], e) > 0.0f ? a4[e] : 0.0f;
+				G[l - 1][mt] = pack4(r);
+			}
+		}
+		// dX0 = W0^T G0 (32 encoding features), scattered into the tables.
+		for(int mt = 0; mt < 2; ++mt) {
+			f4v4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+			for(int kt = 0; kt < 4; ++kt)
+				acc = mfma(loadAT(W, kRcIn, mt, kt, lane), G[0][kt], acc);
+			if(valid) {
+				const int l0 = 8 * mt + 2 * g;
+				scatterLevel(grid, gt, l0, p, acc.x, acc.y);
+				scatterLevel(grid, gt, l0 + 1, p, acc.z, acc.w);
+			}
+		}
+		// Store activations X0..X5 and gradients G0..G4 (of W0..W4's outputs) and GY (bf16,
+		// feature-major) for the weight gradients dW_l = G_l X_l^T.
+		const uint32_t cap = tb.capacity;
+		for(int l = 0; l < 6; ++l) {
+			const int tiles = l == 0 ? 2 : 4;
+			for(int t = 0; t < tiles; ++t)
+				for(int e = 0; e < 4; ++e) {
+					const int f = 16 * t + 4 * g + e;
+					const short xs = e == 0 ? X[l][t].x : (e == 1 ? X[l][t].y : (e == 2 ? X[l][t].z : X[l][t].w));
+					tb.x[size_t(kXRow[l] + f) * cap + idx] = uint16_t(xs);
+				}
+		}
+		for(int l = 0; l < 6; ++l) {
+			const int tiles = l == 5 ? 1 : 4;
+			for(int t = 0; t < tiles; ++t)
+				for(int e = 0; e < 4; ++e) {
+					const int f = 16 * t + 4 * g + e;
+					const short gs = e == 0 ? G[l][t].x : (e == 1 ? G[l][t].y : (e == 2 ? G[l][t].z : G[l][t].w));
+					tb.g[size_t(kGRow[l] + f) * cap + idx] = valid ? uint16_t(gs) : uint16_t(0);
+				}
+		}
+	}
+}
+
+// dW_l[out][in] += sum over samples of G_l[out][s] X_l[in][s]: one 16x16 tile per wavefront over a
+// range of samples (split K), accumulated with atomics.
+__global__ __launch_bounds__(kRcBlock) void rcWeightGrad(RcTrainBuffers tb, uint32_t nPad, uint32_t chunk, float * gw) {
+	// Tile list: layer 0: 4x2, layers 1..4: 4x4 each, layer 5: 1x4 -> 8 + 64 + 4 = 76 tiles.
+	const int lane = threadIdx.x & 63;
+	const uint32_t wave = blockIdx.x * (kRcBlock / 64) + (threadIdx.x >> 6);
+	const uint32_t tile = wave % 76u, part = wave / 76u;
+	int layer, mt, nt;
+	if(tile < 8) {
+		layer = 0;
+		mt = int(tile) / 2;
+		nt = int(tile) % 2;
+	} else if(tile < 72) {
+		layer = 1 + int(tile - 8) / 16;
+		mt = int((tile - 8) % 16) / 4;
+		nt = int((tile - 8) % 16) % 4;
+	} else {
+		layer = 5;
+		mt = 0;
+		nt = int(tile - 72);
+	}
+	const uint32_t s0 = part * chunk;
+	if(s0 >= nPad)
+		return;
+	const uint32_t s1 = s0 + chunk < nPad ? s0 + chunk : nPad;
+	const uint32_t cap = tb.capacity;
+	const uint16_t * Gr = tb.g + size_t(kGRow[layer] + mt * 16 + (lane & 15)) * cap;
+	const uint16_t * Xr = tb.x + size_t(kXRow[layer] + nt * 16 + (lane & 15)) * cap;
+	f4v4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+	for(uint32_t s = s0; s < s1; s += 16) {
+		const uint32_t k = s + 4u * uint32_t(lane >> 4);
+		const uint2 ga = *reinterpret_cast<const uint2 *>(Gr + k);
+		const uint2 xb = *reinterpret_cast<const uint2 *>(Xr + k);
+		s4 a, b;
+		a.x = short(ga.x & 0xffffu);
+		a.y = short(ga.x >> 16);
+		a.z = short(ga.y & 0xffffu);
+		a.w = short(ga.y >> 16);
+		b.x = short(xb.x & 0xffffu);
+		b.y = short(xb.x >> 16);
+		b.z = short(xb.y & 0xffffu);
+		b.w = short(xb.y >> 16);
+		acc = mfma(a, b, acc);
+	}
+	// D[m][n]: lane n + 16 (m / 4), element m % 4 -> gw[layer][mt*16 + m][nt*16 + n].
+	const int ld = kInDim[layer];
+	float * base = gw + kOffW[layer];
+	const int n = nt * 16 + (lane & 15);
+	const float a4[4] = {acc.x, acc.y, acc.z, acc.w};
+	for(int e = 0; e < 4; ++e) {
+		const int m = mt * 16 + 4 * (lane >> 4) + e;
+		if(layer == 5 && m >= kRcOut)
+			continue;
+		atomicAdd(base + m * ld + n, a4[e]);
+	}
+}
+
+// Adam on one parameter array; refreshes the low-precision copy and clears the gradient.
+template<bool BF16>
+__global__ void rcAdam(float * w, uint16_t * w16, float * gsum, float * m, float * v, size_t n, float lr, float b1, float b2,
+	float eps, float l2, float c1, float c2) {
+	const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if(i >= n)
+		return;
+	float gr = gsum[i] + l2 * w[i];
+	const float mi = b1 * m[i] + (1.0f - b1) * gr;
+	const float vi = b2 * v[i] + (1.0f - b2) * gr * gr;
+	m[i] = mi;
+	v[i] = vi;
+	const float upd = lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+	const float wi = w[i] - upd;
+	w[i] = wi;
+	w16[i] = BF16 ? f2bf(wi) : f2h(wi);
+	gsum[i] = 0.0f;
+}
+
+template<bool BF16>
+__global__ void rcCopy(const float * w, uint16_t * w16, size_t n) {
+	const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if(i < n)
+		w16[i] = BF16 ? f2bf(w[i]) : f2h(w[i]);
+}
+
+unsigned fusedBlocks(uint32_t n) {
+	const uint32_t groups = (n + 15u) / 16u;
+	uint32_t blocks = (groups + 3u) / 4u;
+	return blocks < 1 ? 1u : (blocks > 4096u ? 4096u : blocks);
+}
+
+int weightGradients(RcParams & p, RcTrainBuffers & tb, uint32_t n, hipStream_t stream) {
+	const uint32_t nPad = (n + 15u) & ~15u;
+	const uint32_t chunk = 1024;
+	const uint32_t parts = (nPad + chunk - 1) / chunk;
+	const uint32_t waves = 76u * parts;
+	hipLaunchKernelGGL(rcWeightGrad, dim3((waves + 3) / 4), dim3(kRcBlock), 0, stream, tb, nPad, chunk, p.gw);
+	return int(hipGetLastError());
+}
+
+} // namespace
+
+int rcInfer(const RcGrid & grid, const RcParams & p, const float4 * positions, const uint32_t * countDev, uint32_t n,
+	float4 * out, hipStream_t stream) {
+	if(n == 0)
+		return 0;
+	RcTrainBuffers none;
+	hipLaunchKernelGGL((rcFused<false>), dim3(fusedBlocks(n)), dim3(kRcBlock), 0, stream, grid, p.w16, p.t16, positions,
+		nullptr, countDev, n, out, none, nullptr, 0.0f);
+	return int(hipGetLastError());
+}
+
+int rcGradients(const RcGrid & grid, RcParams & p, RcTrainBuffers & tb, const float4 * positions, const float4 * targets,
+	uint32_t n, hipStream_t stream) {
+	if(n == 0 || n > tb.capacity)
+		return int(hipErrorInvalidValue);
+	const float lossScale = 1.0f / (3.0f * float(n));
+	hipLaunchKernelGGL((rcFused<true>), dim3(fusedBlocks(n)), dim3(kRcBlock), 0, stream, grid, p.w16, p.t16, positions,
+		targets, nullptr, n, nullptr, tb, p.gt, lossScale);
+	int e = int(hipGetLastError());
+	if(e)
+		return e;
+	return weightGradients(p, tb, n, stream);
+}
+
+int rcTrainStep(const RcGrid & grid, RcParams & p, RcTrainBuffers & tb, RcAdam & adam, const float4 * positions,
+	const float4 * targets, uint32_t n, hipStream_t stream) {
+	int e = rcGradients(grid, p, tb, positions, targets, n, stream);
+	if(e)
+		return e;
+	adam.step += 1;
+	const float c1 = 1.0f - powf(adam.beta1, float(adam.step));
+	const float c2 = 1.0f - powf(adam.beta2, float(adam.step));
+	hipLaunchKernelGGL((rcAdam<true>), dim3((kRcMlpParams + 255) / 256), dim3(256), 0, stream, p.w32, p.w16, p.gw, p.mw,
+		p.vw, size_t(kRcMlpParams), adam.lr, adam.beta1, adam.beta2, adam.eps, adam.l2, c1, c2);
+	hipLaunchKernelGGL((rcAdam<false>), dim3(unsigned((kRcTableParams + 255) / 256)), dim3(256), 0, stream, p.t32, p.t16,
+		p.gt, p.mt, p.vt, kRcTableParams, adam.lr, adam.beta1, adam.beta2, adam.eps, 0.0f, c1, c2);
+	return int(hipGetLastError());
+}
+
+int rcSyncCopies(RcParams & p, hipStream_t stream) {
+	hipLaunchKernelGGL((rcCopy<true>), dim3((kRcMlpParams + 255) / 256), dim3(256), 0, stream, p.w32, p.w16,
+		size_t(kRcMlpParams));
+	hipLaunchKernelGGL((rcCopy<false>), dim3(unsigned((kRcTableParams + 255) / 256)), dim3(256), 0, stream, p.t32, p.t16,
+		kRcTableParams);
+	return int(hipGetLastError());
+}
+
+} // namespace rdu

@@ -167,6 +167,16 @@ class Raycaster:
                                          ctypes.c_void_p(stream) if stream else None, ctypes.byref(rep)))
         return {k: getattr(rep, k) for k, _ in _lib.TuneReport._fields_}
 
+    def set_kernel_timing(self, enable: bool) -> None:
+        """Record HIP events around each traversal kernel launch (rdu_raycaster_set_kernel_timing)."""
+        check(lib.rdu_raycaster_set_kernel_timing(self._h, int(enable)))
+
+    def kernel_time(self):
+        """(summed traversal-kernel milliseconds, launches) since the last call; waits for them."""
+        ms, k = ctypes.c_double(0.0), ctypes.c_uint64(0)
+        check(lib.rdu_raycaster_kernel_time(self._h, ctypes.byref(ms), ctypes.byref(k)))
+        return ms.value, k.value
+
     def trace_stats(self, rays: np.ndarray, any_hit: bool = False, reference_order: bool = False):
         """(internal-node expansions, triangle tests) summed over a batch, counted on the GPU."""
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)

@@ -35,3 +35,8 @@ def pytest_collection_modifyitems(config, items):
 def log_case(msg: str) -> None:
     """Name the case on stdout (flushed) before a GPU step, so a failing run says which case it was."""
     print(f"[case] {msg}", flush=True)
+
+
+@pytest.fixture(name="log_case")
+def _log_case_fixture():
+    return log_case

@@ -1,0 +1,159 @@
+"""Radiance cache on the GPU against the CPU reference (oracle/rcache_ref.py): inference, encoding
+closed form, gradients of the fused MFMA backward pass and the weight-gradient GEMMs, training."""
+import numpy as np
+import pytest
+
+from oracle import rcache_ref as R
+
+pytestmark = pytest.mark.gpu
+
+LO, HI = np.array([-1.0, -0.5, 0.0], np.float32), np.array([1.0, 1.5, 3.0], np.float32)
+
+
+def _cache(seed=0):
+    from rendu_amd.rcache import RadianceCache
+    return RadianceCache(LO, HI, seed=seed)
+
+
+def _positions(n, seed):
+    rng = np.random.default_rng(seed)
+    return (LO + rng.random((n, 3)) * (HI - LO)).astype(np.float32)
+
+
+def _set_random(cache, seed, table_scale=1.0):
+    rng = np.random.default_rng(seed)
+    mlp = np.concatenate([rng.uniform(-1, 1, r * c).astype(np.float32) * np.sqrt(6.0 / c)
+                          for r, c in R.LAYER_SHAPES])
+    mlp[-(16 - 3) * 64:] = 0.0  # padded output rows
+    tables = (rng.uniform(-1, 1, R.LEVELS * R.TABLE * 2) * table_scale).astype(np.float32)
+    cache.set_params(mlp, tables)
+    return mlp, tables
+
+
+def test_initial_params(log_case):
+    log_case("rcache initial params")
+    c = _cache(3)
+    mlp, tables, res = c.get_params()
+    assert list(res) == list(R.level_resolutions())
+    assert np.abs(tables).max() <= 1e-4 and np.abs(tables).max() > 0
+    assert np.all(mlp[-(16 - 3) * 64:] == 0)
+    w0 = mlp[:64 * 32]
+    assert np.abs(w0).max() <= np.sqrt(6 / 32) and w0.std() > 0.1
+
+
+def test_infer_matches_reference(log_case):
+    log_case("rcache inference vs reference, n=5000")
+    c = _cache(1)
+    mlp, tables = _set_random(c, 11)
+    _, _, res = c.get_params()
+    pos = _positions(5000, 12)
+    y = c.infer(pos)
+    ref = R.CacheReference(mlp, tables, res, LO, HI)
+    yr = ref.forward(pos).detach().numpy()
+    scale = np.abs(yr).mean()
+    err = np.abs(y - yr)
+    # bf16 activations: a rounding that flips between the two accumulation orders moves one
+    # activation by 2^-8 relative; the output error stays a small fraction of its scale.
+    assert err.mean() <= 5e-3 * scale, (err.mean(), scale)
+    assert err.max() <= 5e-2 * np.abs(yr).max(), (err.max(), np.abs(yr).max())
+
+
+def test_infer_ragged_and_device_pointers(log_case):
+    import torch
+    log_case("rcache inference ragged sizes, device pointers")
+    c = _cache(2)
+    _set_random(c, 21)
+    pos = _positions(4099, 22)
+    full = c.infer(pos)
+    for n in (1, 15, 17, 1000):
+        np.testing.assert_array_equal(c.infer(pos[:n]), full[:n])
+    p4 = np.concatenate([pos, np.zeros((pos.shape[0], 1), np.float32)], axis=1)
+    dp = torch.from_numpy(p4).cuda()
+    do = torch.zeros_like(dp)
+    c.infer_device(dp.data_ptr(), pos.shape[0], do.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(do.cpu().numpy()[:, :3], full)
+
+
+def test_encoding_closed_form(log_case):
+    """MLP routing one encoding feature to the output; tables linear on a dense level."""
+    log_case("rcache encoding closed form")
+    c = _cache(4)
+    res = R.level_resolutions()
+    lvl = 3
+    r1 = int(res[lvl]) + 1
+    tables = np.zeros((R.LEVELS, R.TABLE, 2), np.float32)
+    ci = np.arange(r1 ** 3)
+    cx, cy, cz = ci % r1, (ci // r1) % r1, ci // (r1 * r1)
+    tables[lvl, ci, 0] = 0.25 + (cx + 2 * cy + 3 * cz) / 64.0
+    mlp = np.zeros(19456, np.float32)
+    W = []
+    off = 0
+    for r, cc in R.LAYER_SHAPES:
+        W.append(mlp[off:off + r * cc].reshape(r, cc))
+        off += r * cc
+    W[0][0, 2 * lvl] = 1.0
+    for l in range(1, 6):
+        W[l][0, 0] = 1.0
+    c.set_params(mlp, tables.reshape(-1))
+    pos = _positions(3000, 5)
+    y = c.infer(pos)
+    p = (pos.astype(np.float64) - LO) / (HI - LO)
+    x = np.clip(p, 0, 1) * float(res[lvl])
+    expect = 0.25 + (x[:, 0] + 2 * x[:, 1] + 3 * x[:, 2]) / 64.0
+    np.testing.assert_allclose(y[:, 0], expect, rtol=2 ** -8)
+    assert np.all(y[:, 1:] == 0)
+
+
+def test_gradients_match_autograd(log_case):
+    log_case("rcache gradients vs autograd, n=3000")
+    c = _cache(5)
+    mlp, tables = _set_random(c, 31, table_scale=0.5)
+    _, _, res = c.get_params()
+    pos = _positions(3000, 32)
+    rng = np.random.default_rng(33)
+    tgt = rng.random((3000, 3)).astype(np.float32)
+    gm, gt = c.gradients(pos, tgt)
+    ref = R.CacheReference(mlp, tables, res, LO, HI)
+    _, rm, rt = ref.gradients(pos, tgt)
+    off = 0
+    for l, (r, cc) in enumerate(R.LAYER_SHAPES):
+        a, b = gm[off:off + r * cc], rm[off:off + r * cc]
+        if l == 5:
+            assert np.all(a[3 * 64:] == 0)
+            a, b = a[:3 * 64], b[:3 * 64]
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        # Output gradients and pre-activation gradients are carried in bf16 (8-bit significand).
+        assert rel <= 3e-2, (l, rel)
+        off += r * cc
+    rel_t = np.linalg.norm(gt - rt) / np.linalg.norm(rt)
+    assert rel_t <= 3e-2, rel_t
+    # No gradient lands on an entry the reference leaves untouched.
+    assert np.abs(gt[rt == 0]).max() <= 1e-6 * np.abs(rt).max()
+    # No stale gradients: a second call returns the same values.
+    gm2, _ = c.gradients(pos, tgt, tables=False)
+    assert np.linalg.norm(gm2 - gm) <= 1e-5 * np.linalg.norm(gm)
+
+
+def _field(pos):
+    p = (pos - LO) / (HI - LO)
+    r = 0.6 + 0.4 * np.sin(6 * p[:, 0]) * np.cos(4 * p[:, 1])
+    g = 0.3 + p[:, 2] ** 2
+    b = 0.2 + 0.5 * p[:, 0] * p[:, 1]
+    return np.stack([r, g, b], axis=1).astype(np.float32)
+
+
+def test_training_fits_smooth_field(log_case):
+    log_case("rcache training on a smooth field, 400 steps x 16384")
+    c = _cache(6)
+    test_pos = _positions(8192, 99)
+    test_tgt = _field(test_pos)
+    err0 = np.mean((c.infer(test_pos) - test_tgt) ** 2)
+    rng = np.random.default_rng(7)
+    losses = []
+    for it in range(400):
+        pos = (LO + rng.random((16384, 3)) * (HI - LO)).astype(np.float32)
+        losses.append(c.train(pos, _field(pos)))
+    err1 = np.mean((c.infer(test_pos) - test_tgt) ** 2)
+    assert losses[-1] < 0.05 * losses[0], (losses[0], losses[-1])
+    assert err1 < 0.01 * err0 and err1 < 1e-3, (err0, err1)
