@@ -269,28 +269,82 @@ int rdu_raycaster_set_kernel_timing(rdu_raycaster* rc, int enable);
 int rdu_raycaster_kernel_time(rdu_raycaster* rc, double* total_ms, uint64_t* launches);
 
 /* ------------------------------------------------------------------------------------------------
- * Learned radiance cache (no Rendu counterpart; an extension of the path tracer): a 16-level hash
- * grid (2^19 entries x 2 fp16 features per level) over the scene box, followed by a 5 x 64 ReLU MLP
- * to RGB radiance, trained online with bf16 MFMA kernels accumulating in fp32 (Adam, relative L2).
+ * Output step of the path-tracer app: replaces the tonemap loop of src/apps/pathtracer/main.cpp:109-116
+ * and the 8-bit conversion of Image::saveAsLDR (src/engine/resources/Image.cpp:126-158). pixels:
+ * width x height x components floats, row-major, row 0 on top; out: the same count of bytes.
+ * RDU_IMAGE_TONEMAP: c -> 1 - exp(-c) on the colour channels; RDU_IMAGE_SRGB: v -> pow(v, 1/2.2)
+ * except alpha; then byte = (uint8)min(255, max(0, 255 v)); RDU_IMAGE_IGNORE_ALPHA: alpha byte 255.
+ * flags may add RDU_DEVICE_POINTERS (then `stream` orders the work and nothing is synchronised).
+ * ------------------------------------------------------------------------------------------------ */
+#define RDU_IMAGE_TONEMAP 0x10u
+#define RDU_IMAGE_SRGB 0x20u
+#define RDU_IMAGE_IGNORE_ALPHA 0x40u
+int rdu_image_to_ldr(const float* pixels, size_t width, size_t height, size_t components, uint8_t* out, uint32_t flags,
+	void* stream);
+
+/* ------------------------------------------------------------------------------------------------
+ * Learned radiance cache (no Rendu counterpart; an extension of the path tracer): outgoing RGB
+ * radiance at a position in a direction, from a 16-level hash grid over the scene box (2^19 entries x
+ * 2 fp16 features per level) and degree-3 spherical harmonics of the direction (16 features),
+ * followed by a 5 x 64 ReLU MLP, trained online with bf16 MFMA kernels accumulating in fp32 (Adam,
+ * relative L2 loss). A query is 8 floats laid out like rdu_ray: position xyz, unused, outgoing
+ * direction xyz, unused.
  * ------------------------------------------------------------------------------------------------ */
 typedef struct rdu_rcache rdu_rcache;
-#define RDU_RCACHE_MLP_PARAMS 19456      /* W0 64x32, W1..W4 64x64, W5 16x64 (rows >= 3 zero) */
+#define RDU_RCACHE_MLP_PARAMS 20480      /* W0 64x48, W1..W4 64x64, W5 16x64 (rows >= 3 zero) */
 #define RDU_RCACHE_TABLE_PARAMS 16777216 /* 16 levels x 2^19 entries x 2 features */
 
 int rdu_rcache_create(rdu_rcache** out, const float scene_min[3], const float scene_max[3], uint32_t seed);
 void rdu_rcache_destroy(rdu_rcache* cache);
 int rdu_rcache_set_learning_rate(rdu_rcache* cache, float lr);
-/* positions: n x 4 floats (xyz, pad); radiance: n x 4 floats out. flags: RDU_DEVICE_POINTERS. */
-int rdu_rcache_infer(rdu_rcache* cache, const float* positions, size_t n, float* radiance, uint32_t flags, void* stream);
-/* steps optimisation steps on one batch of (position, target radiance) pairs (n x 4 floats each). */
-int rdu_rcache_train(rdu_rcache* cache, const float* positions, const float* targets, size_t n, uint32_t steps,
+/* queries: n x 8 floats; radiance: n x 4 floats out (rgb, unused). flags: RDU_DEVICE_POINTERS. */
+int rdu_rcache_infer(rdu_rcache* cache, const float* queries, size_t n, float* radiance, uint32_t flags, void* stream);
+/* steps optimisation steps on one batch of (query, target radiance) pairs (n x 8 and n x 4 floats). */
+int rdu_rcache_train(rdu_rcache* cache, const float* queries, const float* targets, size_t n, uint32_t steps,
 	uint32_t flags, void* stream, float* mean_loss);
 /* Gradients of the mean relative L2 loss for one batch, without an update (host arrays):
  * mlp_grad RDU_RCACHE_MLP_PARAMS floats, table_grad RDU_RCACHE_TABLE_PARAMS floats or NULL. */
-int rdu_rcache_gradients(rdu_rcache* cache, const float* positions, const float* targets, size_t n, float* mlp_grad,
+int rdu_rcache_gradients(rdu_rcache* cache, const float* queries, const float* targets, size_t n, float* mlp_grad,
 	float* table_grad);
 int rdu_rcache_get_params(const rdu_rcache* cache, float* mlp, float* tables, int32_t* level_resolutions);
 int rdu_rcache_set_params(rdu_rcache* cache, const float* mlp, const float* tables);
+
+/* Path tracing with the cache (no Rendu counterpart). As rdu_pathtracer_render, except that each
+ * pass of paths:
+ *  - traces one pixel in 16 (a 4 x 4 pattern shifted every pass) in full, recording every surface
+ *    vertex from bounce `terminate_bounce` on; after the pass each record becomes a training pair
+ *    (query at the vertex towards the previous one, target = radiance the rest of the path brought
+ *    back / throughput on arrival, clamped to [0, 100]); 1 record in 8 is held out;
+ *  - ends every other path at its first surface vertex from `terminate_bounce` on whose coarse cell
+ *    (32^3 over the cache's box) the cache is trusted, adding throughput x cache prediction;
+ *  - then runs `train_steps` Adam steps of `train_batch` pairs (none when `frozen`), and updates
+ *    the cells' trust from the held-out pairs: a cell is trusted once it has `min_count` held-out
+ *    pairs (running sums halved every pass) and |sum(prediction) - sum(target)| in luminance is at
+ *    most `max_bias` of the larger sum. Untrusted cells are traced in full, as without the cache.
+ * Zero fields take the defaults: terminate_bounce 2, train_steps 4, train_batch 65536, max_bias 0.2,
+ * min_count 8. `always_trust` skips the cell test (every eligible vertex ends in the cache); with
+ * `frozen` as well nothing is recorded and no pixel is traced in full. The estimator options (RR,
+ * seeds) are those of `opts`. */
+typedef struct {
+	uint32_t terminate_bounce;
+	uint32_t train_steps;
+	uint32_t train_batch;
+	float max_bias;
+	float min_count;
+	uint32_t frozen;
+	uint32_t always_trust;
+} rdu_cache_opts;
+typedef struct {
+	uint64_t cache_queries; /* paths ended in the cache */
+	uint64_t records;       /* training-path vertices recorded */
+	uint64_t train_pairs;   /* pairs available for training (records minus held out) */
+	uint64_t train_steps;   /* optimisation steps run */
+	uint32_t trusted_cells; /* cells trusted at the end of the render */
+} rdu_cache_stats;
+int rdu_pathtracer_render_cached(rdu_scene* scene, rdu_rcache* cache, const rdu_camera_desc* camera, size_t samples,
+	size_t depth, float* image, uint32_t width, uint32_t height, uint32_t components, const rdu_render_opts* opts,
+	const rdu_cache_opts* cache_opts);
+int rdu_scene_last_cache_stats(const rdu_scene* scene, rdu_cache_stats* out);
 
 /* Last error message of the calling thread ("" if none). */
 const char* rdu_last_error(void);

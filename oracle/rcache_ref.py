@@ -12,10 +12,14 @@ read as fp16, encoding and every hidden activation rounded to bf16 (round to nea
 as bf16, products accumulated in fp32. Gradients come from torch autograd on that function (the
 roundings pass gradients straight through), independent of the GPU's hand-written backward pass.
 
+The outgoing direction enters as the 16 real spherical harmonics of degrees 0..3 (inputs 32..47),
+as in the neural radiance cache's direction encoding.
+
 Pinning (tests/test_oracle_rcache.py): trilinear interpolation reproduces a linear function of the
 corner coordinates exactly on a dense level; the hash agrees with arbitrary-precision integer
-arithmetic; autograd agrees with central finite differences in float64; the relative L2 loss
-treats its denominator as a constant (finite differences with the denominator frozen).
+arithmetic; the harmonics are orthonormal on the sphere (quadrature); autograd agrees with central
+finite differences in float64; the relative L2 loss treats its denominator as a constant (finite
+differences with the denominator frozen).
 """
 from __future__ import annotations
 
@@ -26,7 +30,46 @@ LEVELS = 16
 FEATURES = 2
 TABLE = 1 << 19
 PRIMES = (1, 2654435761, 805459861)
-LAYER_SHAPES = [(64, 32), (64, 64), (64, 64), (64, 64), (64, 64), (16, 64)]
+LAYER_SHAPES = [(64, 48), (64, 64), (64, 64), (64, 64), (64, 64), (16, 64)]
+
+# Real spherical harmonics, degrees 0..3, with the Condon-Shortley phase: the standard normalised
+# constants sqrt((2l+1)/(4 pi) (l-|m|)!/(l+|m|)!) times the usual factors (for example Sloan,
+# "Stupid Spherical Harmonics Tricks", 2008, appendix A2).
+SH_C = {
+    "c0": 0.28209479177387814, "c1": 0.4886025119029199, "c2a": 1.0925484305920792,
+    "c2b": 0.31539156525252005, "c2c": 0.5462742152960396, "c3a": 0.5900435899266435,
+    "c3b": 2.890611442640554, "c3c": 0.4570457994644658, "c3d": 0.3731763325901154,
+    "c3e": 1.445305721320277,
+}
+
+
+def unit_dirs(directions: np.ndarray) -> np.ndarray:
+    """Directions normalised as the GPU does (zero or non-finite length -> +z), float32."""
+    d = np.asarray(directions, np.float32)[:, :3]
+    with np.errstate(invalid="ignore", over="ignore", divide="ignore"):
+        l2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        ok = (l2 > 0) & (l2 < np.float32(3.0e38))
+        inv = np.float32(1.0) / np.sqrt(np.where(ok, l2, np.float32(1.0)))
+        out = d * inv[:, None]
+    out[~ok] = np.float32([0.0, 0.0, 1.0])
+    return out.astype(np.float32)
+
+
+def sh_features(directions: np.ndarray) -> np.ndarray:
+    """(n, 16) float32 SH values of the unit directions, in the GPU's order and operation order."""
+    d = unit_dirs(directions)
+    x, y, z = d[:, 0], d[:, 1], d[:, 2]
+    f = lambda v: np.float32(v)
+    xx, yy, zz = x * x, y * y, z * z
+    c = {k: f(v) for k, v in SH_C.items()}
+    cols = [
+        np.full_like(x, c["c0"]), -c["c1"] * y, c["c1"] * z, -c["c1"] * x,
+        c["c2a"] * (x * y), -c["c2a"] * (y * z), c["c2b"] * (f(3) * zz - f(1)), -c["c2a"] * (x * z),
+        c["c2c"] * (xx - yy), -c["c3a"] * (y * (f(3) * xx - yy)), c["c3b"] * ((x * y) * z),
+        -c["c3c"] * (y * (f(5) * zz - f(1))), c["c3d"] * (z * (f(5) * zz - f(3))),
+        -c["c3c"] * (x * (f(5) * zz - f(1))), c["c3e"] * (z * (xx - yy)), -c["c3a"] * (x * (xx - f(3) * yy)),
+    ]
+    return np.stack(cols, axis=1).astype(np.float32)
 
 
 def level_resolutions() -> np.ndarray:
@@ -118,8 +161,11 @@ class CacheReference:
             feats.append(acc)
         return torch.cat(feats, dim=1)                               # (n, 32): level l -> 2l, 2l+1
 
-    def forward(self, positions: np.ndarray) -> torch.Tensor:
-        x = self.encode(positions)
+    def forward(self, positions: np.ndarray, directions: np.ndarray | None = None) -> torch.Tensor:
+        if directions is None:
+            directions = np.tile(np.float32([0, 0, 1]), (positions.shape[0], 1))
+        sh = torch.from_numpy(sh_features(directions)).to(self.dtype)
+        x = torch.cat([self.encode(positions), sh], dim=1)          # (n, 48)
         if self.round:
             x = bf16(x)
         for l in range(5):
@@ -128,18 +174,18 @@ class CacheReference:
                 x = bf16(x)
         return (x @ self.w[5].t())[:, :3]
 
-    def loss(self, positions: np.ndarray, targets: np.ndarray) -> torch.Tensor:
+    def loss(self, positions: np.ndarray, targets: np.ndarray, directions: np.ndarray | None = None) -> torch.Tensor:
         """Mean over samples and channels of (y - t)^2 / (sg(y)^2 + 0.01)."""
-        y = self.forward(positions)
+        y = self.forward(positions, directions)
         t = torch.tensor(np.asarray(targets, np.float32)[:, :3], dtype=self.dtype)
         den = (y.detach() ** 2 + 0.01)
         return ((y - t) ** 2 / den).mean()
 
-    def gradients(self, positions: np.ndarray, targets: np.ndarray):
+    def gradients(self, positions: np.ndarray, targets: np.ndarray, directions: np.ndarray | None = None):
         for w in self.w:
             w.grad = None
         self.t.grad = None
-        L = self.loss(positions, targets)
+        L = self.loss(positions, targets, directions)
         L.backward()
         mlp = np.concatenate([w.grad.detach().numpy().reshape(-1) for w in self.w]).astype(np.float32)
         tab = self.t.grad.detach().numpy().reshape(-1).astype(np.float32)

@@ -97,6 +97,9 @@ def _load() -> ctypes.CDLL:
         "rdu_rcache_gradients": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, fp, fp]),
         "rdu_rcache_get_params": (c_int, [c_void_p, fp, fp, POINTER(ctypes.c_int32)]),
         "rdu_rcache_set_params": (c_int, [c_void_p, fp, fp]),
+        "rdu_pathtracer_render_cached": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_size_t, POINTER(c_float),
+                                                 c_uint32, c_uint32, c_uint32, c_void_p, c_void_p]),
+        "rdu_scene_last_cache_stats": (c_int, [c_void_p, c_void_p]),
         "rdu_last_error": (c_char_p, []),
         "rdu_version": (c_char_p, []),
     }

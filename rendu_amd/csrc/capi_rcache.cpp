@@ -15,6 +15,7 @@ struct rdu_rcache {
 	rdu::RcTrainBuffers tb;
 	rdu::RcAdam adam;
 	rdu::DevBuffer w32, w16, gw, mw, vw, t32, t16, gt, mt, vt, tx, tg, tloss, stagePos, stageTgt, stageOut;
+	rdu::DevBuffer cells; // per coarse cell trust statistics of the path tracer's cache mode
 	int device = -1;
 };
 
@@ -27,7 +28,8 @@ int ensureTrain(rdu_rcache * c, uint32_t n) {
 	if(cap <= c->tb.capacity)
 		return RDU_OK;
 	int st;
-	if((st = c->tx.reserve(size_t(352) * cap * 2)) || (st = c->tg.reserve(size_t(336) * cap * 2)) ||
+	if((st = c->tx.reserve(size_t(kRcIn + 5 * kRcWidth) * cap * 2)) ||
+		(st = c->tg.reserve(size_t(5 * kRcWidth + kRcOutPad) * cap * 2)) ||
 		(st = c->tloss.reserve(size_t(cap) * 4)))
 		return st;
 	c->tb.x = static_cast<uint16_t *>(c->tx.ptr);
@@ -37,16 +39,16 @@ int ensureTrain(rdu_rcache * c, uint32_t n) {
 	return RDU_OK;
 }
 
-// Copies host inputs when the caller gives host pointers.
-int stage(DevBuffer & b, const float * src, size_t n, bool device, const float4 ** out, hipStream_t s) {
+// Copies host inputs (`bytes` of them) when the caller gives host pointers.
+int stage(DevBuffer & b, const float * src, size_t bytes, bool device, const float4 ** out, hipStream_t s) {
 	if(device) {
 		*out = reinterpret_cast<const float4 *>(src);
 		return RDU_OK;
 	}
-	int st = b.reserve(n * 16);
+	int st = b.reserve(bytes);
 	if(st)
 		return st;
-	if((st = hipCheck(hipMemcpyAsync(b.ptr, src, n * 16, hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D")))
+	if((st = hipCheck(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D")))
 		return st;
 	*out = static_cast<const float4 *>(b.ptr);
 	return RDU_OK;
@@ -103,11 +105,12 @@ int rdu_rcache_create(rdu_rcache ** out, const float scene_min[3], const float s
 	if((st = c->w32.upload(w.data(), w.size() * 4)) || (st = c->w16.reserve(w.size() * 2)) ||
 		(st = c->gw.reserve(w.size() * 4)) || (st = c->mw.reserve(w.size() * 4)) || (st = c->vw.reserve(w.size() * 4)) ||
 		(st = c->t32.upload(tab.data(), tab.size() * 4)) || (st = c->t16.reserve(tab.size() * 2)) ||
-		(st = c->gt.reserve(tab.size() * 4)) || (st = c->mt.reserve(tab.size() * 4)) || (st = c->vt.reserve(tab.size() * 4))) {
+		(st = c->gt.reserve(tab.size() * 4)) || (st = c->mt.reserve(tab.size() * 4)) || (st = c->vt.reserve(tab.size() * 4)) ||
+		(st = c->cells.reserve(size_t(kRcCellRes) * kRcCellRes * kRcCellRes * 16))) {
 		delete c;
 		return st;
 	}
-	for(DevBuffer * b : {&c->gw, &c->mw, &c->vw, &c->gt, &c->mt, &c->vt})
+	for(DevBuffer * b : {&c->gw, &c->mw, &c->vw, &c->gt, &c->mt, &c->vt, &c->cells})
 		if((st = hipCheck(hipMemset(b->ptr, 0, b->bytes), "hipMemset"))) {
 			delete c;
 			return st;
@@ -141,9 +144,9 @@ int rdu_rcache_set_learning_rate(rdu_rcache * c, float lr) {
 	return RDU_OK;
 }
 
-int rdu_rcache_infer(rdu_rcache * c, const float * positions, size_t n, float * radiance, uint32_t flags, void * stream) {
+int rdu_rcache_infer(rdu_rcache * c, const float * queries, size_t n, float * radiance, uint32_t flags, void * stream) {
 	clearError();
-	if(!c || (n && (!positions || !radiance)))
+	if(!c || (n && (!queries || !radiance)))
 		return fail(RDU_ERR_INVALID, "null argument");
 	if(n == 0)
 		return RDU_OK;
@@ -155,7 +158,7 @@ int rdu_rcache_infer(rdu_rcache * c, const float * positions, size_t n, float * 
 	hipStream_t s = static_cast<hipStream_t>(stream);
 	const bool dev = (flags & RDU_DEVICE_POINTERS) != 0;
 	const float4 * dPos;
-	if((st = stage(c->stagePos, positions, n, dev, &dPos, s)))
+	if((st = stage(c->stagePos, queries, n * 32, dev, &dPos, s)))
 		return st;
 	float4 * dOut = reinterpret_cast<float4 *>(radiance);
 	if(!dev) {
@@ -173,10 +176,10 @@ int rdu_rcache_infer(rdu_rcache * c, const float * positions, size_t n, float * 
 	return RDU_OK;
 }
 
-int rdu_rcache_train(rdu_rcache * c, const float * positions, const float * targets, size_t n, uint32_t steps,
+int rdu_rcache_train(rdu_rcache * c, const float * queries, const float * targets, size_t n, uint32_t steps,
 	uint32_t flags, void * stream, float * mean_loss) {
 	clearError();
-	if(!c || !positions || !targets || n == 0)
+	if(!c || !queries || !targets || n == 0)
 		return fail(RDU_ERR_INVALID, "null argument or empty batch");
 	if(n > (size_t(1) << 26))
 		return fail(RDU_ERR_UNSUPPORTED, "training batch above 2^26 samples");
@@ -186,7 +189,7 @@ int rdu_rcache_train(rdu_rcache * c, const float * positions, const float * targ
 	hipStream_t s = static_cast<hipStream_t>(stream);
 	const bool dev = (flags & RDU_DEVICE_POINTERS) != 0;
 	const float4 *dPos, *dTgt;
-	if((st = stage(c->stagePos, positions, n, dev, &dPos, s)) || (st = stage(c->stageTgt, targets, n, dev, &dTgt, s)) ||
+	if((st = stage(c->stagePos, queries, n * 32, dev, &dPos, s)) || (st = stage(c->stageTgt, targets, n * 16, dev, &dTgt, s)) ||
 		(st = ensureTrain(c, uint32_t(n))))
 		return st;
 	for(uint32_t k = 0; k < steps; ++k)
@@ -205,17 +208,17 @@ int rdu_rcache_train(rdu_rcache * c, const float * positions, const float * targ
 	return RDU_OK;
 }
 
-int rdu_rcache_gradients(rdu_rcache * c, const float * positions, const float * targets, size_t n, float * mlp_grad,
+int rdu_rcache_gradients(rdu_rcache * c, const float * queries, const float * targets, size_t n, float * mlp_grad,
 	float * table_grad) {
 	clearError();
-	if(!c || !positions || !targets || n == 0 || !mlp_grad)
+	if(!c || !queries || !targets || n == 0 || !mlp_grad)
 		return fail(RDU_ERR_INVALID, "null argument or empty batch");
 	int st = setDevice(c->device);
 	if(st)
 		return st;
 	const float4 *dPos, *dTgt;
-	if((st = stage(c->stagePos, positions, n, false, &dPos, nullptr)) ||
-		(st = stage(c->stageTgt, targets, n, false, &dTgt, nullptr)) || (st = ensureTrain(c, uint32_t(n))))
+	if((st = stage(c->stagePos, queries, n * 32, false, &dPos, nullptr)) ||
+		(st = stage(c->stageTgt, targets, n * 16, false, &dTgt, nullptr)) || (st = ensureTrain(c, uint32_t(n))))
 		return st;
 	if((st = hipCheck(hipMemset(c->gw.ptr, 0, c->gw.bytes), "hipMemset")) ||
 		(st = hipCheck(hipMemset(c->gt.ptr, 0, c->gt.bytes), "hipMemset")) ||
@@ -273,4 +276,6 @@ const RcGrid * rcacheGrid(const rdu_rcache * c) { return &c->grid; }
 RcParams * rcacheParams(rdu_rcache * c) { return &c->p; }
 RcTrainBuffers * rcacheTrain(rdu_rcache * c, uint32_t n) { return ensureTrain(c, n) ? nullptr : &c->tb; }
 RcAdam * rcacheAdam(rdu_rcache * c) { return &c->adam; }
+float4 * rcacheCells(rdu_rcache * c) { return static_cast<float4 *>(c->cells.ptr); }
+int rcacheDevice(const rdu_rcache * c) { return c->device; }
 } // namespace rdu

@@ -33,7 +33,11 @@ struct rdu_scene {
 	rdu::DevBuffer pRays, pHits, pAtten, pColor, pQ0, pQ1, pCounts, pTotals, pSRays, pSContrib, pSQ0, pSQ1, pOcc,
 		pSHits, pGstack, pImage;
 	size_t pathCap = 0;
+	// Radiance-cache mode buffers, grown on demand.
+	rdu::DevBuffer cQuery, cQAtten, cQPath, cPred, cRec, cRecPath, cTrainQ, cTrainT, cHoldQ, cHoldT, cCounts;
+	size_t cacheCap = 0;
 	rdu_render_stats last{};
+	rdu_cache_stats lastCache{};
 	~rdu_scene() {
 		if(rc)
 			rdu_raycaster_destroy(rc);
@@ -90,6 +94,22 @@ int ensurePaths(rdu_scene * s, size_t P) {
 	if(g && (st = s->pGstack.reserve(g * sizeof(int))))
 		return st;
 	s->pathCap = P;
+	return RDU_OK;
+}
+
+// Cache-mode buffers for passes of up to P paths with `records` training records.
+int ensureCacheBuffers(rdu_scene * s, size_t P, size_t records) {
+	if(P <= s->cacheCap && records <= s->cRecPath.bytes / 4)
+		return RDU_OK;
+	const size_t hold = records / 8 + 1;
+	const size_t pred = P > hold ? P : hold;
+	int st;
+	if((st = s->cQuery.reserve(P * 32)) || (st = s->cQAtten.reserve(P * 16)) || (st = s->cQPath.reserve(P * 4)) ||
+		(st = s->cPred.reserve(pred * 16)) || (st = s->cRec.reserve(records * 64)) || (st = s->cRecPath.reserve(records * 4)) ||
+		(st = s->cTrainQ.reserve(records * 32)) || (st = s->cTrainT.reserve(records * 16)) ||
+		(st = s->cHoldQ.reserve(hold * 32)) || (st = s->cHoldT.reserve(hold * 16)) || (st = s->cCounts.reserve(kCacheSlots * 4)))
+		return st;
+	s->cacheCap = P;
 	return RDU_OK;
 }
 
@@ -277,8 +297,9 @@ int rdu_scene_commit(rdu_scene * s) {
 
 const rdu_raycaster * rdu_scene_raycaster(const rdu_scene * s) { return s ? s->rc : nullptr; }
 
-int rdu_pathtracer_render(rdu_scene * s, const rdu_camera_desc * cam, size_t samples, size_t depth, float * image,
-	uint32_t width, uint32_t height, uint32_t components, const rdu_render_opts * opts) {
+static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples, size_t depth, float * image,
+	uint32_t width, uint32_t height, uint32_t components, const rdu_render_opts * opts, rdu_rcache * cache,
+	const rdu_cache_opts * copts) {
 	clearError();
 	const auto t0 = std::chrono::steady_clock::now();
 	if(!s)
@@ -365,13 +386,96 @@ int rdu_pathtracer_render(rdu_scene * s, const rdu_camera_desc * cam, size_t sam
 	P.shadowHits = static_cast<rdu_hit *>(s->pSHits.ptr);
 	PtScratch scratch;
 	scratch.gstack = static_cast<int *>(s->pGstack.ptr);
-	for(size_t base = 0; base < spp; base += perPass) {
+	// Radiance-cache mode set-up.
+	PtCache C;
+	rdu_cache_opts co{};
+	if(copts)
+		co = *copts;
+	const uint32_t trainSteps = co.train_steps ? co.train_steps : 4u;
+	const uint32_t trainBatch = co.train_batch ? co.train_batch : 65536u;
+	rdu_cache_stats cstats{};
+	if(cache) {
+		if(rcacheDevice(cache) != s->rc->device)
+			return fail(RDU_ERR_INVALID, "cache and scene live on different devices");
+		C.terminateBounce = int32_t(co.terminate_bounce ? co.terminate_bounce : 2u);
+		C.maxBias = co.max_bias > 0.0f ? co.max_bias : 0.2f;
+		C.minCount = co.min_count > 0.0f ? co.min_count : 8.0f;
+		C.alwaysTrust = co.always_trust ? 1u : 0u;
+		C.records = (co.frozen && co.always_trust) ? 0u : 1u;
+		const size_t passPaths = pixels * perPass;
+		const size_t tail = depth > size_t(C.terminateBounce) ? depth - size_t(C.terminateBounce) : 0;
+		// One pixel in 16 trains; the 4 x 4 pattern may hold up to ceil(w/4) ceil(h/4) pixels.
+		const size_t trainPaths = size_t((width + 3) / 4) * ((height + 3) / 4) * perPass;
+		const size_t records = trainPaths * tail + 64;
+		if(records > 0xffffffffull)
+			return fail(RDU_ERR_UNSUPPORTED, "too many training records per pass");
+		if((st = ensureCacheBuffers(s, passPaths, records)))
+			return st;
+		if(!rcacheTrain(cache, trainBatch))
+			return fail(RDU_ERR_ALLOC, "cache training buffers");
+		C.grid = *rcacheGrid(cache);
+		C.params = rcacheParams(cache);
+		C.counts = static_cast<uint32_t *>(s->cCounts.ptr);
+		C.query = static_cast<float4 *>(s->cQuery.ptr);
+		C.queryAtten = static_cast<float4 *>(s->cQAtten.ptr);
+		C.queryPath = static_cast<uint32_t *>(s->cQPath.ptr);
+		C.pred = static_cast<float4 *>(s->cPred.ptr);
+		C.rec = static_cast<float4 *>(s->cRec.ptr);
+		C.recPath = static_cast<uint32_t *>(s->cRecPath.ptr);
+		C.recCapacity = uint32_t(records);
+		C.trainQ = static_cast<float4 *>(s->cTrainQ.ptr);
+		C.trainT = static_cast<float4 *>(s->cTrainT.ptr);
+		C.holdQ = static_cast<float4 *>(s->cHoldQ.ptr);
+		C.holdT = static_cast<float4 *>(s->cHoldT.ptr);
+		C.cells = rcacheCells(cache);
+	}
+	uint32_t pass = 0;
+	for(size_t base = 0; base < spp; base += perPass, ++pass) {
 		const size_t n = (spp - base) < perPass ? (spp - base) : perPass;
+		if(cache) {
+			// Shift the training pattern every pass (and with the seed) so every pixel trains.
+			const uint32_t k = pass + opt.seed * 7u;
+			C.trainX = k & 3u;
+			C.trainY = (k >> 2) & 3u;
+		}
 		if((st = hipCheck(hipError_t(ptRenderPass(s->dev, s->rc->dev, f, P, uint32_t(base), uint32_t(n), po,
-								  static_cast<float *>(s->pImage.ptr), components, scratch, stream)),
+								  static_cast<float *>(s->pImage.ptr), components, scratch, cache ? &C : nullptr, stream)),
 				"path tracing pass")))
 			return st;
+		if(cache) {
+			uint32_t counts[kCacheSlots] = {0, 0, 0, 0};
+			if((st = hipCheck(hipMemcpyAsync(counts, C.counts, sizeof(counts), hipMemcpyDeviceToHost, stream), "counts")) ||
+				(st = hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize")))
+				return st;
+			cstats.cache_queries += counts[kCacheQueries];
+			cstats.records += counts[kCacheRecords];
+			cstats.train_pairs += counts[kCacheTrain];
+			const uint32_t avail = counts[kCacheTrain];
+			if(!co.frozen && avail > 0) {
+				RcTrainBuffers * tb = rcacheTrain(cache, trainBatch);
+				for(uint32_t k = 0; k < trainSteps; ++k) {
+					const uint32_t off = avail > trainBatch ? uint32_t((uint64_t(k) * trainBatch) % (avail - trainBatch + 1)) : 0u;
+					const uint32_t cnt = avail < trainBatch ? avail : trainBatch;
+					if((st = hipCheck(hipError_t(rcTrainStep(C.grid, *rcacheParams(cache), *tb, *rcacheAdam(cache), C.trainQ + 2 * size_t(off),
+										  C.trainT + off, cnt, stream)),
+							"cache training")))
+						return st;
+					cstats.train_steps += 1;
+				}
+			}
+		}
 	}
+	if(cache) {
+		const size_t nc = size_t(kCacheCellRes) * kCacheCellRes * kCacheCellRes;
+		std::vector<float4> cells(nc);
+		if((st = hipCheck(hipMemcpyAsync(cells.data(), C.cells, nc * 16, hipMemcpyDeviceToHost, stream), "cells")) ||
+			(st = hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize")))
+			return st;
+		for(const float4 & c : cells)
+			if(c.z >= C.minCount && std::fabs(c.x - c.y) <= C.maxBias * std::max(c.x, c.y) + 1e-3f * c.z)
+				cstats.trusted_cells += 1;
+	}
+	s->lastCache = cstats;
 	if((st = hipCheck(hipError_t(ptNormalizeImage(static_cast<float *>(s->pImage.ptr), uint32_t(pixels), components,
 							  uint32_t(spp), stream)),
 			"normalise")))
@@ -391,11 +495,34 @@ int rdu_pathtracer_render(rdu_scene * s, const rdu_camera_desc * cam, size_t sam
 	return RDU_OK;
 }
 
+int rdu_pathtracer_render(rdu_scene * s, const rdu_camera_desc * cam, size_t samples, size_t depth, float * image,
+	uint32_t width, uint32_t height, uint32_t components, const rdu_render_opts * opts) {
+	return renderImpl(s, cam, samples, depth, image, width, height, components, opts, nullptr, nullptr);
+}
+
+int rdu_pathtracer_render_cached(rdu_scene * s, rdu_rcache * cache, const rdu_camera_desc * cam, size_t samples,
+	size_t depth, float * image, uint32_t width, uint32_t height, uint32_t components, const rdu_render_opts * opts,
+	const rdu_cache_opts * copts) {
+	if(!cache) {
+		clearError();
+		return fail(RDU_ERR_INVALID, "null cache");
+	}
+	return renderImpl(s, cam, samples, depth, image, width, height, components, opts, cache, copts);
+}
+
 int rdu_scene_last_render_stats(const rdu_scene * s, rdu_render_stats * out) {
 	clearError();
 	if(!s || !out)
 		return fail(RDU_ERR_INVALID, "null argument");
 	*out = s->last;
+	return RDU_OK;
+}
+
+int rdu_scene_last_cache_stats(const rdu_scene * s, rdu_cache_stats * out) {
+	clearError();
+	if(!s || !out)
+		return fail(RDU_ERR_INVALID, "null argument");
+	*out = s->lastCache;
 	return RDU_OK;
 }
 

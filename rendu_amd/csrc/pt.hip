@@ -41,6 +41,42 @@ __device__ __forceinline__ void appendQueue(bool want, uint32_t p, uint32_t * qu
 	}
 }
 
+// Reserve one slot of an array for every lane whose `want` is set (one atomic per wavefront);
+// returns the lane's slot (meaningless where `want` is false). Called by whole wavefronts.
+__device__ __forceinline__ uint32_t appendSlot(bool want, uint32_t * count) {
+	const uint64_t m = __ballot(want);
+	if(!m)
+		return 0;
+	const uint32_t lane = __lane_id();
+	const uint32_t leader = uint32_t(__ffsll((unsigned long long)m) - 1);
+	uint32_t base = 0;
+	if(lane == leader)
+		base = atomicAdd(count, uint32_t(__popcll(m)));
+	base = __shfl(base, int(leader), 64);
+	return base + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+}
+
+// Coarse cell of a position in the cache's box.
+__device__ __forceinline__ uint32_t cacheCell(const RcGrid & g, f3 p) {
+	const float v[3] = {p.x, p.y, p.z};
+	uint32_t c[3];
+	for(int a = 0; a < 3; ++a) {
+		const float t = (v[a] - g.lo[a]) * g.inv[a] * float(kCacheCellRes);
+		const int i = t > 0.0f ? (t < float(kCacheCellRes - 1) ? int(t) : kCacheCellRes - 1) : 0;
+		c[a] = uint32_t(i);
+	}
+	return c[0] + uint32_t(kCacheCellRes) * (c[1] + uint32_t(kCacheCellRes) * c[2]);
+}
+
+__device__ __forceinline__ bool cacheTrusted(const PtCache & C, f3 p) {
+	if(C.alwaysTrust)
+		return true;
+	const float4 s = C.cells[cacheCell(C.grid, p)];
+	return s.z >= C.minCount && fabsf(s.x - s.y) <= C.maxBias * gmax(s.x, s.y) + 1e-3f * s.z;
+}
+
+__device__ __forceinline__ float luminance(float4 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
 __device__ __forceinline__ void putRay(float4 * rays, uint32_t p, f3 o, f3 d, float tmin, float tmax) {
 	rays[2 * size_t(p)] = make_float4(o.x, o.y, o.z, tmin);
 	rays[2 * size_t(p) + 1] = make_float4(d.x, d.y, d.z, tmax);
@@ -104,11 +140,14 @@ __global__ void ptShadowSwap(uint32_t * counts, unsigned long long * totals) {
 }
 
 // One bounce of PathTracer::render's loop body (PathTracer.cpp:202-286) for every queued path.
+template<bool CACHE>
 __global__ __launch_bounds__(kPtBlock) void ptShade(DevScene S, PtFrame f, PtPaths P, const uint32_t * queueIn,
-	uint32_t * queueOut, uint32_t * counts, uint32_t bounce, uint32_t sampleBase, PtOptions opt) {
+	uint32_t * queueOut, uint32_t * counts, uint32_t bounce, uint32_t sampleBase, PtOptions opt, PtCache C) {
 	const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
 	const uint32_t n = counts[kCountActive];
 	bool cont = false, shadow = false;
+	bool doQuery = false, doRecord = false; // radiance-cache mode only
+	f3 cPos{0, 0, 0}, cDir{0, 0, 0}, cColor{0, 0, 0}, cAtten{0, 0, 0};
 	uint32_t p = 0;
 	if(i < n) {
 		p = queueIn[i];
@@ -144,6 +183,20 @@ __global__ __launch_bounds__(kPtBlock) void ptShade(DevScene S, PtFrame f, PtPat
 			} else if(mat.type == RDU_MATERIAL_EMISSIVE) {
 				color = color + atten * mk(bCol.x, bCol.y, bCol.z);
 			} else {
+				bool terminate = false;
+				if(CACHE && int(bounce) >= C.terminateBounce) {
+					const uint32_t px = pix % f.width, py = pix / f.width;
+					const bool trainPixel = ((px + C.trainX) & 3u) == 0 && ((py + C.trainY) & 3u) == 0;
+					cPos = pt;
+					cDir = -rayDir;
+					cColor = color;
+					cAtten = atten;
+					if(trainPixel && C.records)
+						doRecord = true;
+					else if(cacheTrusted(C, pt))
+						doQuery = terminate = true;
+				}
+				if(!terminate) {
 				const M3 tbn = buildLocalFrame(S, obj, mat, hv, hw, hit.u, hit.v, rayDir, uv);
 				const f3 wo = normalize(mulT(tbn, -rayDir));
 				const f3 baseColor = {powf(bCol.x, 2.2f), powf(bCol.y, 2.2f), powf(bCol.z, 2.2f)};
@@ -194,12 +247,104 @@ __global__ __launch_bounds__(kPtBlock) void ptShade(DevScene S, PtFrame f, PtPat
 					P.atten[p] = make_float4(atten.x, atten.y, atten.z, 0.0f);
 					cont = true;
 				}
+				}
 			}
 		}
 		P.color[p] = make_float4(color.x, color.y, color.z, 0.0f);
 	}
 	appendQueue(shadow, p, P.shadowQueue[0], counts + kCountShadow);
 	appendQueue(cont, p, queueOut, counts + kCountNext);
+	if(CACHE) {
+		const uint32_t qs = appendSlot(doQuery, C.counts + kCacheQueries);
+		if(doQuery) {
+			C.query[2 * size_t(qs)] = make_float4(cPos.x, cPos.y, cPos.z, 0.0f);
+			C.query[2 * size_t(qs) + 1] = make_float4(cDir.x, cDir.y, cDir.z, 0.0f);
+			C.queryAtten[qs] = make_float4(cAtten.x, cAtten.y, cAtten.z, 0.0f);
+			C.queryPath[qs] = p;
+		}
+		const uint32_t rs = appendSlot(doRecord, C.counts + kCacheRecords);
+		if(doRecord && rs < C.recCapacity) {
+			C.rec[4 * size_t(rs)] = make_float4(cPos.x, cPos.y, cPos.z, 0.0f);
+			C.rec[4 * size_t(rs) + 1] = make_float4(cDir.x, cDir.y, cDir.z, 0.0f);
+			C.rec[4 * size_t(rs) + 2] = make_float4(cColor.x, cColor.y, cColor.z, 0.0f);
+			C.rec[4 * size_t(rs) + 3] = make_float4(cAtten.x, cAtten.y, cAtten.z, 0.0f);
+			C.recPath[rs] = p;
+		}
+	}
+}
+
+// Terminated paths add throughput x cache prediction.
+__global__ __launch_bounds__(kPtBlock) void ptCacheApply(PtPaths P, PtCache C) {
+	const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
+	if(i >= C.counts[kCacheQueries])
+		return;
+	const uint32_t p = C.queryPath[i];
+	const float4 a = C.queryAtten[i], l = C.pred[i], c = P.color[p];
+	P.color[p] = make_float4(c.x + a.x * gmax(l.x, 0.0f), c.y + a.y * gmax(l.y, 0.0f), c.z + a.z * gmax(l.z, 0.0f), 0.0f);
+}
+
+// Records -> (query, target) pairs: target = (final colour - colour on arrival) / throughput, per
+// channel (0 where the throughput is 0), clamped to [0, 100]; index % 8 == 7 held out.
+__global__ __launch_bounds__(kPtBlock) void ptCacheTargets(PtPaths P, PtCache C) {
+	const uint32_t r = blockIdx.x * kPtBlock + threadIdx.x;
+	const uint32_t nRec = C.counts[kCacheRecords] < C.recCapacity ? C.counts[kCacheRecords] : C.recCapacity;
+	const bool valid = r < nRec;
+	float4 q0 = {}, q1 = {}, t = {};
+	if(valid) {
+		const float4 cf = P.color[C.recPath[r]];
+		const float4 cb = C.rec[4 * size_t(r) + 2], tb = C.rec[4 * size_t(r) + 3];
+		const float d[3] = {cf.x - cb.x, cf.y - cb.y, cf.z - cb.z}, w[3] = {tb.x, tb.y, tb.z};
+		float o[3];
+		for(int k = 0; k < 3; ++k) {
+			const float v = w[k] > 0.0f ? d[k] / w[k] : 0.0f;
+			o[k] = v > 0.0f ? (v < 100.0f ? v : 100.0f) : 0.0f;
+		}
+		q0 = C.rec[4 * size_t(r)];
+		q1 = C.rec[4 * size_t(r) + 1];
+		t = make_float4(o[0], o[1], o[2], 0.0f);
+	}
+	const bool hold = valid && (r & 7u) == 7u;
+	const bool train = valid && !hold;
+	const uint32_t ts = appendSlot(train, C.counts + kCacheTrain);
+	const uint32_t hs = appendSlot(hold, C.counts + kCacheHold);
+	if(train) {
+		C.trainQ[2 * size_t(ts)] = q0;
+		C.trainQ[2 * size_t(ts) + 1] = q1;
+		C.trainT[ts] = t;
+	}
+	if(hold) {
+		C.holdQ[2 * size_t(hs)] = q0;
+		C.holdQ[2 * size_t(hs) + 1] = q1;
+		C.holdT[hs] = t;
+	}
+}
+
+__global__ void ptCacheReset(uint32_t * counts) {
+	if(threadIdx.x < kCacheSlots)
+		counts[threadIdx.x] = 0;
+}
+
+// Cell statistics: halve the running sums, then add this pass's held-out luminances.
+__global__ __launch_bounds__(kPtBlock) void ptCacheDecay(float4 * cells, uint32_t n) {
+	const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
+	if(i < n) {
+		const float4 c = cells[i];
+		cells[i] = make_float4(0.5f * c.x, 0.5f * c.y, 0.5f * c.z, 0.0f);
+	}
+}
+
+__global__ __launch_bounds__(kPtBlock) void ptCacheHeldOut(PtCache C) {
+	const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
+	if(i >= C.counts[kCacheHold])
+		return;
+	const float4 q = C.holdQ[2 * size_t(i)];
+	const float4 pr = C.pred[i];
+	const uint32_t cell = cacheCell(C.grid, mk(q.x, q.y, q.z));
+	const float lp = gmax(luminance(pr), 0.0f), lt = luminance(C.holdT[i]);
+	float * c = reinterpret_cast<float *>(C.cells + cell);
+	atomicAdd(c, lp);
+	atomicAdd(c + 1, lt);
+	atomicAdd(c + 2, 1.0f);
 }
 
 // Visible shadow rays add their contribution (any-hit result per path).
@@ -295,10 +440,13 @@ inline unsigned blocksFor(size_t n) { return unsigned((n + kPtBlock - 1) / kPtBl
 
 int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, const PtPaths & P, uint32_t sampleBase,
 	uint32_t samplesInPass, const PtOptions & opt, float * dImage, uint32_t components, PtScratch & scratch,
-	hipStream_t stream) {
+	const PtCache * cache, hipStream_t stream) {
 	const uint32_t pixels = f.width * f.height;
 	const uint32_t nPaths = pixels * samplesInPass;
+	const PtCache C = cache ? *cache : PtCache{};
 	hipLaunchKernelGGL(ptResetCounts, dim3(1), dim3(64), 0, stream, P.counts, nPaths);
+	if(cache)
+		hipLaunchKernelGGL(ptCacheReset, dim3(1), dim3(64), 0, stream, C.counts);
 	hipLaunchKernelGGL(ptGenerate, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, f, P, sampleBase, nPaths);
 	int q = 0;
 	for(uint32_t b = 0; b < opt.depth; ++b) {
@@ -309,8 +457,12 @@ int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, cons
 		int e = launchClosest(bvh, P.rays, nPaths, P.hits, to, scratch.gstack, nullptr, stream);
 		if(e)
 			return e;
-		hipLaunchKernelGGL(ptShade, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, S, f, P, P.queue[q], P.queue[1 - q],
-			P.counts, b, sampleBase, opt);
+		if(cache)
+			hipLaunchKernelGGL(ptShade<true>, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, S, f, P, P.queue[q],
+				P.queue[1 - q], P.counts, b, sampleBase, opt, C);
+		else
+			hipLaunchKernelGGL(ptShade<false>, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, S, f, P, P.queue[q],
+				P.queue[1 - q], P.counts, b, sampleBase, opt, C);
 		if(S.nLights > 0) {
 			if(!S.anyMasked) {
 				TraceOptions so;
@@ -341,8 +493,25 @@ int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, cons
 		hipLaunchKernelGGL(ptAdvance, dim3(1), dim3(64), 0, stream, P.counts, P.totals);
 		q = 1 - q;
 	}
+	if(cache) {
+		// Terminated paths: one inference over the compacted queries (count read on the device).
+		int e = rcInfer(C.grid, *C.params, C.query, C.counts + kCacheQueries, nPaths, C.pred, stream);
+		if(e)
+			return e;
+		hipLaunchKernelGGL(ptCacheApply, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, P, C);
+		hipLaunchKernelGGL(ptCacheTargets, dim3(blocksFor(C.recCapacity)), dim3(kPtBlock), 0, stream, P, C);
+	}
 	hipLaunchKernelGGL(ptAccumulate, dim3(blocksFor(pixels)), dim3(kPtBlock), 0, stream, f, P, dImage, components,
 		samplesInPass);
+	if(cache) {
+		const uint32_t cells = uint32_t(kCacheCellRes) * kCacheCellRes * kCacheCellRes;
+		const uint32_t holdCap = C.recCapacity / 8u + 1u;
+		int e = rcInfer(C.grid, *C.params, C.holdQ, C.counts + kCacheHold, holdCap, C.pred, stream);
+		if(e)
+			return e;
+		hipLaunchKernelGGL(ptCacheDecay, dim3(blocksFor(cells)), dim3(kPtBlock), 0, stream, C.cells, cells);
+		hipLaunchKernelGGL(ptCacheHeldOut, dim3(blocksFor(holdCap)), dim3(kPtBlock), 0, stream, C);
+	}
 	return int(hipGetLastError());
 }
 

@@ -1,6 +1,7 @@
 // Host-visible interface of the wavefront path tracer kernels (pt.hip).
 #pragma once
 #include "pt_device.hpp"
+#include "rcache.hpp"
 
 namespace rdu {
 
@@ -44,9 +45,42 @@ struct PtScratch {
 	int * gstack = nullptr; // traversal stack spill, globalStackInts() for the pass size
 };
 
+// Radiance-cache mode of a pass (an extension; plain tracing when absent). Paths of the pass's
+// training pixels ((x + trainX) % 4 == 0 and (y + trainY) % 4 == 0: one pixel in 16) are traced in
+// full and record every surface vertex from bounce `terminateBounce` on; at the end of the pass each
+// record becomes a (query, target radiance) pair, target = (final colour - colour on arrival) /
+// throughput on arrival. Every other path ends at its first surface vertex from that bounce on whose
+// coarse cell the cache is trusted, adding throughput x cache(position, outgoing direction).
+// Records with index % 8 == 7 are held out of training; their predictions against their targets
+// (luminance sums per cell, halved every pass) decide which cells are trusted: at least `minCount`
+// records and |sum(prediction) - sum(target)| <= maxBias * max(sums) + 1e-3 count.
+constexpr int kCacheCellRes = kRcCellRes; // cells per axis over the cache's box
+enum : int { kCacheQueries = 0, kCacheRecords = 1, kCacheTrain = 2, kCacheHold = 3, kCacheSlots = 4 };
+struct PtCache {
+	int32_t terminateBounce = 2;
+	uint32_t trainX = 0, trainY = 0;
+	float maxBias = 0.2f;
+	float minCount = 8.0f;
+	uint32_t alwaysTrust = 0;
+	uint32_t records = 1; // 0 (frozen cache, no trust test): no training pixels, no records
+	RcGrid grid{};
+	const RcParams * params = nullptr;
+	uint32_t * counts = nullptr;       // kCacheSlots
+	float4 * query = nullptr;          // terminated paths: 2 float4 (position, outgoing direction) per slot
+	float4 * queryAtten = nullptr;     // throughput per slot
+	uint32_t * queryPath = nullptr;    // path per slot
+	float4 * pred = nullptr;           // cache output per slot (also reused for held-out predictions)
+	float4 * rec = nullptr;            // records: position, outgoing direction, colour on arrival, throughput
+	uint32_t * recPath = nullptr;
+	uint32_t recCapacity = 0;
+	float4 * trainQ = nullptr, * trainT = nullptr; // compact training pairs (2 float4 + 1 float4)
+	float4 * holdQ = nullptr, * holdT = nullptr;   // held-out pairs
+	float4 * cells = nullptr;          // kCacheCellRes^3 x {sum prediction, sum target, count, unused}
+};
+
 int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, const PtPaths & P, uint32_t sampleBase,
 	uint32_t samplesInPass, const PtOptions & opt, float * dImage, uint32_t components, PtScratch & scratch,
-	hipStream_t stream);
+	const PtCache * cache, hipStream_t stream);
 int ptNormalizeImage(float * dImage, uint32_t pixels, uint32_t components, uint32_t spp, hipStream_t stream);
 
 } // namespace rdu

@@ -1,5 +1,5 @@
-// Radiance cache kernels for gfx950: hash-grid encoding, fused 5 x 64 MLP forward / backward on
-// bf16 MFMA with fp32 accumulation, split-K weight-gradient GEMMs and Adam.
+// Radiance cache kernels for gfx950: hash-grid and spherical-harmonics encodings, fused 5 x 64 MLP
+// forward / backward on bf16 MFMA with fp32 accumulation, split-K weight-gradient GEMMs and Adam.
 //
 // MFMA mapping (v_mfma_f32_16x16x16_bf16, one wavefront): D[16x16] += A[16x16] B[16x16] with
 //   A[m][k] in lane m + 16 * (k / 4), element k % 4;   B[k][n] in lane n + 16 * (k / 4), element k % 4;
@@ -23,7 +23,7 @@ constexpr int kOffW[6] = {0, kRcW0, kRcW0 + kRcWh, kRcW0 + 2 * kRcWh, kRcW0 + 3 
 constexpr int kInDim[6] = {kRcIn, kRcWidth, kRcWidth, kRcWidth, kRcWidth, kRcWidth};
 constexpr int kOutDim[6] = {kRcWidth, kRcWidth, kRcWidth, kRcWidth, kRcWidth, kRcOutPad};
 // Row offsets of the per-sample buffers (feature-major).
-constexpr int kXRow[6] = {0, 32, 96, 160, 224, 288}; // X0..X5
+constexpr int kXRow[6] = {0, 48, 112, 176, 240, 304}; // X0..X5
 constexpr int kGRow[6] = {0, 64, 128, 192, 256, 320}; // G1..G5, GY
 
 __device__ __forceinline__ uint16_t f2bf(float x) {
@@ -127,6 +127,48 @@ __device__ __forceinline__ void scatterLevel(const RcGrid & grid, float * gt, in
 	}
 }
 
+// Real spherical harmonics of degrees 0..3 of a unit direction, coefficients 4 * chunk .. + 3
+// (order: l = 0; l = 1, m = -1, 0, 1; l = 2, m = -2..2; l = 3, m = -3..3).
+__device__ __forceinline__ void shChunk(int chunk, float x, float y, float z, float e[4]) {
+	const float xx = x * x, yy = y * y, zz = z * z;
+	if(chunk == 0) {
+		e[0] = 0.28209479177387814f;
+		e[1] = -0.4886025119029199f * y;
+		e[2] = 0.4886025119029199f * z;
+		e[3] = -0.4886025119029199f * x;
+	} else if(chunk == 1) {
+		e[0] = 1.0925484305920792f * (x * y);
+		e[1] = -1.0925484305920792f * (y * z);
+		e[2] = 0.31539156525252005f * (3.0f * zz - 1.0f);
+		e[3] = -1.0925484305920792f * (x * z);
+	} else if(chunk == 2) {
+		e[0] = 0.5462742152960396f * (xx - yy);
+		e[1] = -0.5900435899266435f * (y * (3.0f * xx - yy));
+		e[2] = 2.890611442640554f * ((x * y) * z);
+		e[3] = -0.4570457994644658f * (y * (5.0f * zz - 1.0f));
+	} else {
+		e[0] = 0.3731763325901154f * (z * (5.0f * zz - 3.0f));
+		e[1] = -0.4570457994644658f * (x * (5.0f * zz - 1.0f));
+		e[2] = 1.445305721320277f * (z * (xx - yy));
+		e[3] = -0.5900435899266435f * (x * (xx - 3.0f * yy));
+	}
+}
+
+// Unit direction of a query (a zero or non-finite direction encodes as +z).
+__device__ __forceinline__ void queryDir(float4 q, float d[3]) {
+	const float l2 = (q.x * q.x + q.y * q.y) + q.z * q.z;
+	if(!(l2 > 0.0f) || !(l2 < 3.0e38f)) {
+		d[0] = 0.0f;
+		d[1] = 0.0f;
+		d[2] = 1.0f;
+		return;
+	}
+	const float inv = 1.0f / sqrtf(l2);
+	d[0] = q.x * inv;
+	d[1] = q.y * inv;
+	d[2] = q.z * inv;
+}
+
 __device__ __forceinline__ void normalisePos(const RcGrid & grid, float4 q, float p[3]) {
 	const float v[3] = {q.x, q.y, q.z};
 	for(int a = 0; a < 3; ++a) {
@@ -145,7 +187,7 @@ __device__ __forceinline__ void loadWeights(uint16_t * W, const uint16_t * w16) 
 // weight-gradient GEMMs. Otherwise: forward only, radiance written to `out`.
 template<bool TRAIN>
 __global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t * __restrict__ w16,
-	const uint16_t * __restrict__ t16, const float4 * __restrict__ pos, const float4 * __restrict__ targets,
+	const uint16_t * __restrict__ t16, const float4 * __restrict__ queries, const float4 * __restrict__ targets,
 	const uint32_t * __restrict__ countDev, uint32_t n, float4 * __restrict__ out, RcTrainBuffers tb,
 	float * __restrict__ gt, float lossScale) {
 	__shared__ uint16_t W[kRcMlpParams];
@@ -161,10 +203,13 @@ __global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t 
 	for(uint32_t base = waveGlobal * 16u; base < nPad; base += waves * 16u) {
 		const uint32_t idx = base + uint32_t(q);
 		const bool valid = idx < n;
-		float p[3] = {0.0f, 0.0f, 0.0f};
-		if(valid)
-			normalisePos(grid, pos[idx], p);
-		// Encoding: this lane computes levels 2g, 2g+1 (chunk 0) and 8+2g, 9+2g (chunk 1).
+		float p[3] = {0.0f, 0.0f, 0.0f}, d[3] = {0.0f, 0.0f, 1.0f};
+		if(valid) {
+			normalisePos(grid, queries[2 * size_t(idx)], p);
+			queryDir(queries[2 * size_t(idx) + 1], d);
+		}
+		// Encoding: this lane computes levels 2g, 2g+1 (chunk 0), 8+2g, 9+2g (chunk 1) and the
+		// spherical harmonics 4g..4g+3 (chunk 2).
 		s4 X[6][4];
 		{
 			float e[4];
@@ -174,6 +219,8 @@ __global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t 
 			encodeLevel(grid, t16, 8 + 2 * g, p, e[0], e[1]);
 			encodeLevel(grid, t16, 9 + 2 * g, p, e[2], e[3]);
 			X[0][1] = pack4(e);
+			shChunk(g, d[0], d[1], d[2], e);
+			X[0][2] = pack4(e);
 		}
 		// Hidden layers 0..4 (ReLU).
 #pragma unroll
@@ -235,7 +282,8 @@ __global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t 
 				G[l - 1][mt] = pack4(r);
 			}
 		}
-		// dX0 = W0^T G0 (32 encoding features), scattered into the tables.
+		// dX0 = W0^T G0 for the 32 hash-grid features, scattered into the tables (the spherical
+		// harmonics have no parameters).
 		for(int mt = 0; mt < 2; ++mt) {
 			f4v4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 			for(int kt = 0; kt < 4; ++kt)
@@ -250,7 +298,7 @@ __global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t 
 		// feature-major) for the weight gradients dW_l = G_l X_l^T.
 		const uint32_t cap = tb.capacity;
 		for(int l = 0; l < 6; ++l) {
-			const int tiles = l == 0 ? 2 : 4;
+			const int tiles = l == 0 ? 3 : 4;
 			for(int t = 0; t < tiles; ++t)
 				for(int e = 0; e < 4; ++e) {
 					const int f = 16 * t + 4 * g + e;
@@ -272,24 +320,25 @@ __global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t 
 
 // dW_l[out][in] += sum over samples of G_l[out][s] X_l[in][s]: one 16x16 tile per wavefront over a
 // range of samples (split K), accumulated with atomics.
+constexpr uint32_t kGradTiles = 80;
 __global__ __launch_bounds__(kRcBlock) void rcWeightGrad(RcTrainBuffers tb, uint32_t nPad, uint32_t chunk, float * gw) {
-	// Tile list: layer 0: 4x2, layers 1..4: 4x4 each, layer 5: 1x4 -> 8 + 64 + 4 = 76 tiles.
+	// Tile list: layer 0: 4x3, layers 1..4: 4x4 each, layer 5: 1x4 -> 12 + 64 + 4 = 80 tiles.
 	const int lane = threadIdx.x & 63;
 	const uint32_t wave = blockIdx.x * (kRcBlock / 64) + (threadIdx.x >> 6);
-	const uint32_t tile = wave % 76u, part = wave / 76u;
+	const uint32_t tile = wave % kGradTiles, part = wave / kGradTiles;
 	int layer, mt, nt;
-	if(tile < 8) {
+	if(tile < 12) {
 		layer = 0;
-		mt = int(tile) / 2;
-		nt = int(tile) % 2;
-	} else if(tile < 72) {
-		layer = 1 + int(tile - 8) / 16;
-		mt = int((tile - 8) % 16) / 4;
-		nt = int((tile - 8) % 16) % 4;
+		mt = int(tile) / 3;
+		nt = int(tile) % 3;
+	} else if(tile < 76) {
+		layer = 1 + int(tile - 12) / 16;
+		mt = int((tile - 12) % 16) / 4;
+		nt = int((tile - 12) % 16) % 4;
 	} else {
 		layer = 5;
 		mt = 0;
-		nt = int(tile - 72);
+		nt = int(tile - 76);
 	}
 	const uint32_t s0 = part * chunk;
 	if(s0 >= nPad)
@@ -363,29 +412,29 @@ int weightGradients(RcParams & p, RcTrainBuffers & tb, uint32_t n, hipStream_t s
 	const uint32_t nPad = (n + 15u) & ~15u;
 	const uint32_t chunk = 1024;
 	const uint32_t parts = (nPad + chunk - 1) / chunk;
-	const uint32_t waves = 76u * parts;
+	const uint32_t waves = kGradTiles * parts;
 	hipLaunchKernelGGL(rcWeightGrad, dim3((waves + 3) / 4), dim3(kRcBlock), 0, stream, tb, nPad, chunk, p.gw);
 	return int(hipGetLastError());
 }
 
 } // namespace
 
-int rcInfer(const RcGrid & grid, const RcParams & p, const float4 * positions, const uint32_t * countDev, uint32_t n,
+int rcInfer(const RcGrid & grid, const RcParams & p, const float4 * queries, const uint32_t * countDev, uint32_t n,
 	float4 * out, hipStream_t stream) {
 	if(n == 0)
 		return 0;
 	RcTrainBuffers none;
-	hipLaunchKernelGGL((rcFused<false>), dim3(fusedBlocks(n)), dim3(kRcBlock), 0, stream, grid, p.w16, p.t16, positions,
+	hipLaunchKernelGGL((rcFused<false>), dim3(fusedBlocks(n)), dim3(kRcBlock), 0, stream, grid, p.w16, p.t16, queries,
 		nullptr, countDev, n, out, none, nullptr, 0.0f);
 	return int(hipGetLastError());
 }
 
-int rcGradients(const RcGrid & grid, RcParams & p, RcTrainBuffers & tb, const float4 * positions, const float4 * targets,
+int rcGradients(const RcGrid & grid, RcParams & p, RcTrainBuffers & tb, const float4 * queries, const float4 * targets,
 	uint32_t n, hipStream_t stream) {
 	if(n == 0 || n > tb.capacity)
 		return int(hipErrorInvalidValue);
 	const float lossScale = 1.0f / (3.0f * float(n));
-	hipLaunchKernelGGL((rcFused<true>), dim3(fusedBlocks(n)), dim3(kRcBlock), 0, stream, grid, p.w16, p.t16, positions,
+	hipLaunchKernelGGL((rcFused<true>), dim3(fusedBlocks(n)), dim3(kRcBlock), 0, stream, grid, p.w16, p.t16, queries,
 		targets, nullptr, n, nullptr, tb, p.gt, lossScale);
 	int e = int(hipGetLastError());
 	if(e)
@@ -393,9 +442,9 @@ int rcGradients(const RcGrid & grid, RcParams & p, RcTrainBuffers & tb, const fl
 	return weightGradients(p, tb, n, stream);
 }
 
-int rcTrainStep(const RcGrid & grid, RcParams & p, RcTrainBuffers & tb, RcAdam & adam, const float4 * positions,
+int rcTrainStep(const RcGrid & grid, RcParams & p, RcTrainBuffers & tb, RcAdam & adam, const float4 * queries,
 	const float4 * targets, uint32_t n, hipStream_t stream) {
-	int e = rcGradients(grid, p, tb, positions, targets, n, stream);
+	int e = rcGradients(grid, p, tb, queries, targets, n, stream);
 	if(e)
 		return e;
 	adam.step += 1;

@@ -78,6 +78,27 @@ class PathTracer:
                                         ctypes.byref(opts)))
         return self.last_stats()
 
+    def render_cached(self, cache, camera, samples: int, depth: int, image: np.ndarray, seed: int = 0,
+                      estimator: int = S.ESTIMATOR_EXTENDED, rr_start: int = 3, max_paths: int = 0,
+                      terminate_bounce: int = 0, train_steps: int = 0, train_batch: int = 0, max_bias: float = 0.0,
+                      min_count: float = 0.0, frozen: bool = False, always_trust: bool = False) -> dict:
+        """render() with a rendu_amd.rcache.RadianceCache (rdu_pathtracer_render_cached): paths end in
+        the cache where it is trusted, one pixel in 16 trains it. Zero options take the defaults."""
+        if image.dtype != np.float32 or not image.flags.c_contiguous or image.ndim != 3:
+            raise ValueError("image must be a C-contiguous (height, width, components) float32 array")
+        h, w, c = image.shape
+        opts = S.RenderOpts(seed, estimator, rr_start, max_paths)
+        copts = S.CacheOpts(terminate_bounce, train_steps, train_batch, max_bias, min_count, int(frozen),
+                            int(always_trust))
+        cam = S.camera_desc(camera)
+        check(lib.rdu_pathtracer_render_cached(self._h, cache.handle, ctypes.byref(cam), samples, depth, fptr(image),
+                                               w, h, c, ctypes.byref(opts), ctypes.byref(copts)))
+        out = self.last_stats()
+        st = S.CacheStats()
+        check(lib.rdu_scene_last_cache_stats(self._h, ctypes.byref(st)))
+        out.update({k: getattr(st, k) for k, _ in S.CacheStats._fields_})
+        return out
+
     def last_stats(self) -> dict:
         st = S.RenderStats()
         check(lib.rdu_scene_last_render_stats(self._h, ctypes.byref(st)))

@@ -1,8 +1,9 @@
 """Learned radiance cache (rdu_rcache_* in include/rdu.h).
 
-No Rendu counterpart: an extension of the path tracer that maps a world position to RGB outgoing
-radiance through a 16-level hash-grid encoding (2^19 entries x 2 features per level) and a 5 x 64
-ReLU MLP, evaluated and trained on the GPU with bf16 MFMA kernels (fp32 accumulation, fp32 master
+No Rendu counterpart: an extension of the path tracer that maps a world position and an outgoing
+direction to RGB radiance through a 16-level hash-grid encoding of the position (2^19 entries x 2
+features per level), degree-3 spherical harmonics of the direction (16 features) and a 5 x 64 ReLU
+MLP, evaluated and trained on the GPU with bf16 MFMA kernels (fp32 accumulation, fp32 master
 weights, Adam on a relative L2 loss).
 """
 from __future__ import annotations
@@ -13,12 +14,12 @@ import numpy as np
 
 from . import _lib
 
-MLP_PARAMS = 19456
+MLP_PARAMS = 20480
 TABLE_PARAMS = 16 * (1 << 19) * 2
 LEVELS = 16
 TABLE_SIZE = 1 << 19
 # (rows, cols) of each MLP weight matrix, row-major [out][in], concatenated in this order.
-LAYER_SHAPES = [(64, 32), (64, 64), (64, 64), (64, 64), (64, 64), (16, 64)]
+LAYER_SHAPES = [(64, 48), (64, 64), (64, 64), (64, 64), (64, 64), (16, 64)]
 
 
 def _as4(a: np.ndarray) -> np.ndarray:
@@ -28,6 +29,15 @@ def _as4(a: np.ndarray) -> np.ndarray:
     if a.shape[1] == 3:
         a = np.concatenate([a, np.zeros((a.shape[0], 1), np.float32)], axis=1)
     return np.ascontiguousarray(a)
+
+
+def make_queries(positions: np.ndarray, directions: np.ndarray | None = None) -> np.ndarray:
+    """(n, 8) float32 query records: position xyz, 0, outgoing direction xyz, 0 (default +z)."""
+    p = _as4(positions)
+    d = _as4(directions) if directions is not None else np.tile(np.float32([0, 0, 1, 0]), (p.shape[0], 1))
+    if d.shape[0] != p.shape[0]:
+        raise ValueError("positions and directions differ in length")
+    return np.ascontiguousarray(np.concatenate([p, d], axis=1))
 
 
 class RadianceCache:
@@ -55,34 +65,36 @@ class RadianceCache:
     def set_learning_rate(self, lr: float) -> None:
         _lib.check(_lib.lib.rdu_rcache_set_learning_rate(self._h, float(lr)))
 
-    def infer(self, positions: np.ndarray) -> np.ndarray:
-        p = _as4(positions)
-        out = np.zeros((p.shape[0], 4), np.float32)
-        _lib.check(_lib.lib.rdu_rcache_infer(self._h, _lib.vptr(p), p.shape[0], _lib.vptr(out), 0, None))
+    def infer(self, positions: np.ndarray, directions: np.ndarray | None = None) -> np.ndarray:
+        q = make_queries(positions, directions)
+        out = np.zeros((q.shape[0], 4), np.float32)
+        _lib.check(_lib.lib.rdu_rcache_infer(self._h, _lib.vptr(q), q.shape[0], _lib.vptr(out), 0, None))
         return out[:, :3]
 
-    def infer_device(self, pos_ptr: int, n: int, out_ptr: int, stream: int = 0) -> None:
-        _lib.check(_lib.lib.rdu_rcache_infer(self._h, ctypes.c_void_p(pos_ptr), n, ctypes.c_void_p(out_ptr),
+    def infer_device(self, queries_ptr: int, n: int, out_ptr: int, stream: int = 0) -> None:
+        _lib.check(_lib.lib.rdu_rcache_infer(self._h, ctypes.c_void_p(queries_ptr), n, ctypes.c_void_p(out_ptr),
                                              _lib.RDU_DEVICE_POINTERS, ctypes.c_void_p(stream)))
 
-    def train(self, positions: np.ndarray, targets: np.ndarray, steps: int = 1) -> float:
-        p, t = _as4(positions), _as4(targets)
-        if p.shape[0] != t.shape[0]:
-            raise ValueError("positions and targets differ in length")
+    def train(self, positions: np.ndarray, targets: np.ndarray, directions: np.ndarray | None = None,
+              steps: int = 1) -> float:
+        q, t = make_queries(positions, directions), _as4(targets)
+        if q.shape[0] != t.shape[0]:
+            raise ValueError("queries and targets differ in length")
         loss = ctypes.c_float(0.0)
-        _lib.check(_lib.lib.rdu_rcache_train(self._h, _lib.vptr(p), _lib.vptr(t), p.shape[0], steps, 0, None,
+        _lib.check(_lib.lib.rdu_rcache_train(self._h, _lib.vptr(q), _lib.vptr(t), q.shape[0], steps, 0, None,
                                              ctypes.byref(loss)))
         return float(loss.value)
 
-    def train_device(self, pos_ptr: int, tgt_ptr: int, n: int, steps: int = 1, stream: int = 0) -> None:
-        _lib.check(_lib.lib.rdu_rcache_train(self._h, ctypes.c_void_p(pos_ptr), ctypes.c_void_p(tgt_ptr), n, steps,
-                                             _lib.RDU_DEVICE_POINTERS, ctypes.c_void_p(stream), None))
+    def train_device(self, queries_ptr: int, targets_ptr: int, n: int, steps: int = 1, stream: int = 0) -> None:
+        _lib.check(_lib.lib.rdu_rcache_train(self._h, ctypes.c_void_p(queries_ptr), ctypes.c_void_p(targets_ptr), n,
+                                             steps, _lib.RDU_DEVICE_POINTERS, ctypes.c_void_p(stream), None))
 
-    def gradients(self, positions: np.ndarray, targets: np.ndarray, tables: bool = True):
-        p, t = _as4(positions), _as4(targets)
+    def gradients(self, positions: np.ndarray, targets: np.ndarray, directions: np.ndarray | None = None,
+                  tables: bool = True):
+        q, t = make_queries(positions, directions), _as4(targets)
         gm = np.zeros(MLP_PARAMS, np.float32)
         gt = np.zeros(TABLE_PARAMS, np.float32) if tables else None
-        _lib.check(_lib.lib.rdu_rcache_gradients(self._h, _lib.vptr(p), _lib.vptr(t), p.shape[0], _lib.fptr(gm),
+        _lib.check(_lib.lib.rdu_rcache_gradients(self._h, _lib.vptr(q), _lib.vptr(t), q.shape[0], _lib.fptr(gm),
                                                  _lib.fptr(gt) if tables else None))
         return gm, gt
 

@@ -66,6 +66,30 @@ class RenderStats(ctypes.Structure):
                 ("shadow_rays", ctypes.c_uint64), ("samples", c_uint32)]
 
 
+class CacheOpts(ctypes.Structure):
+    """rdu_cache_opts: zero fields take the library defaults."""
+    _fields_ = [("terminate_bounce", c_uint32), ("train_steps", c_uint32), ("train_batch", c_uint32),
+                ("max_bias", c_float), ("min_count", c_float), ("frozen", c_uint32), ("always_trust", c_uint32)]
+
+
+class CacheStats(ctypes.Structure):
+    _fields_ = [("cache_queries", ctypes.c_uint64), ("records", ctypes.c_uint64), ("train_pairs", ctypes.c_uint64),
+                ("train_steps", ctypes.c_uint64), ("trusted_cells", c_uint32)]
+
+
+def scene_bounds(scene: "Scene"):
+    """(min, max) corners of the scene's objects after their model transforms."""
+    lo = np.full(3, np.inf)
+    hi = np.full(3, -np.inf)
+    for o in scene.objects:
+        p = np.asarray(scene.meshes[o.mesh].positions, np.float64).reshape(-1, 3)
+        m = np.asarray(o.model, np.float64).reshape(4, 4)
+        q = p @ m[:3, :3].T + m[:3, 3]
+        lo = np.minimum(lo, q.min(axis=0))
+        hi = np.maximum(hi, q.max(axis=0))
+    return lo.astype(F32), hi.astype(F32)
+
+
 def _fp(a):
     return a.ctypes.data_as(POINTER(c_float))
 

@@ -76,6 +76,118 @@ def cornell_spheres(seed: int = 3, subdivisions: int = 4, radiance: float = 15.0
     return scn, cornell_camera
 
 
+def fbm_heights(n: int, seed: int, octaves: int = 7) -> np.ndarray:
+    """(n, n) fractal Brownian motion: bilinearly upsampled value noise, octave k at 2^(k+2) cells
+    per side with amplitude 0.5^k, normalised to [0, 1]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    h = np.zeros((n, n), np.float64)
+    t = np.linspace(0.0, 1.0, n)
+    for k in range(octaves):
+        cells = 2 ** (k + 2)
+        g = rng.random((cells + 1, cells + 1))
+        x = t * cells
+        i = np.minimum(np.floor(x).astype(int), cells - 1)
+        f = x - i
+        f = f * f * (3.0 - 2.0 * f)  # smoothstep between lattice values
+        rows = g[i] * (1 - f)[:, None] + g[i + 1] * f[:, None]          # (n, cells+1)
+        h += (rows[:, i] * (1 - f)[None, :] + rows[:, i + 1] * f[None, :]) * 0.5 ** k
+    h -= h.min()
+    return h / h.max()
+
+
+def heightfield_mesh(heights: np.ndarray, size: float, amplitude: float) -> MeshData:
+    """Grid mesh over [-size/2, size/2]^2 in xz, y = amplitude * height; 2 (n-1)^2 triangles."""
+    n = heights.shape[0]
+    t = np.linspace(-0.5 * size, 0.5 * size, n)
+    xx, zz = np.meshgrid(t, t)
+    y = heights * amplitude
+    pos = np.stack([xx, y, zz], axis=-1).reshape(-1, 3).astype(F32)
+    d = size / (n - 1)
+    gx = np.gradient(y, d, axis=1)
+    gz = np.gradient(y, d, axis=0)
+    nrm = np.stack([-gx, np.ones_like(y), -gz], axis=-1)
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    tan = np.stack([np.ones_like(y), gx, np.zeros_like(y)], axis=-1)
+    tan /= np.linalg.norm(tan, axis=-1, keepdims=True)
+    bit = np.cross(nrm, tan)
+    uv = np.stack([(xx / size + 0.5), (zz / size + 0.5)], axis=-1).reshape(-1, 2).astype(F32)
+    ii, jj = np.meshgrid(np.arange(n - 1), np.arange(n - 1), indexing="ij")
+    a = (ii * n + jj).ravel()
+    b, c, e = a + 1, a + n, a + n + 1
+    idx = np.stack([a, c, b, b, c, e], axis=1).reshape(-1).astype(np.uint32)  # counter-clockwise seen from +y
+    return MeshData(pos, idx, nrm.reshape(-1, 3).astype(F32), tan.reshape(-1, 3).astype(F32),
+                    bit.reshape(-1, 3).astype(F32), uv)
+
+
+def latlong_sky(width: int = 1024, height: int = 512, sun=(0.35, 0.55, -0.76), sun_radiance: float = 25.0,
+                sun_angle: float = 0.05) -> np.ndarray:
+    """Procedural (height, width, 4) fp32 lat-long sky: row 0 is the zenith, u follows the azimuth
+    from -z towards +x (the tracer's lat-long lookup). Horizon-to-zenith gradient, dark ground below
+    the horizon and a sun disk of angular radius `sun_angle`."""
+    v = (np.arange(height) + 0.5) / height
+    u = (np.arange(width) + 0.5) / width
+    theta = v[:, None] * np.pi
+    phi = (u[None, :] - 0.5) * 2.0 * np.pi
+    d = np.stack([np.sin(theta) * np.sin(phi), np.cos(theta) * np.ones_like(phi), -np.sin(theta) * np.cos(phi)], -1)
+    up = np.clip(d[..., 1], 0.0, 1.0)[..., None]
+    horizon, zenith = np.array([0.85, 0.88, 0.92]), np.array([0.22, 0.42, 0.85])
+    sky = horizon * (1 - up) ** 3 + zenith * (1 - (1 - up) ** 3)
+    ground = np.array([0.12, 0.1, 0.08])
+    col = np.where(d[..., 1:2] >= 0, sky, ground)
+    s = np.asarray(sun, np.float64)
+    s /= np.linalg.norm(s)
+    col = col + np.where((d @ s)[..., None] > np.cos(sun_angle), sun_radiance, 0.0)
+    out = np.ones((height, width, 4), F32)
+    out[..., :3] = col
+    return out
+
+
+def terrain_spheres(seed: int = 11, grid: int = 512, spheres: int = 2000, subdivisions: int = 3,
+                    sky_size=(1024, 512)):
+    """The single-frame workload: a grid x grid fBm heightfield (2 (grid-1)^2 triangles) plus
+    `spheres` icospheres (20 * 4^subdivisions triangles each, one instance of a shared mesh each,
+    placed by model matrix), a lat-long sky, and one Regular material per object with albedo
+    U[0.1, 0.9], roughness U[0.05, 1] and metalness Bernoulli(0.3). (scene, camera factory)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    size, amp = 100.0, 9.0
+    heights = fbm_heights(grid, seed)
+    scn = S.Scene()
+    t_flat = scn.add_texture(S.constant_texture([0.5, 0.5, 1.0, 1.0]))
+
+    def material(albedo, rough, metal):
+        a = np.asarray(albedo, F32)
+        col = np.concatenate([np.power(a, F32(1.0 / 2.2)), [1.0]]).astype(F32)
+        t0 = scn.add_texture(S.constant_texture(col))
+        t2 = scn.add_texture(S.constant_texture([rough, metal, 1.0, 1.0]))
+        return scn.add_material(S.Material(S.MATERIAL_REGULAR, (t0, t_flat, t2), False))
+
+    m = scn.add_mesh(heightfield_mesh(heights, size, amp))
+    scn.objects.append(S.Object(m, material([0.45, 0.4, 0.3], 0.9, 0.0)))
+    sph = scn.add_mesh(icosphere(subdivisions, radius=1.0))
+    n = grid
+    for k in range(spheres):
+        r = float(np.exp(rng.uniform(np.log(0.3), np.log(1.6))))
+        x, z = rng.uniform(-0.45 * size, 0.45 * size, 2)
+        i = int(round((z / size + 0.5) * (n - 1)))
+        j = int(round((x / size + 0.5) * (n - 1)))
+        y = float(heights[i, j] * amp) + 0.7 * r
+        model = np.diag([r, r, r, 1.0]).astype(F32)
+        model[:3, 3] = (x, y, z)
+        albedo = rng.uniform(0.1, 0.9, 3)
+        rough = float(rng.uniform(0.05, 1.0))
+        metal = float(rng.random() < 0.3)
+        scn.objects.append(S.Object(sph, material(albedo, rough, metal), model))
+    scn.background = S.Background(S.BG_LATLONG, texture=scn.add_texture(latlong_sky(*sky_size)))
+
+    def cam(width, height):
+        c = Camera()
+        c.pose([0.0, 16.0, 58.0], [0.0, 3.0, 0.0], [0.0, 1.0, 0.0])
+        c.projection(ratio=width / height, fov=0.85)
+        return c
+
+    return scn, cam
+
+
 def furnace(subdivisions: int = 3, level: float = 0.5, roughness: float = 1.0, metalness: float = 0.0):
     """White furnace: an albedo-1 sphere in a constant `level` cubemap environment."""
     scn = S.Scene()

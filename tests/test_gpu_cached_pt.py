@@ -1,0 +1,114 @@
+"""Path tracing with the radiance cache (rdu_pathtracer_render_cached).
+
+The cache mode has no Rendu counterpart. What pins it:
+* a fresh cache trusts no cell, so a single-pass cached render is the plain render, bit for bit;
+* a frozen cache that predicts exactly zero and is trusted everywhere turns the render into the
+  plain render truncated after bounce 2 (depth 3) in a scene without analytic lights, bit for bit;
+* the prediction enters linearly: image(2 L) - image(0) = 2 (image(L) - image(0));
+* trained online on a Cornell box, the cached render stays close to a converged plain render.
+"""
+import numpy as np
+import pytest
+
+from conftest import log_case
+from oracle import rcache_ref as R
+from rendu_amd import scene as S
+from rendu_amd import scenes
+from rendu_amd.pathtracer import PathTracer
+from rendu_amd.rcache import RadianceCache
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+EXT = S.ESTIMATOR_EXTENDED
+
+
+def _cornell(subdiv=2):
+    scn, cam_of = scenes.cornell_spheres(seed=3, subdivisions=subdiv)
+    lo, hi = S.scene_bounds(scn)
+    return scn, cam_of, lo, hi
+
+
+def _constant_cache(lo, hi, value):
+    """A cache whose output is `value` * bf16(Y_0^0) on every channel (MLP routes the constant harmonic)."""
+    c = RadianceCache(lo, hi, seed=1)
+    mlp = np.zeros(sum(r * k for r, k in R.LAYER_SHAPES), F32)
+    W, off = [], 0
+    for r, k in R.LAYER_SHAPES:
+        W.append(mlp[off:off + r * k].reshape(r, k))
+        off += r * k
+    W[0][0, 32] = 1.0
+    for l in range(1, 5):
+        W[l][0, 0] = 1.0
+    W[5][:3, 0] = value
+    c.set_params(mlp, np.zeros(R.LEVELS * R.TABLE * 2, F32))
+    return c
+
+
+def test_fresh_cache_single_pass_is_plain():
+    scn, cam_of, lo, hi = _cornell()
+    log_case("cached render, fresh cache, one pass == plain")
+    pt = PathTracer(scn)
+    a = np.zeros((40, 48, 3), F32)
+    pt.render(cam_of(48, 40), 16, 6, a, seed=4, estimator=EXT, rr_start=3)
+    b = np.zeros_like(a)
+    st = pt.render_cached(RadianceCache(lo, hi, seed=2), cam_of(48, 40), 16, 6, b, seed=4, estimator=EXT, rr_start=3)
+    np.testing.assert_array_equal(a, b)
+    assert st["cache_queries"] == 0 and st["records"] > 0 and st["train_steps"] == 4
+
+
+def test_zero_cache_always_trusted_truncates_after_bounce_2():
+    scn, cam_of, lo, hi = _cornell()
+    log_case("cached render, zero frozen cache == plain depth 3, 4 passes")
+    pt = PathTracer(scn)
+    w, h = 40, 32
+    plain = np.zeros((h, w, 3), F32)
+    pt.render(cam_of(w, h), 16, 3, plain, seed=7, estimator=EXT, rr_start=3, max_paths=w * h * 4)
+    got = np.zeros_like(plain)
+    st = pt.render_cached(_constant_cache(lo, hi, 0.0), cam_of(w, h), 16, 8, got, seed=7, estimator=EXT, rr_start=3,
+                          max_paths=w * h * 4, frozen=True, always_trust=True)
+    np.testing.assert_array_equal(got, plain)
+    assert st["cache_queries"] > 0 and st["records"] == 0 and st["train_steps"] == 0
+
+
+def test_prediction_enters_linearly():
+    scn, cam_of, lo, hi = _cornell()
+    log_case("cached render linearity in the prediction")
+    pt = PathTracer(scn)
+    w, h = 32, 32
+    imgs = []
+    for v in (0.0, 1.0, 2.0):
+        img = np.zeros((h, w, 3), F32)
+        pt.render_cached(_constant_cache(lo, hi, v), cam_of(w, h), 8, 8, img, seed=9, estimator=EXT, rr_start=3,
+                         frozen=True, always_trust=True)
+        imgs.append(img.astype(np.float64))
+    d1, d2 = imgs[1] - imgs[0], imgs[2] - imgs[0]
+    assert d1.min() >= -1e-6 and d1.mean() > 1e-3
+    # Per-path min(c, 5) clamping breaks linearity only where a sample exceeds 5.
+    ok = np.abs(d2 - 2 * d1) <= 1e-4 * np.abs(d2) + 1e-6
+    assert ok.mean() >= 0.99, ok.mean()
+
+
+def test_trained_cache_close_to_converged_render():
+    scn, cam_of, lo, hi = _cornell(subdiv=3)
+    log_case("cached Cornell 64x64 vs 1024-spp plain")
+    pt = PathTracer(scn)
+    w = h = 64
+    ref = np.zeros((h, w, 3), F32)
+    pt.render(cam_of(w, h), 1024, 8, ref, seed=100, estimator=EXT, rr_start=3)
+    plain = np.zeros_like(ref)
+    pt.render(cam_of(w, h), 64, 8, plain, seed=1, estimator=EXT, rr_start=3, max_paths=w * h * 4)
+    cache = RadianceCache(lo, hi, seed=3)
+    warm = np.zeros_like(ref)
+    pt.render_cached(cache, cam_of(w, h), 64, 8, warm, seed=2, estimator=EXT, rr_start=3, max_paths=w * h * 4,
+                     train_steps=8, train_batch=16384)
+    got = np.zeros_like(ref)
+    st = pt.render_cached(cache, cam_of(w, h), 64, 8, got, seed=1, estimator=EXT, rr_start=3, max_paths=w * h * 4,
+                          train_steps=8, train_batch=16384)
+    assert st["cache_queries"] > 0 and st["trusted_cells"] > 0
+    bias = abs(got.mean() - ref.mean()) / ref.mean()
+    rmse_c = np.sqrt(np.mean((got - ref) ** 2))
+    rmse_p = np.sqrt(np.mean((plain - ref) ** 2))
+    print(f"[cornell cache] bias {bias:.4f} rmse cached {rmse_c:.5f} plain {rmse_p:.5f} "
+          f"queries {st['cache_queries']} trusted {st['trusted_cells']}", flush=True)
+    assert bias < 0.05, bias
+    assert rmse_c < 1.5 * rmse_p, (rmse_c, rmse_p)

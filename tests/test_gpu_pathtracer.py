@@ -165,3 +165,24 @@ def test_raycaster_of_the_scene():
     h = rc.intersects([0.0, 1.0, 2.5], [0.0, 0.0, -1.0])
     assert h.hit and h.meshId < len(scn.objects)
     assert rc.info()["meshes"] == len(scn.objects)
+
+
+def test_terrain_spheres_small():
+    """The single-frame workload's construction at a reduced size: fBm terrain, instanced
+    icospheres with random GGX materials, lat-long sky, RR from bounce 3."""
+    scn, cam_of = scenes.terrain_spheres(seed=5, grid=64, spheres=60, subdivisions=1, sky_size=(128, 64))
+    log_case("terrain+spheres small 64x36 8spp depth8 extended")
+    got, exp, _ = render_both(scn, cam_of(64, 36), 64, 36, 8, 8, seed=9, estimator=S.ESTIMATOR_EXTENDED, rr_start=3)
+    assert exp.mean() > 0.05
+    assert_close(got, exp)
+
+
+def test_terrain_spheres_full_geometry():
+    """Full single-frame scene (3,082,242 triangles over 2001 meshes, 1024x512 sky) at a reduced
+    resolution and sample count, sample for sample against the oracle."""
+    scn, cam_of = scenes.terrain_spheres()
+    assert sum(scn.meshes[o.mesh].triangle_count for o in scn.objects) == 3_082_242
+    log_case("terrain+spheres full geometry 96x54 4spp depth8 extended")
+    got, exp, st = render_both(scn, cam_of(96, 54), 96, 54, 4, 8, seed=2, estimator=S.ESTIMATOR_EXTENDED, rr_start=3)
+    assert_close(got, exp)
+    assert st["paths"] == 96 * 54 * 4

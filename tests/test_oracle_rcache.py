@@ -51,7 +51,7 @@ def _linear_tables(res: int, level: int):
 def test_trilinear_reproduces_linear_function():
     res = R.level_resolutions()
     t = _linear_tables(int(res[2]), 2)
-    mlp = np.zeros(19456, np.float32)
+    mlp = np.zeros(sum(r * c for r, c in R.LAYER_SHAPES), np.float32)
     ref = R.CacheReference(mlp, t, res, [0, 0, 0], [1, 1, 1])
     rng = np.random.default_rng(3)
     pos = rng.random((2000, 3)).astype(np.float32)
@@ -62,6 +62,27 @@ def test_trilinear_reproduces_linear_function():
     np.testing.assert_allclose(enc[:, 4], expect0, rtol=2e-6)
     np.testing.assert_allclose(enc[:, 5], expect1, rtol=2e-6)
     assert np.abs(np.delete(enc, [4, 5], axis=1)).max() == 0.0
+
+
+def test_spherical_harmonics_orthonormal():
+    """Gauss-Legendre in cos(theta) x uniform in phi integrates products of degree <= 6 exactly."""
+    xs, ws = np.polynomial.legendre.leggauss(16)
+    nphi = 32
+    phi = (np.arange(nphi) + 0.5) * 2 * np.pi / nphi
+    z = np.repeat(xs, nphi)
+    w = np.repeat(ws, nphi) * (2 * np.pi / nphi)
+    s = np.sqrt(1 - z * z)
+    d = np.stack([s * np.cos(np.tile(phi, 16)), s * np.sin(np.tile(phi, 16)), z], axis=1)
+    Y = R.sh_features(d).astype(np.float64)
+    gram = (Y * w[:, None]).T @ Y
+    np.testing.assert_allclose(gram, np.eye(16), atol=2e-6)
+
+
+def test_direction_normalisation():
+    d = np.array([[0, 0, 5], [3, 4, 0], [0, 0, 0], [np.nan, 1, 0], [np.inf, 0, 0]], np.float32)
+    u = R.unit_dirs(d)
+    np.testing.assert_allclose(u[:2], [[0, 0, 1], [0.6, 0.8, 0]], rtol=1e-7)
+    assert np.all(u[2:] == [0, 0, 1])
 
 
 def _random_params(seed):
