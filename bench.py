@@ -203,6 +203,18 @@ def main():
     extra["frame_scene_setup_s"] = round(setup3, 3)
     del pt3
 
+    # Batched jobs: 64 seeded scenes (1e4..5e5 triangles, 128^2..1024^2 pixels, 4..256 spp, depth 6),
+    # each built (host BVH + upload) and rendered; 4 host threads, one HIP stream per scene.
+    from rendu_amd import jobs as J
+    job_list = []
+    for s in range(64):
+        scn_j, cam_j, w_j, h_j, spp_j, depth_j = scenes.random_job(s)
+        job_list.append(J.Job(scn_j, cam_j(w_j, h_j), w_j, h_j, spp_j, depth_j, seed=s))
+    J.run_batch(job_list[:4], workers=4)  # warm-up (kernel loads, allocator)
+    res_j, wall_j = J.run_batch(job_list, workers=4)
+    extra["jobs64"] = J.summary(res_j, wall_j)
+    del res_j
+
     # Algorithmic bytes per traversal launch: each ray's order index (4 B) and ray (32 B) read once,
     # each hit written once (24 B), and the traversed scene arrays (4-wide nodes, triangles) once.
     alg_bytes = N_HALF * (4 + 32 + 24) + info["wide_bytes"]

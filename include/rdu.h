@@ -315,12 +315,15 @@ int rdu_rcache_set_params(rdu_rcache* cache, const float* mlp, const float* tabl
  *    vertex from bounce `terminate_bounce` on; after the pass each record becomes a training pair
  *    (query at the vertex towards the previous one, target = radiance the rest of the path brought
  *    back / throughput on arrival, clamped to [0, 100]); 1 record in 8 is held out;
- *  - ends every other path at its first surface vertex from `terminate_bounce` on whose coarse cell
- *    (32^3 over the cache's box) the cache is trusted, adding throughput x cache prediction;
+ *  - ends every other path at its first surface vertex from `terminate_bounce` on where the cache
+ *    is trusted, adding throughput x cache prediction;
  *  - then runs `train_steps` Adam steps of `train_batch` pairs (none when `frozen`), and updates
- *    the cells' trust from the held-out pairs: a cell is trusted once it has `min_count` held-out
- *    pairs (running sums halved every pass) and |sum(prediction) - sum(target)| in luminance is at
- *    most `max_bias` of the larger sum. Untrusted cells are traced in full, as without the cache.
+ *    trust from the held-out pairs, per cell of a fine 32^3 and a coarse 8^3 grid over the cache's
+ *    box: a cell is trusted once it has `min_count` held-out pairs (running sums scaled by 0.9
+ *    every pass) and |sum(prediction) - sum(target)| in luminance is at most `max_bias` of the
+ *    larger sum plus twice the standard deviation of the target sum; a vertex uses its fine cell,
+ *    or its coarse cell while the fine one has too few pairs. Untrusted vertices are traced on,
+ *    as without the cache.
  * Zero fields take the defaults: terminate_bounce 2, train_steps 4, train_batch 65536, max_bias 0.2,
  * min_count 8. `always_trust` skips the cell test (every eligible vertex ends in the cache); with
  * `frozen` as well nothing is recorded and no pixel is traced in full. The estimator options (RR,
@@ -345,6 +348,9 @@ int rdu_pathtracer_render_cached(rdu_scene* scene, rdu_rcache* cache, const rdu_
 	size_t depth, float* image, uint32_t width, uint32_t height, uint32_t components, const rdu_render_opts* opts,
 	const rdu_cache_opts* cache_opts);
 int rdu_scene_last_cache_stats(const rdu_scene* scene, rdu_cache_stats* out);
+/* The training pairs of the last pass of the last cached render (host arrays of `capacity` pairs:
+ * queries 8 floats, targets 4 floats each); *count receives the number of pairs available. */
+int rdu_scene_last_cache_pairs(const rdu_scene* scene, float* queries, float* targets, size_t capacity, size_t* count);
 
 /* Last error message of the calling thread ("" if none). */
 const char* rdu_last_error(void);

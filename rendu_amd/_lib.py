@@ -100,6 +100,7 @@ def _load() -> ctypes.CDLL:
         "rdu_pathtracer_render_cached": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_size_t, POINTER(c_float),
                                                  c_uint32, c_uint32, c_uint32, c_void_p, c_void_p]),
         "rdu_scene_last_cache_stats": (c_int, [c_void_p, c_void_p]),
+        "rdu_scene_last_cache_pairs": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, POINTER(c_size_t)]),
         "rdu_last_error": (c_char_p, []),
         "rdu_version": (c_char_p, []),
     }

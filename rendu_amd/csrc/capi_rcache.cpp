@@ -106,7 +106,7 @@ int rdu_rcache_create(rdu_rcache ** out, const float scene_min[3], const float s
 		(st = c->gw.reserve(w.size() * 4)) || (st = c->mw.reserve(w.size() * 4)) || (st = c->vw.reserve(w.size() * 4)) ||
 		(st = c->t32.upload(tab.data(), tab.size() * 4)) || (st = c->t16.reserve(tab.size() * 2)) ||
 		(st = c->gt.reserve(tab.size() * 4)) || (st = c->mt.reserve(tab.size() * 4)) || (st = c->vt.reserve(tab.size() * 4)) ||
-		(st = c->cells.reserve(size_t(kRcCellRes) * kRcCellRes * kRcCellRes * 16))) {
+		(st = c->cells.reserve(size_t(kRcCells) * 16))) {
 		delete c;
 		return st;
 	}

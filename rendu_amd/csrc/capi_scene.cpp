@@ -38,7 +38,11 @@ struct rdu_scene {
 	size_t cacheCap = 0;
 	rdu_render_stats last{};
 	rdu_cache_stats lastCache{};
+	size_t lastTrainPairs = 0; // pairs left in cTrainQ / cTrainT by the last cached pass
+	hipStream_t stream = nullptr; // renders of this scene run here (created on first render)
 	~rdu_scene() {
+		if(stream)
+			(void)hipStreamDestroy(stream);
 		if(rc)
 			rdu_raycaster_destroy(rc);
 	}
@@ -364,7 +368,10 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 	const size_t imgBytes = pixels * components * sizeof(float);
 	if((st = s->pImage.reserve(imgBytes)))
 		return st;
-	hipStream_t stream = nullptr;
+	// A stream per scene: renders of different scenes from different host threads overlap on the GPU.
+	if(!s->stream && (st = hipCheck(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate")))
+		return st;
+	hipStream_t stream = s->stream;
 	if((st = hipCheck(hipMemcpyAsync(s->pImage.ptr, image, imgBytes, hipMemcpyHostToDevice, stream), "image upload")))
 		return st;
 	if((st = hipCheck(hipMemsetAsync(s->pTotals.ptr, 0, 16, stream), "hipMemsetAsync")))
@@ -451,6 +458,7 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 			cstats.records += counts[kCacheRecords];
 			cstats.train_pairs += counts[kCacheTrain];
 			const uint32_t avail = counts[kCacheTrain];
+			s->lastTrainPairs = avail;
 			if(!co.frozen && avail > 0) {
 				RcTrainBuffers * tb = rcacheTrain(cache, trainBatch);
 				for(uint32_t k = 0; k < trainSteps; ++k) {
@@ -466,14 +474,17 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 		}
 	}
 	if(cache) {
-		const size_t nc = size_t(kCacheCellRes) * kCacheCellRes * kCacheCellRes;
+		const size_t nc = size_t(kRcCells); // fine and coarse cells passing the trust test
 		std::vector<float4> cells(nc);
 		if((st = hipCheck(hipMemcpyAsync(cells.data(), C.cells, nc * 16, hipMemcpyDeviceToHost, stream), "cells")) ||
 			(st = hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize")))
 			return st;
-		for(const float4 & c : cells)
-			if(c.z >= C.minCount && std::fabs(c.x - c.y) <= C.maxBias * std::max(c.x, c.y) + 1e-3f * c.z)
+		for(const float4 & c : cells) {
+			const float var = std::max(c.w - c.y * c.y / std::max(c.z, 1e-6f), 0.0f);
+			if(c.z >= C.minCount &&
+				std::fabs(c.x - c.y) <= C.maxBias * std::max(c.x, c.y) + 2.0f * std::sqrt(var) + 1e-3f * c.z)
 				cstats.trusted_cells += 1;
+		}
 	}
 	s->lastCache = cstats;
 	if((st = hipCheck(hipError_t(ptNormalizeImage(static_cast<float *>(s->pImage.ptr), uint32_t(pixels), components,
@@ -523,6 +534,25 @@ int rdu_scene_last_cache_stats(const rdu_scene * s, rdu_cache_stats * out) {
 	if(!s || !out)
 		return fail(RDU_ERR_INVALID, "null argument");
 	*out = s->lastCache;
+	return RDU_OK;
+}
+
+int rdu_scene_last_cache_pairs(const rdu_scene * s, float * queries, float * targets, size_t capacity, size_t * count) {
+	clearError();
+	if(!s || !count)
+		return fail(RDU_ERR_INVALID, "null argument");
+	*count = s->lastTrainPairs;
+	const size_t n = capacity < s->lastTrainPairs ? capacity : s->lastTrainPairs;
+	if(n == 0)
+		return RDU_OK;
+	if(!queries || !targets)
+		return fail(RDU_ERR_INVALID, "null output arrays");
+	int st = setDevice(s->rc->device);
+	if(st)
+		return st;
+	if((st = hipCheck(hipMemcpy(queries, s->cTrainQ.ptr, n * 32, hipMemcpyDeviceToHost), "copy")) ||
+		(st = hipCheck(hipMemcpy(targets, s->cTrainT.ptr, n * 16, hipMemcpyDeviceToHost), "copy")))
+		return st;
 	return RDU_OK;
 }
 

@@ -56,23 +56,33 @@ __device__ __forceinline__ uint32_t appendSlot(bool want, uint32_t * count) {
 	return base + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
 }
 
-// Coarse cell of a position in the cache's box.
-__device__ __forceinline__ uint32_t cacheCell(const RcGrid & g, f3 p) {
+// Trust cell of a position in the cache's box on a res^3 grid.
+__device__ __forceinline__ uint32_t cacheCell(const RcGrid & g, f3 p, int res) {
 	const float v[3] = {p.x, p.y, p.z};
 	uint32_t c[3];
 	for(int a = 0; a < 3; ++a) {
-		const float t = (v[a] - g.lo[a]) * g.inv[a] * float(kCacheCellRes);
-		const int i = t > 0.0f ? (t < float(kCacheCellRes - 1) ? int(t) : kCacheCellRes - 1) : 0;
+		const float t = (v[a] - g.lo[a]) * g.inv[a] * float(res);
+		const int i = t > 0.0f ? (t < float(res - 1) ? int(t) : res - 1) : 0;
 		c[a] = uint32_t(i);
 	}
-	return c[0] + uint32_t(kCacheCellRes) * (c[1] + uint32_t(kCacheCellRes) * c[2]);
+	return c[0] + uint32_t(res) * (c[1] + uint32_t(res) * c[2]);
+}
+
+// |sum(prediction) - sum(target)| within maxBias of the larger sum, plus twice the standard
+// deviation of the target sum (targets are one-sample estimates), plus 1e-3 per pair.
+__device__ __forceinline__ bool cellPasses(const PtCache & C, float4 s) {
+	const float var = gmax(s.w - s.y * s.y / gmax(s.z, 1e-6f), 0.0f);
+	return fabsf(s.x - s.y) <= C.maxBias * gmax(s.x, s.y) + 2.0f * sqrtf(var) + 1e-3f * s.z;
 }
 
 __device__ __forceinline__ bool cacheTrusted(const PtCache & C, f3 p) {
 	if(C.alwaysTrust)
 		return true;
-	const float4 s = C.cells[cacheCell(C.grid, p)];
-	return s.z >= C.minCount && fabsf(s.x - s.y) <= C.maxBias * gmax(s.x, s.y) + 1e-3f * s.z;
+	const float4 s = C.cells[cacheCell(C.grid, p, kCacheCellRes)];
+	if(s.z >= C.minCount)
+		return cellPasses(C, s);
+	const float4 c = C.cells[kCacheFineCells + cacheCell(C.grid, p, kCacheCoarseRes)];
+	return c.z >= C.minCount && cellPasses(C, c);
 }
 
 __device__ __forceinline__ float luminance(float4 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
@@ -283,8 +293,10 @@ __global__ __launch_bounds__(kPtBlock) void ptCacheApply(PtPaths P, PtCache C) {
 	P.color[p] = make_float4(c.x + a.x * gmax(l.x, 0.0f), c.y + a.y * gmax(l.y, 0.0f), c.z + a.z * gmax(l.z, 0.0f), 0.0f);
 }
 
-// Records -> (query, target) pairs: target = (final colour - colour on arrival) / throughput, per
-// channel (0 where the throughput is 0), clamped to [0, 100]; index % 8 == 7 held out.
+// Records -> (query, target) pairs: target = (min(final colour, 5) - min(colour on arrival, 5)) /
+// throughput, per channel (0 where the throughput is 0), clamped to [0, 100]: the radiance the rest
+// of the path brought back, as seen through the estimator's per-sample clamp (PathTracer.cpp:288).
+// Index % 8 == 7 is held out.
 __global__ __launch_bounds__(kPtBlock) void ptCacheTargets(PtPaths P, PtCache C) {
 	const uint32_t r = blockIdx.x * kPtBlock + threadIdx.x;
 	const uint32_t nRec = C.counts[kCacheRecords] < C.recCapacity ? C.counts[kCacheRecords] : C.recCapacity;
@@ -293,7 +305,9 @@ __global__ __launch_bounds__(kPtBlock) void ptCacheTargets(PtPaths P, PtCache C)
 	if(valid) {
 		const float4 cf = P.color[C.recPath[r]];
 		const float4 cb = C.rec[4 * size_t(r) + 2], tb = C.rec[4 * size_t(r) + 3];
-		const float d[3] = {cf.x - cb.x, cf.y - cb.y, cf.z - cb.z}, w[3] = {tb.x, tb.y, tb.z};
+		const float d[3] = {gmin(cf.x, 5.0f) - gmin(cb.x, 5.0f), gmin(cf.y, 5.0f) - gmin(cb.y, 5.0f),
+			gmin(cf.z, 5.0f) - gmin(cb.z, 5.0f)};
+		const float w[3] = {tb.x, tb.y, tb.z};
 		float o[3];
 		for(int k = 0; k < 3; ++k) {
 			const float v = w[k] > 0.0f ? d[k] / w[k] : 0.0f;
@@ -324,12 +338,12 @@ __global__ void ptCacheReset(uint32_t * counts) {
 		counts[threadIdx.x] = 0;
 }
 
-// Cell statistics: halve the running sums, then add this pass's held-out luminances.
+// Cell statistics: scale the running sums by kCacheDecay, then add this pass's held-out luminances.
 __global__ __launch_bounds__(kPtBlock) void ptCacheDecay(float4 * cells, uint32_t n) {
 	const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
 	if(i < n) {
 		const float4 c = cells[i];
-		cells[i] = make_float4(0.5f * c.x, 0.5f * c.y, 0.5f * c.z, 0.0f);
+		cells[i] = make_float4(kCacheDecay * c.x, kCacheDecay * c.y, kCacheDecay * c.z, kCacheDecay * c.w);
 	}
 }
 
@@ -338,13 +352,18 @@ __global__ __launch_bounds__(kPtBlock) void ptCacheHeldOut(PtCache C) {
 	if(i >= C.counts[kCacheHold])
 		return;
 	const float4 q = C.holdQ[2 * size_t(i)];
+	const f3 p = mk(q.x, q.y, q.z);
 	const float4 pr = C.pred[i];
-	const uint32_t cell = cacheCell(C.grid, mk(q.x, q.y, q.z));
 	const float lp = gmax(luminance(pr), 0.0f), lt = luminance(C.holdT[i]);
-	float * c = reinterpret_cast<float *>(C.cells + cell);
-	atomicAdd(c, lp);
-	atomicAdd(c + 1, lt);
-	atomicAdd(c + 2, 1.0f);
+	const uint32_t cells[2] = {cacheCell(C.grid, p, kCacheCellRes),
+		uint32_t(kCacheFineCells) + cacheCell(C.grid, p, kCacheCoarseRes)};
+	for(uint32_t cell : cells) {
+		float * c = reinterpret_cast<float *>(C.cells + cell);
+		atomicAdd(c, lp);
+		atomicAdd(c + 1, lt);
+		atomicAdd(c + 2, 1.0f);
+		atomicAdd(c + 3, lt * lt);
+	}
 }
 
 // Visible shadow rays add their contribution (any-hit result per path).
@@ -504,7 +523,7 @@ int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, cons
 	hipLaunchKernelGGL(ptAccumulate, dim3(blocksFor(pixels)), dim3(kPtBlock), 0, stream, f, P, dImage, components,
 		samplesInPass);
 	if(cache) {
-		const uint32_t cells = uint32_t(kCacheCellRes) * kCacheCellRes * kCacheCellRes;
+		const uint32_t cells = uint32_t(kRcCells);
 		const uint32_t holdCap = C.recCapacity / 8u + 1u;
 		int e = rcInfer(C.grid, *C.params, C.holdQ, C.counts + kCacheHold, holdCap, C.pred, stream);
 		if(e)

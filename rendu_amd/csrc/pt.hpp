@@ -49,12 +49,17 @@ struct PtScratch {
 // training pixels ((x + trainX) % 4 == 0 and (y + trainY) % 4 == 0: one pixel in 16) are traced in
 // full and record every surface vertex from bounce `terminateBounce` on; at the end of the pass each
 // record becomes a (query, target radiance) pair, target = (final colour - colour on arrival) /
-// throughput on arrival. Every other path ends at its first surface vertex from that bounce on whose
-// coarse cell the cache is trusted, adding throughput x cache(position, outgoing direction).
+// throughput on arrival. Every other path ends at its first surface vertex from that bounce on where
+// the cache is trusted, adding throughput x cache(position, outgoing direction).
 // Records with index % 8 == 7 are held out of training; their predictions against their targets
-// (luminance sums per cell, halved every pass) decide which cells are trusted: at least `minCount`
-// records and |sum(prediction) - sum(target)| <= maxBias * max(sums) + 1e-3 count.
-constexpr int kCacheCellRes = kRcCellRes; // cells per axis over the cache's box
+// (luminance sums per cell of a fine 32^3 and a coarse 8^3 grid, scaled by kCacheDecay every pass)
+// decide trust: a cell is trusted with at least `minCount` records and |sum(prediction) -
+// sum(target)| <= maxBias * max(sums) + 1e-3 count; a vertex is trusted when its fine cell is, or
+// when its fine cell has too few records and its coarse cell is.
+constexpr int kCacheCellRes = kRcCellRes;     // fine cells per axis over the cache's box
+constexpr int kCacheCoarseRes = kRcCoarseRes; // coarse cells per axis
+constexpr int kCacheFineCells = kRcCellRes * kRcCellRes * kRcCellRes;
+constexpr float kCacheDecay = 0.9f;
 enum : int { kCacheQueries = 0, kCacheRecords = 1, kCacheTrain = 2, kCacheHold = 3, kCacheSlots = 4 };
 struct PtCache {
 	int32_t terminateBounce = 2;
@@ -75,7 +80,7 @@ struct PtCache {
 	uint32_t recCapacity = 0;
 	float4 * trainQ = nullptr, * trainT = nullptr; // compact training pairs (2 float4 + 1 float4)
 	float4 * holdQ = nullptr, * holdT = nullptr;   // held-out pairs
-	float4 * cells = nullptr;          // kCacheCellRes^3 x {sum prediction, sum target, count, unused}
+	float4 * cells = nullptr;          // kRcCells x {sum prediction, sum target, count, unused}: fine, then coarse
 };
 
 int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, const PtPaths & P, uint32_t sampleBase,

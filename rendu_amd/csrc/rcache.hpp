@@ -28,7 +28,10 @@ constexpr int kRcWh = kRcWidth * kRcWidth;
 constexpr int kRcW5 = kRcOutPad * kRcWidth;
 constexpr int kRcMlpParams = kRcW0 + 4 * kRcWh + kRcW5; // 20480 (19648 used + 832 padding)
 constexpr size_t kRcTableParams = size_t(kRcLevels) * kRcTable * kRcFeatures;
-constexpr int kRcCellRes = 32; // coarse trust cells per axis (path tracer's cache mode)
+// Trust cells of the path tracer's cache mode: a fine 32^3 grid and a coarse 8^3 grid over the box.
+constexpr int kRcCellRes = 32;
+constexpr int kRcCoarseRes = 8;
+constexpr int kRcCells = kRcCellRes * kRcCellRes * kRcCellRes + kRcCoarseRes * kRcCoarseRes * kRcCoarseRes;
 
 struct RcGrid {
 	float lo[3], inv[3]; // position normalisation to [0, 1]^3
@@ -57,7 +60,8 @@ struct RcTrainBuffers {
 };
 
 struct RcAdam {
-	float lr = 1e-2f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-15f;
+	// lr 3e-3: one-sample path-tracing targets are sparse and noisy; 1e-2 overshoots on them.
+	float lr = 3e-3f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-15f;
 	float l2 = 1e-6f; // weight decay on MLP weights only
 	uint32_t step = 0;
 };

@@ -691,17 +691,21 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 
 template<bool ANY, bool SPILL, bool STATS>
 PersistentGrid persistentGrid(size_t n) {
-	static int blocksPerCu = -1, cus = -1;
-	if(blocksPerCu < 0) {
+	struct Occupancy {
+		int blocksPerCu = 1, cus = 1;
+	};
+	// Queried once per kernel variant (thread-safe initialisation of a function-local static).
+	static const Occupancy occ = [] {
+		Occupancy o;
 		int dev = 0;
 		(void)hipGetDevice(&dev);
-		(void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-		(void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocksPerCu, tracePersistent<ANY, SPILL, STATS>, kBlock, 0);
-		if(blocksPerCu < 1)
-			blocksPerCu = 1;
-		if(cus < 1)
-			cus = 1;
-	}
+		(void)hipDeviceGetAttribute(&o.cus, hipDeviceAttributeMultiprocessorCount, dev);
+		(void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocksPerCu, tracePersistent<ANY, SPILL, STATS>, kBlock, 0);
+		o.blocksPerCu = o.blocksPerCu < 1 ? 1 : o.blocksPerCu;
+		o.cus = o.cus < 1 ? 1 : o.cus;
+		return o;
+	}();
+	const int blocksPerCu = occ.blocksPerCu, cus = occ.cus;
 	PersistentGrid g;
 	// One resident grid, trimmed for small batches (about one 64-ray batch per wavefront),
 	// always a whole number of blocks per die.

@@ -1,0 +1,99 @@
+"""Batched render jobs: many independent (scene, camera, image, spp, depth) jobs on one GPU.
+
+No Rendu counterpart (Rendu renders one scene per process). A job is the whole drop-in sequence of
+the path-tracer app for one scene: build the raycaster over the scene's objects (host BVH build and
+upload, rdu_scene_commit) and render it (rdu_pathtracer_render). Jobs run on a pool of host threads;
+the library releases Python's lock inside every call and renders each scene on its own HIP stream,
+so one job's host-side build overlaps other jobs' GPU work and small renders share the GPU.
+Scenes stay alive until the batch ends, so that no device free (which waits for the whole device)
+lands inside the batch.
+"""
+from __future__ import annotations
+
+import threading
+import time
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+from typing import Callable, List, Optional
+
+import numpy as np
+
+from . import scene as S
+from .pathtracer import PathTracer
+
+
+@dataclass
+class Job:
+    scene: S.Scene
+    camera: object
+    width: int
+    height: int
+    samples: int
+    depth: int
+    seed: int = 0
+    estimator: int = S.ESTIMATOR_REFERENCE
+    rr_start: int = 3
+
+
+@dataclass
+class JobResult:
+    index: int
+    image: np.ndarray
+    latency_s: float           # build + render, from the job's start
+    build_s: float
+    render_s: float
+    paths: int
+    rays: int
+    started_s: float = 0.0     # relative to the batch start
+    stats: dict = field(default_factory=dict)
+
+
+def run_job(job: Job, keep: Optional[list] = None) -> JobResult:
+    t0 = time.perf_counter()
+    pt = PathTracer(job.scene)
+    t1 = time.perf_counter()
+    img = np.zeros((job.height, job.width, 3), np.float32)
+    st = pt.render(job.camera, job.samples, job.depth, img, seed=job.seed, estimator=job.estimator,
+                   rr_start=job.rr_start)
+    t2 = time.perf_counter()
+    if keep is not None:
+        keep.append(pt)
+    return JobResult(-1, img, t2 - t0, t1 - t0, t2 - t1, st["paths"], st["extension_rays"] + st["shadow_rays"],
+                     stats=st)
+
+
+def run_batch(jobs: List[Job], workers: int = 4, on_done: Optional[Callable[[JobResult], None]] = None):
+    """Run every job; returns (results in job order, wall seconds of the batch)."""
+    keep: list = []
+    lock = threading.Lock()
+    results: List[Optional[JobResult]] = [None] * len(jobs)
+    b0 = time.perf_counter()
+
+    def work(i):
+        s = time.perf_counter() - b0
+        r = run_job(jobs[i], keep)
+        r.index, r.started_s = i, s
+        with lock:
+            results[i] = r
+        if on_done:
+            on_done(r)
+
+    with ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
+        list(ex.map(work, range(len(jobs))))
+    wall = time.perf_counter() - b0
+    del keep
+    return results, wall
+
+
+def summary(results: List[JobResult], wall: float) -> dict:
+    lat = np.array([r.latency_s for r in results])
+    return {
+        "jobs": len(results),
+        "jobs_per_s": round(len(results) / wall, 3),
+        "wall_s": round(wall, 3),
+        "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
+        "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
+        "build_s_total": round(float(sum(r.build_s for r in results)), 3),
+        "msamples_per_s": round(sum(r.paths for r in results) / wall / 1e6, 2),
+        "mrays_per_s": round(sum(r.rays for r in results) / wall / 1e6, 2),
+    }

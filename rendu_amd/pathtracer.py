@@ -99,6 +99,17 @@ class PathTracer:
         out.update({k: getattr(st, k) for k, _ in S.CacheStats._fields_})
         return out
 
+    def last_cache_pairs(self):
+        """(queries (n, 8), targets (n, 4)) of the last pass of the last cached render."""
+        n = ctypes.c_size_t(0)
+        check(lib.rdu_scene_last_cache_pairs(self._h, None, None, 0, ctypes.byref(n)))
+        q = np.zeros((n.value, 8), np.float32)
+        t = np.zeros((n.value, 4), np.float32)
+        if n.value:
+            check(lib.rdu_scene_last_cache_pairs(self._h, q.ctypes.data_as(ctypes.c_void_p),
+                                                 t.ctypes.data_as(ctypes.c_void_p), n.value, ctypes.byref(n)))
+        return q, t
+
     def last_stats(self) -> dict:
         st = S.RenderStats()
         check(lib.rdu_scene_last_render_stats(self._h, ctypes.byref(st)))

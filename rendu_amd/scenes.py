@@ -188,6 +188,50 @@ def terrain_spheres(seed: int = 11, grid: int = 512, spheres: int = 2000, subdiv
     return scn, cam
 
 
+def random_job(seed: int):
+    """One job of the batched workload: (scene, camera factory, width, height, spp, depth) with a
+    triangle count log-uniform on [1e4, 5e5] (a ground quad plus icospheres at subdivision 3, 1280
+    triangles each, one shared mesh placed by model matrix), a square image whose pixel count is
+    log-uniform on [128^2, 1024^2], spp uniform on {4..256} and depth 6. Materials as in
+    terrain_spheres; a 256 x 128 lat-long sky."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    tris = float(np.exp(rng.uniform(np.log(1e4), np.log(5e5))))
+    side = int(round(np.exp(rng.uniform(np.log(128.0), np.log(1024.0)))))
+    spp = int(rng.integers(4, 257))
+    n_sph = max(1, int(round((tris - 2) / 1280.0)))
+    extent = 4.0 * np.sqrt(n_sph)
+    scn = S.Scene()
+    t_flat = scn.add_texture(S.constant_texture([0.5, 0.5, 1.0, 1.0]))
+
+    def material(albedo, rough, metal):
+        a = np.asarray(albedo, F32)
+        col = np.concatenate([np.power(a, F32(1.0 / 2.2)), [1.0]]).astype(F32)
+        t0 = scn.add_texture(S.constant_texture(col))
+        t2 = scn.add_texture(S.constant_texture([rough, metal, 1.0, 1.0]))
+        return scn.add_material(S.Material(S.MATERIAL_REGULAR, (t0, t_flat, t2), False))
+
+    e = extent
+    m = scn.add_mesh(quad((-e, 0, e), (e, 0, e), (e, 0, -e), (-e, 0, -e)))
+    scn.objects.append(S.Object(m, material([0.5, 0.5, 0.5], 0.8, 0.0)))
+    sph = scn.add_mesh(icosphere(3, radius=1.0))
+    for k in range(n_sph):
+        r = float(rng.uniform(0.4, 1.2))
+        x, z = rng.uniform(-0.8 * e, 0.8 * e, 2)
+        model = np.diag([r, r, r, 1.0]).astype(F32)
+        model[:3, 3] = (x, r, z)
+        scn.objects.append(S.Object(sph, material(rng.uniform(0.1, 0.9, 3), float(rng.uniform(0.05, 1.0)),
+                                                  float(rng.random() < 0.3)), model))
+    scn.background = S.Background(S.BG_LATLONG, texture=scn.add_texture(latlong_sky(256, 128)))
+
+    def cam(width, height):
+        c = Camera()
+        c.pose([0.0, 0.9 * e, 1.6 * e], [0.0, 0.0, 0.0], [0.0, 1.0, 0.0])
+        c.projection(ratio=width / height, fov=0.9)
+        return c
+
+    return scn, cam, side, side, spp, 6
+
+
 def furnace(subdivisions: int = 3, level: float = 0.5, roughness: float = 1.0, metalness: float = 0.0):
     """White furnace: an albedo-1 sphere in a constant `level` cubemap environment."""
     scn = S.Scene()

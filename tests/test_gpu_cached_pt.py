@@ -89,26 +89,33 @@ def test_prediction_enters_linearly():
 
 
 def test_trained_cache_close_to_converged_render():
+    """Online training on a Cornell box (128 x 128, 8 passes of 8 spp per render, two warm-up
+    renders): against a 1024-spp plain render, the cached render (trust test on) is neither darker
+    or brighter by more than 3% nor noisier than the plain render at the same spp by more than 5%;
+    with every vertex ending in the cache (no trust test) the bias stays under 5%."""
     scn, cam_of, lo, hi = _cornell(subdiv=3)
-    log_case("cached Cornell 64x64 vs 1024-spp plain")
+    log_case("cached Cornell 128x128 vs 1024-spp plain")
     pt = PathTracer(scn)
-    w = h = 64
+    w = h = 128
+    mp = w * h * 8
     ref = np.zeros((h, w, 3), F32)
     pt.render(cam_of(w, h), 1024, 8, ref, seed=100, estimator=EXT, rr_start=3)
     plain = np.zeros_like(ref)
-    pt.render(cam_of(w, h), 64, 8, plain, seed=1, estimator=EXT, rr_start=3, max_paths=w * h * 4)
+    pt.render(cam_of(w, h), 64, 8, plain, seed=1, estimator=EXT, rr_start=3, max_paths=mp)
     cache = RadianceCache(lo, hi, seed=3)
-    warm = np.zeros_like(ref)
-    pt.render_cached(cache, cam_of(w, h), 64, 8, warm, seed=2, estimator=EXT, rr_start=3, max_paths=w * h * 4,
-                     train_steps=8, train_batch=16384)
+    for warm_seed in (2, 3):
+        warm = np.zeros_like(ref)
+        pt.render_cached(cache, cam_of(w, h), 64, 8, warm, seed=warm_seed, estimator=EXT, rr_start=3, max_paths=mp)
     got = np.zeros_like(ref)
-    st = pt.render_cached(cache, cam_of(w, h), 64, 8, got, seed=1, estimator=EXT, rr_start=3, max_paths=w * h * 4,
-                          train_steps=8, train_batch=16384)
-    assert st["cache_queries"] > 0 and st["trusted_cells"] > 0
-    bias = abs(got.mean() - ref.mean()) / ref.mean()
-    rmse_c = np.sqrt(np.mean((got - ref) ** 2))
-    rmse_p = np.sqrt(np.mean((plain - ref) ** 2))
-    print(f"[cornell cache] bias {bias:.4f} rmse cached {rmse_c:.5f} plain {rmse_p:.5f} "
-          f"queries {st['cache_queries']} trusted {st['trusted_cells']}", flush=True)
-    assert bias < 0.05, bias
-    assert rmse_c < 1.5 * rmse_p, (rmse_c, rmse_p)
+    st = pt.render_cached(cache, cam_of(w, h), 64, 8, got, seed=1, estimator=EXT, rr_start=3, max_paths=mp)
+    allc = np.zeros_like(ref)
+    st2 = pt.render_cached(cache, cam_of(w, h), 64, 8, allc, seed=1, estimator=EXT, rr_start=3, max_paths=mp,
+                           always_trust=True)
+    rmse = lambda a: float(np.sqrt(np.mean((a - ref) ** 2)))
+    bias = lambda a: float(abs(a.mean() - ref.mean()) / ref.mean())
+    print(f"[cornell cache] trust: bias {bias(got):.4f} rmse {rmse(got):.5f} queries {st['cache_queries']} "
+          f"trusted {st['trusted_cells']} | always: bias {bias(allc):.4f} rmse {rmse(allc):.5f} "
+          f"queries {st2['cache_queries']} | plain rmse {rmse(plain):.5f} pairs {st['train_pairs']}", flush=True)
+    assert st["train_pairs"] > 0 and st2["cache_queries"] > 0
+    assert bias(got) < 0.03 and rmse(got) < 1.05 * rmse(plain)
+    assert bias(allc) < 0.05

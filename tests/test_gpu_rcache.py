@@ -183,6 +183,7 @@ def _field(pos, dirs):
 def test_training_fits_smooth_field(log_case):
     log_case("rcache training on a smooth field, 400 steps x 16384")
     c = _cache(6)
+    c.set_learning_rate(1e-2)
     test_pos, test_dir = _positions(8192, 99), _directions(8192, 98)
     test_tgt = _field(test_pos, test_dir)
     err0 = np.mean((c.infer(test_pos, test_dir) - test_tgt) ** 2)

@@ -1,0 +1,110 @@
+"""Scene ingestion (rendu_amd/ingest.py): Rendu's .scene tokens, OBJ Indexed loading, transforms,
+normals, PNG textures, and Rendu's own Cornell box scene (tests/golden/cornellbox: the reference's
+scene, meshes and textures, copied as data) rendered on the GPU against the oracle."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from rendu_amd import ingest
+from rendu_amd import scene as S
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "cornellbox")
+
+
+def test_decode_tokens_nesting_and_arrays():
+    text = """# comment
+* object: # trailing comment
+\tmesh: box
+\ttranslation: 1.0, 2.0 3.0
+* material:
+\ttextures:
+\t\t- srgb: red
+\t\t- rgb: 0.5,0.5,1.0
+\tname: m
+* background:
+\tcube: rgbcube: grey_cube
+no colon here
+"""
+    toks = ingest.decode(text)
+    assert [t.key for t in toks] == ["object", "material", "background"]
+    assert toks[0].elements[1].values == ["1.0", "2.0", "3.0"]
+    tex = toks[1].elements[0]
+    assert tex.key == "textures" and [e.key for e in tex.elements] == ["srgb", "rgb"]
+    assert tex.elements[1].values == ["0.5", "0.5", "1.0"]
+    cube = toks[2].elements[0]
+    assert cube.key == "cube" and cube.elements[0].key == "rgbcube" and cube.elements[0].values == ["grey_cube"]
+
+
+def test_obj_indexed_sharing_and_uv_flip():
+    text = "v 0 0 0\nv 1 0 0\nv 0 1 0\nv 1 1 0\nvt 0 0\nvt 1 0.25\nvn 0 0 1\n" \
+           "f 1/1/1 2/2/1 3/1/1\nf 2/2/1 4/1/1 3/1/1 1/1/1\n"
+    m = ingest.load_obj(text)
+    # Distinct 'v/vt/vn' strings: 1/1/1 2/2/1 3/1/1 4/1/1; the fourth vertex of a face is ignored.
+    assert m.positions.tolist() == [[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]]
+    assert m.indices.tolist() == [0, 1, 2, 1, 3, 2]
+    np.testing.assert_array_equal(m.texcoords, [[0, 1], [1, 0.75], [0, 1], [0, 1]])
+    assert np.all(m.normals == [0, 0, 1])
+
+
+def test_obj_uv_index_substring_quirk():
+    """'v/vt' faces: the uv index is read from foundL characters after the first slash."""
+    text = "v 0 0 0\nv 1 0 0\nv 0 1 0\n" + "".join(f"vt {k} 0\n" for k in range(12)) + "f 1/12 2/2 3/3\n"
+    m = ingest.load_obj(text)
+    assert m.texcoords[0, 0] == 0.0  # '12' truncated to '1'
+    assert m.texcoords[1, 0] == 1.0 and m.texcoords[2, 0] == 2.0
+
+
+def test_compute_normals_and_tangent_frame():
+    m = ingest.load_obj("v 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0 0\nvt 1 0\nvt 0 1\nf 1/1 2/2 3/3\n")
+    assert m.normals is None
+    ingest.compute_normals(m)
+    np.testing.assert_allclose(m.normals, [[0, 0, 1]] * 3, atol=1e-7)
+    ingest.compute_tangents(m)
+    np.testing.assert_allclose(m.tangents, [[1, 0, 0]] * 3, atol=1e-6)
+    # v is flipped on load (v' = 1 - v): the bitangent follows increasing v', here -y.
+    np.testing.assert_allclose(m.bitangents, [[0, -1, 0]] * 3, atol=1e-6)
+
+
+def test_transformation_trs():
+    toks = ingest.decode("* object:\n\torientation: 0 2 0 1.5707963\n\ttranslation: 1 2 3\n\tscaling: 2\n")
+    M = ingest.transformation(toks[0].elements)
+    p = M @ np.array([1.0, 0.0, 0.0, 1.0])
+    # Rotation by +90 degrees about +y sends +x to -z; then scale 2 and translate.
+    np.testing.assert_allclose(p[:3], [1.0, 2.0, 1.0], atol=1e-6)
+    np.testing.assert_allclose(M[:3, :3] @ M[:3, :3].T, 4 * np.eye(3), atol=1e-5)
+
+
+def test_cornellbox_scene_loads():
+    scn, cam_of = ingest.load_scene(os.path.join(GOLD, "cornellbox.scene"), [GOLD])
+    assert len(scn.objects) == 8 and len(scn.materials) == 5 and len(scn.lights) == 1
+    l = scn.lights[0]
+    assert l.type == S.LIGHT_POINT and np.allclose(l.intensity, [9.0, 8.4, 6.0]) and l.radius == 3.0
+    assert scn.background.mode == S.BG_COLOR
+    tris = sum(scn.meshes[o.mesh].triangle_count for o in scn.objects)
+    assert tris == 32
+    c = cam_of(256, 256)
+    assert np.allclose(c.position, [0, 1, 2.5]) and abs(c.fov - 1.3) < 1e-7
+    # PNG texels are bytes / 255, 4 channels.
+    grey = ingest.texture(ingest.Resources([GOLD]), "cornellbox_grey")
+    assert grey.shape == (8, 8, 4) and np.all(grey[..., 3] == 1.0)
+    assert np.all(np.abs(grey[..., :3] * 255 - np.round(grey[..., :3] * 255)) < 1e-4)
+
+
+@pytest.mark.gpu
+def test_cornellbox_scene_gpu_vs_oracle():
+    from conftest import log_case
+    from oracle import oracle as orc
+    from rendu_amd.pathtracer import PathTracer
+    scn, cam_of = ingest.load_scene(os.path.join(GOLD, "cornellbox.scene"), [GOLD])
+    log_case("Rendu cornellbox.scene 64x64 16spp depth5")
+    cam = cam_of(64, 64)
+    got = np.zeros((64, 64, 3), np.float32)
+    PathTracer(scn).render(cam, 16, 5, got, seed=1)
+    exp = np.zeros_like(got)
+    orc.OraclePathTracer(scn).render(S.camera_desc(cam), 16, 5, exp, S.RenderOpts(1, 0, 3, 0))
+    assert exp.mean() > 0.01
+    d = np.abs(got.astype(np.float64) - exp)
+    assert (d <= 1e-4 * np.abs(exp) + 1e-6).mean() >= 0.98
+    assert d.mean() <= 1e-3 * np.abs(exp).mean()

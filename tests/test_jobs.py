@@ -1,0 +1,60 @@
+"""Batched render jobs (rendu_amd/jobs.py): job generator on the CPU; concurrent runs on the GPU
+give each job exactly the image it gets alone, and one small job matches the oracle."""
+import numpy as np
+import pytest
+
+from rendu_amd import scenes
+
+
+def test_random_job_distribution():
+    tris, sides, spps = [], [], []
+    for s in range(64):
+        scn, cam, w, h, spp, depth = scenes.random_job(s)
+        t = sum(scn.meshes[o.mesh].triangle_count for o in scn.objects)
+        tris.append(t)
+        sides.append(w)
+        spps.append(spp)
+        assert depth == 6 and w == h
+    assert 1e4 * 0.85 <= min(tris) and max(tris) <= 5e5 * 1.01
+    assert 128 <= min(sides) and max(sides) <= 1024
+    assert 4 <= min(spps) and max(spps) <= 256
+    # Log-uniform: the median triangle count sits near the geometric mean of the range.
+    assert 2e4 < np.median(tris) < 2e5
+    a = scenes.random_job(5)
+    b = scenes.random_job(5)
+    assert a[2:] == b[2:] and len(a[0].objects) == len(b[0].objects)
+
+
+@pytest.mark.gpu
+def test_concurrent_jobs_match_sequential():
+    from conftest import log_case
+    from rendu_amd import jobs as J
+    log_case("6 small jobs, 3 workers vs alone")
+    js = []
+    for s in range(6):
+        scn, cam, w, h, spp, depth = scenes.random_job(100 + s)
+        side = 48 + 8 * s
+        js.append(J.Job(scn, cam(side, side), side, side, min(spp, 16), depth, seed=s))
+    alone = [J.run_job(j).image for j in js]
+    res, wall = J.run_batch(js, workers=3)
+    for r, a in zip(res, alone):
+        np.testing.assert_array_equal(r.image, a)
+    summ = J.summary(res, wall)
+    assert summ["jobs"] == 6 and summ["jobs_per_s"] > 0
+
+
+@pytest.mark.gpu
+def test_small_job_matches_oracle():
+    from conftest import log_case
+    from oracle import oracle as orc
+    from rendu_amd import jobs as J
+    from rendu_amd import scene as S
+    log_case("job 7 at 40x40 8spp vs oracle")
+    scn, cam, w, h, spp, depth = scenes.random_job(7)
+    c = cam(40, 40)
+    got = J.run_job(J.Job(scn, c, 40, 40, 8, depth, seed=3)).image
+    exp = np.zeros((40, 40, 3), np.float32)
+    orc.OraclePathTracer(scn).render(S.camera_desc(c), 8, depth, exp, S.RenderOpts(3, 0, 3, 0))
+    d = np.abs(got.astype(np.float64) - exp)
+    assert (d <= 1e-4 * np.abs(exp) + 1e-6).mean() >= 0.98
+    assert d.mean() <= 1e-3 * max(np.abs(exp).mean(), 1e-6)
