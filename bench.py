@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--main-only", action="store_true", help="only the timed incoherent closest-hit passes")
     ap.add_argument("--refill", type=int, default=0,
                     help="fixed refill threshold instead of autotuning (profiling runs: one kernel configuration)")
+    ap.add_argument("--no-cache-experiment", action="store_true", help="skip the radiance-cache comparison")
+    ap.add_argument("--cache-ref-spp", type=int, default=4096, help="samples of the reference frame")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device three times: the reference reading of the "
                          "FETCH_SIZE / WRITE_SIZE counters that converts them to bytes")
@@ -57,6 +59,61 @@ def copy_calibration():
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copied_bytes_per_dispatch": a.numel() * 4, "dispatches": 3}), flush=True)
+
+
+def cache_experiment(pt, scn, cam, ref_spp):
+    """Radiance cache on the single frame (1920 x 1080, depth 8, RR from bounce 3): RMSE against a
+    ref_spp plain render, plain at 64 spp against cached renders (fresh cache each, online training
+    included in the wall time) at 16 / 32 / 64 spp, with the trust test and with every vertex from
+    the termination bounce ending in the cache; also a cache warmed by one earlier 64-spp frame.
+    Reports the wall-time ratio of the fastest cached render whose RMSE reaches plain 64 spp."""
+    from rendu_amd import scene as S
+    from rendu_amd.rcache import RadianceCache
+    ext = S.ESTIMATOR_EXTENDED
+    h, w = 1080, 1920
+    lo, hi = S.scene_bounds(scn)
+    ref = np.zeros((h, w, 3), np.float32)
+    t0 = time.perf_counter()
+    pt.render(cam, ref_spp, 8, ref, seed=1000, estimator=ext, rr_start=3)
+    ref_s = time.perf_counter() - t0
+    rmse = lambda a: float(np.sqrt(np.mean((a.astype(np.float64) - ref) ** 2)))
+
+    def plain(spp):
+        img = np.zeros((h, w, 3), np.float32)
+        t = time.perf_counter()
+        pt.render(cam, spp, 8, img, seed=7, estimator=ext, rr_start=3)
+        return {"spp": spp, "s": round(time.perf_counter() - t, 4), "rmse": round(rmse(img), 6),
+                "mean": round(float(img.mean()), 5)}
+
+    def cached(spp, cache=None, **kw):
+        cache = cache or RadianceCache(lo, hi, seed=5)
+        img = np.zeros((h, w, 3), np.float32)
+        t = time.perf_counter()
+        st = pt.render_cached(cache, cam, spp, 8, img, seed=7, estimator=ext, rr_start=3, **kw)
+        out = {"spp": spp, "s": round(time.perf_counter() - t, 4), "rmse": round(rmse(img), 6),
+               "mean": round(float(img.mean()), 5), "cache_queries": st["cache_queries"],
+               "train_steps": st["train_steps"], "trusted_cells": st["trusted_cells"]}
+        out.update({k: v for k, v in kw.items()})
+        return out
+
+    pt.render(cam, 1, 8, np.zeros((h, w, 3), np.float32), seed=1, estimator=ext, rr_start=3)  # warm-up
+    p64 = plain(64)
+    runs = {"plain": [plain(16), plain(32), p64]}
+    runs["cached"] = [cached(s) for s in (16, 32, 64)]
+    runs["cached_always"] = [cached(s, always_trust=True) for s in (16, 32, 64)]
+    runs["cached_always_b1"] = [cached(s, always_trust=True, terminate_bounce=1) for s in (16, 32, 64)]
+    warm = RadianceCache(lo, hi, seed=5)
+    pt.render_cached(warm, cam, 64, 8, np.zeros((h, w, 3), np.float32), seed=3, estimator=ext, rr_start=3)
+    runs["warm_cache"] = [cached(s, cache=warm) for s in (16, 32)]
+    best = None
+    for key in ("cached", "cached_always", "cached_always_b1", "warm_cache"):
+        for r in runs[key]:
+            if r["rmse"] <= p64["rmse"] and (best is None or r["s"] < best[1]["s"]):
+                best = (key, r)
+    return {"ref_spp": ref_spp, "ref_s": round(ref_s, 3), "ref_mean": round(float(ref.mean()), 5), "runs": runs,
+            "plain64_rmse": p64["rmse"], "plain64_s": p64["s"],
+            "fastest_reaching_plain64_rmse": None if best is None else
+            {"mode": best[0], **best[1], "time_ratio": round(best[1]["s"] / p64["s"], 3)}}
 
 
 def load_traffic():
@@ -201,6 +258,8 @@ def main():
     extra["frame_1080p_mrays_per_s"] = round(rays3 / frame_s / 1e6, 2)
     extra["frame_1080p_rays_per_sample"] = round(rays3 / st3["paths"], 3)
     extra["frame_scene_setup_s"] = round(setup3, 3)
+    if not args.no_cache_experiment:
+        extra["cache_1080p"] = cache_experiment(pt3, scn3, cam3(1920, 1080), args.cache_ref_spp)
     del pt3
 
     # Batched jobs: 64 seeded scenes (1e4..5e5 triangles, 128^2..1024^2 pixels, 4..256 spp, depth 6),

@@ -38,7 +38,9 @@ float fromBits(uint32_t u) {
 void HostScene::addMesh(const float * xyz, size_t nv, const uint32_t * idx, size_t ni, const float * model) {
 	const uint32_t indexOffset = uint32_t(vertices.size());
 	const size_t start = vertices.size();
-	vertices.reserve(start + nv);
+	// Geometric growth: reserving exactly per mesh would copy everything at every call.
+	if(vertices.capacity() < start + nv)
+		vertices.reserve(std::max(start + nv, 2 * vertices.capacity()));
 	for(size_t i = 0; i < nv; ++i)
 		vertices.push_back(mk(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]));
 	bool identity = true;
@@ -62,7 +64,8 @@ void HostScene::addMesh(const float * xyz, size_t nv, const uint32_t * idx, size
 	}
 	const size_t startTri = tris.size();
 	const size_t count = ni / 3;
-	tris.reserve(startTri + count);
+	if(tris.capacity() < startTri + count)
+		tris.reserve(std::max(startTri + count, 2 * tris.capacity()));
 	for(size_t t = 0; t < count; ++t) {
 		HTri tr;
 		tr.v0 = indexOffset + idx[3 * t + 0];
