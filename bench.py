@@ -43,6 +43,9 @@ def parse():
                     help="fixed refill threshold instead of autotuning (profiling runs: one kernel configuration)")
     ap.add_argument("--no-cache-experiment", action="store_true", help="skip the radiance-cache comparison")
     ap.add_argument("--cache-ref-spp", type=int, default=4096, help="samples of the reference frame")
+    ap.add_argument("--profile-frame", action="store_true",
+                    help="only the single frame (1920x1080, 64 spp, depth 8), plain then with a fresh radiance "
+                         "cache: the workload of the path-tracer kernel profiles")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device three times: the reference reading of the "
                          "FETCH_SIZE / WRITE_SIZE counters that converts them to bytes")
@@ -59,6 +62,26 @@ def copy_calibration():
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copied_bytes_per_dispatch": a.numel() * 4, "dispatches": 3}), flush=True)
+
+
+def profile_frame():
+    from rendu_amd import scene as S
+    from rendu_amd import scenes
+    from rendu_amd.pathtracer import PathTracer
+    from rendu_amd.rcache import RadianceCache
+    scn, cam_of = scenes.terrain_spheres()
+    pt = PathTracer(scn)
+    cam = cam_of(1920, 1080)
+    img = np.zeros((1080, 1920, 3), np.float32)
+    t0 = time.perf_counter()
+    st = pt.render(cam, 64, 8, img, seed=2, estimator=S.ESTIMATOR_EXTENDED, rr_start=3)
+    t1 = time.perf_counter()
+    img[:] = 0
+    cst = pt.render_cached(RadianceCache(*S.scene_bounds(scn), seed=5), cam, 64, 8, img, seed=2,
+                           estimator=S.ESTIMATOR_EXTENDED, rr_start=3)
+    t2 = time.perf_counter()
+    print(json.dumps({"plain_s": round(t1 - t0, 4), "cached_s": round(t2 - t1, 4), "paths": st["paths"],
+                      "cache_queries": cst["cache_queries"], "train_steps": cst["train_steps"]}), flush=True)
 
 
 def cache_experiment(pt, scn, cam, ref_spp):
@@ -136,6 +159,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.profile_frame:
+        profile_frame()
         return
 
     import torch
