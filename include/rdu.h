@@ -324,7 +324,7 @@ int rdu_rcache_set_params(rdu_rcache* cache, const float* mlp, const float* tabl
  *    larger sum plus twice the standard deviation of the target sum; a vertex uses its fine cell,
  *    or its coarse cell while the fine one has too few pairs. Untrusted vertices are traced on,
  *    as without the cache.
- * Zero fields take the defaults: terminate_bounce 2, train_steps 4, train_batch 65536, max_bias 0.2,
+ * Zero fields take the defaults: terminate_bounce 2, train_steps 4, train_batch 16384, max_bias 0.2,
  * min_count 8. `always_trust` skips the cell test (every eligible vertex ends in the cache); with
  * `frozen` as well nothing is recorded and no pixel is traced in full. The estimator options (RR,
  * seeds) are those of `opts`. */

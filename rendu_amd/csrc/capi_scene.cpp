@@ -399,7 +399,7 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 	if(copts)
 		co = *copts;
 	const uint32_t trainSteps = co.train_steps ? co.train_steps : 4u;
-	const uint32_t trainBatch = co.train_batch ? co.train_batch : 65536u;
+	const uint32_t trainBatch = co.train_batch ? co.train_batch : 16384u;
 	rdu_cache_stats cstats{};
 	if(cache) {
 		if(rcacheDevice(cache) != s->rc->device)
