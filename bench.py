@@ -247,6 +247,21 @@ def main():
     extra["near_first_wide_nodes_per_ray"] = round(nv / sub.shape[0], 2)
     extra["near_first_triangle_tests_per_ray"] = round(tt / sub.shape[0], 2)
 
+    # The GPU-built tree (linear BVH) over the same soup: build time and the same query.
+    rg = Raycaster()
+    rg.addMesh(mesh)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rg.updateHierarchyGpu()
+    torch.cuda.synchronize()
+    extra["gpu_build_s"] = round(time.perf_counter() - t0, 4)
+    rg.autotune(rays_ptr=d_inc.data_ptr(), n=N_HALF, incoherent=True, stream=sh)
+    wall_g = timed(lambda: rg.intersect_device(d_inc.data_ptr(), N_HALF, d_hits.data_ptr(), sh, incoherent=True),
+                   steps2, 1)[0]
+    extra["gpu_built_tree_incoherent_closest_mrays"] = rate(wall_g, steps2)
+    extra["gpu_built_tree_wide_nodes"] = rg.info()["wide_nodes"]
+    del rg
+
     # Small path-tracing workload on the GPU (samples/s).
     from rendu_amd import scenes
     from rendu_amd.pathtracer import PathTracer
@@ -298,6 +313,11 @@ def main():
     J.run_batch(job_list[:4], workers=4)  # warm-up (kernel loads, allocator)
     res_j, wall_j = J.run_batch(job_list, workers=4)
     extra["jobs64"] = J.summary(res_j, wall_j)
+    del res_j
+    for j in job_list:
+        j.gpu_build = True
+    res_j, wall_j = J.run_batch(job_list, workers=4)
+    extra["jobs64_gpu_build"] = J.summary(res_j, wall_j)
     del res_j
 
     # Algorithmic bytes per traversal launch: each ray's order index (4 B) and ray (32 B) read once,

@@ -70,6 +70,15 @@ int rdu_raycaster_add_mesh(rdu_raycaster* rc, const float* xyz, size_t n_vertice
  * exact tree on the host, flattens it into the device node/triangle layout and uploads it. */
 int rdu_raycaster_update_hierarchy(rdu_raycaster* rc);
 
+/* Alternative to updateHierarchy (no reference counterpart): builds a different tree on the GPU
+ * (linear BVH over all triangles: Morton codes of the centroids, radix sort, Karras radix tree, one
+ * triangle per leaf, boxes bottom-up, 4-wide collapse) on `stream`. Closest hits keep Rendu's t,
+ * u, v and triangle except where triangles tie at equal t; any-hit results are identical. The tree
+ * has no Rendu visiting order: RDU_REFERENCE_ORDER batches fail with RDU_ERR_STATE, per-ray calls
+ * use near-first order, export_nodes / export_triangles are unavailable. Call instead of
+ * rdu_raycaster_update_hierarchy, once. */
+int rdu_raycaster_update_hierarchy_gpu(rdu_raycaster* rc, void* stream);
+
 /* Batch form of Raycaster::intersects(origin, direction, mini, maxi) (Raycaster.hpp:65,
  * Raycaster.cpp:144-185): hits[i] = closest hit of rays[i]. With RDU_DEVICE_POINTERS the call is
  * asynchronous on `stream` (a hipStream_t, NULL = default stream); otherwise it copies through
@@ -226,6 +235,9 @@ int rdu_scene_set_background(rdu_scene* scene, const rdu_background_desc* desc);
 /* Replaces PathTracer::PathTracer(scene) (PathTracer.cpp:11-21): adds every object's mesh with its
  * model to a raycaster (mesh id = object index), builds Rendu's hierarchy and uploads scene data. */
 int rdu_scene_commit(rdu_scene* scene);
+/* Before commit (no reference counterpart): build the scene's tree on the GPU
+ * (rdu_raycaster_update_hierarchy_gpu) instead of Rendu's host build. */
+int rdu_scene_set_gpu_build(rdu_scene* scene, int enable);
 
 /* The raycaster owned by a committed scene (PathTracer::raycaster(), PathTracer.hpp:29). */
 const rdu_raycaster* rdu_scene_raycaster(const rdu_scene* scene);

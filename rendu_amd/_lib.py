@@ -63,6 +63,8 @@ def _load() -> ctypes.CDLL:
         "rdu_raycaster_destroy": (None, [c_void_p]),
         "rdu_raycaster_add_mesh": (c_int, [c_void_p, fp, c_size_t, POINTER(c_uint32), c_size_t, fp]),
         "rdu_raycaster_update_hierarchy": (c_int, [c_void_p]),
+        "rdu_raycaster_update_hierarchy_gpu": (c_int, [c_void_p, c_void_p]),
+        "rdu_scene_set_gpu_build": (c_int, [c_void_p, c_int]),
         "rdu_raycaster_intersect": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_uint32, c_void_p]),
         "rdu_raycaster_intersect_any": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_uint32, c_void_p]),
         "rdu_raycaster_intersect1": (c_int, [c_void_p, fp, fp, c_float, c_float, c_void_p]),

@@ -17,7 +17,7 @@ namespace {
 int checkBuilt(const rdu_raycaster * rc) {
 	if(!rc)
 		return fail(RDU_ERR_INVALID, "null raycaster");
-	if(!rc->scene.built)
+	if(!rc->scene.built && !rc->gpuBuilt)
 		return fail(RDU_ERR_STATE, "query before updateHierarchy");
 	if(!rc->uploaded)
 		return fail(RDU_ERR_HIP, "the hierarchy is not resident on a GPU (no HIP device or upload failed)");
@@ -41,6 +41,9 @@ TraceOptions traceOptions(uint32_t flags) {
 	return o;
 }
 
+// Per-ray queries follow Rendu's visiting order on the host-built tree, near-first on a GPU-built one.
+uint32_t perRayFlags(const rdu_raycaster * rc) { return (rc && rc->gpuBuilt) ? 0u : RDU_REFERENCE_ORDER; }
+
 // Incoherent batches below this size are traced in input order: the sort does not pay off.
 constexpr size_t kReorderMin = size_t(1) << 16;
 
@@ -57,6 +60,8 @@ int query(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, void * out, 
 	if(n > (size_t(1) << 31))
 		return fail(RDU_ERR_UNSUPPORTED, "batch larger than 2^31 rays");
 	TraceOptions opt = traceOptions(flags);
+	if(rc->gpuBuilt && (opt.referenceOrder || !opt.wide))
+		return fail(RDU_ERR_STATE, "a GPU-built tree has no reference visiting order or binary layout");
 	opt.refillMin = rc->tunedRefill[(flags & RDU_INCOHERENT) ? 1 : 0][ANY ? 1 : 0];
 	const size_t outBytes = ANY ? n : n * sizeof(rdu_hit);
 	std::lock_guard<std::mutex> guard(rc->lock);
@@ -225,6 +230,43 @@ int rdu_raycaster_update_hierarchy(rdu_raycaster * rc) {
 	return RDU_OK;
 }
 
+int rdu_raycaster_update_hierarchy_gpu(rdu_raycaster * rc, void * stream) {
+	clearError();
+	if(!rc)
+		return fail(RDU_ERR_INVALID, "null raycaster");
+	if(rc->scene.built || rc->gpuBuilt)
+		return fail(RDU_ERR_STATE, "updateHierarchy called twice");
+	if(rc->scene.tris.empty())
+		return fail(RDU_ERR_INVALID, "no triangles");
+	for(size_t m = 0; m < rc->scene.meshCount; ++m)
+		if(rc->scene.nodes[m].right == 0)
+			return fail(RDU_ERR_INVALID, "mesh with no triangle (Rendu reads past its triangle list here)");
+	int st = currentDevice(&rc->device);
+	if(st)
+		return st;
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	if((st = gpuBuildBvh(rc->scene, rc->gpu, s)))
+		return st;
+	if(!rc->gpu.wide.ptr && (st = rc->gpu.wide.reserve(128))) // one-triangle scene: a placeholder node
+		return st;
+	rc->dev = DevBVH{};
+	rc->dev.tris = static_cast<const float4 *>(rc->gpu.tris.ptr);
+	rc->dev.wide = static_cast<const float4 *>(rc->gpu.wide.ptr);
+	rc->dev.wideDepth = rc->gpu.wideDepth;
+	std::memcpy(rc->dev.sceneMin, rc->gpu.sceneMin, sizeof(rc->gpu.sceneMin));
+	std::memcpy(rc->dev.sceneMax, rc->gpu.sceneMax, sizeof(rc->gpu.sceneMax));
+	std::memcpy(rc->dev.wideTopMin, rc->gpu.topMin, sizeof(rc->gpu.topMin));
+	std::memcpy(rc->dev.wideTopMax, rc->gpu.topMax, sizeof(rc->gpu.topMax));
+	rc->dev.wideTopCode = rc->gpu.topCode;
+	if(rc->gpu.triangles == 1) {
+		std::memcpy(rc->dev.wideTopMin, rc->gpu.sceneMin, sizeof(rc->gpu.sceneMin));
+		std::memcpy(rc->dev.wideTopMax, rc->gpu.sceneMax, sizeof(rc->gpu.sceneMax));
+	}
+	rc->gpuBuilt = true;
+	rc->uploaded = true;
+	return RDU_OK;
+}
+
 int rdu_raycaster_intersect(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, rdu_hit * hits, uint32_t flags,
 	void * stream) {
 	clearError();
@@ -243,7 +285,7 @@ int rdu_raycaster_intersect1(const rdu_raycaster * rc, const float o[3], const f
 	if(!o || !d || !out)
 		return fail(RDU_ERR_INVALID, "null argument");
 	const rdu_ray r = {o[0], o[1], o[2], mini, d[0], d[1], d[2], maxi};
-	return query<false>(rc, &r, 1, out, RDU_REFERENCE_ORDER, nullptr, nullptr);
+	return query<false>(rc, &r, 1, out, perRayFlags(rc), nullptr, nullptr);
 }
 
 int rdu_raycaster_intersect_any1(const rdu_raycaster * rc, const float o[3], const float d[3], float mini, float maxi,
@@ -253,7 +295,7 @@ int rdu_raycaster_intersect_any1(const rdu_raycaster * rc, const float o[3], con
 		return fail(RDU_ERR_INVALID, "null argument");
 	const rdu_ray r = {o[0], o[1], o[2], mini, d[0], d[1], d[2], maxi};
 	uint8_t occ = 0;
-	const int st = query<true>(rc, &r, 1, &occ, RDU_REFERENCE_ORDER, nullptr, nullptr);
+	const int st = query<true>(rc, &r, 1, &occ, perRayFlags(rc), nullptr, nullptr);
 	*occluded = occ;
 	return st;
 }
@@ -280,6 +322,17 @@ int rdu_raycaster_info(const rdu_raycaster * rc, rdu_bvh_info * out) {
 	out->meshes = rc->scene.meshCount;
 	out->triangles = rc->scene.tris.size();
 	out->vertices = rc->scene.vertices.size();
+	if(rc->gpuBuilt) {
+		// One binary radix tree over all triangles (one per leaf); depth is not tracked.
+		const uint64_t n = rc->gpu.triangles;
+		out->nodes = 2 * n - 1;
+		out->internal_nodes = rc->gpu.innerCount;
+		out->wide_nodes = rc->gpu.wideCount;
+		out->wide_depth = rc->gpu.wideDepth;
+		out->wide_bytes = uint64_t(rc->gpu.wideCount) * 128 + n * 48;
+		out->device_bytes = out->wide_bytes;
+		return RDU_OK;
+	}
 	out->nodes = rc->scene.nodes.size();
 	out->internal_nodes = rc->layout.internalCount;
 	out->max_depth = rc->layout.maxDepth;
@@ -297,7 +350,7 @@ int rdu_raycaster_export_nodes(const rdu_raycaster * rc, float * boxes, uint64_t
 	if(!rc || !boxes || !left || !right || !leaf)
 		return fail(RDU_ERR_INVALID, "null argument");
 	if(!rc->scene.built)
-		return fail(RDU_ERR_STATE, "export before updateHierarchy");
+		return fail(RDU_ERR_STATE, "export before updateHierarchy (or on a GPU-built tree: no Rendu node array)");
 	for(size_t i = 0; i < rc->scene.nodes.size(); ++i) {
 		const HNode & nd = rc->scene.nodes[i];
 		const float b[6] = {nd.box.mn.x, nd.box.mn.y, nd.box.mn.z, nd.box.mx.x, nd.box.mx.y, nd.box.mx.z};
@@ -314,7 +367,7 @@ int rdu_raycaster_export_triangles(const rdu_raycaster * rc, uint32_t * mesh_id,
 	if(!rc || !mesh_id || !local_id)
 		return fail(RDU_ERR_INVALID, "null argument");
 	if(!rc->scene.built)
-		return fail(RDU_ERR_STATE, "export before updateHierarchy");
+		return fail(RDU_ERR_STATE, "export before updateHierarchy (or on a GPU-built tree: no Rendu triangle order)");
 	for(size_t i = 0; i < rc->scene.tris.size(); ++i) {
 		mesh_id[i] = rc->scene.tris[i].meshId;
 		local_id[i] = rc->scene.tris[i].localId;

@@ -26,6 +26,7 @@ struct rdu_scene {
 	std::vector<rdu_light_desc> lights;
 	rdu_background_desc bg{};
 	bool committed = false;
+	bool gpuBuild = false; // commit builds the tree with rdu_raycaster_update_hierarchy_gpu
 	rdu_raycaster * rc = nullptr;
 	rdu::DevScene dev;
 	rdu::DevBuffer dTexels, dTextures, dNrm, dTan, dBit, dUv, dIdx, dMeshes, dObjects, dMaterials, dLights;
@@ -130,6 +131,16 @@ int rdu_scene_create(rdu_scene ** out) {
 }
 
 void rdu_scene_destroy(rdu_scene * s) { delete s; }
+
+int rdu_scene_set_gpu_build(rdu_scene * s, int enable) {
+	clearError();
+	if(!s)
+		return fail(RDU_ERR_INVALID, "null scene");
+	if(s->committed)
+		return fail(RDU_ERR_STATE, "scene already committed");
+	s->gpuBuild = enable != 0;
+	return RDU_OK;
+}
 
 int rdu_scene_add_texture(rdu_scene * s, const rdu_texture_desc * d, int32_t * id) {
 	clearError();
@@ -252,7 +263,7 @@ int rdu_scene_commit(rdu_scene * s) {
 				o.model)))
 			return st;
 	}
-	if((st = rdu_raycaster_update_hierarchy(s->rc)))
+	if((st = s->gpuBuild ? rdu_raycaster_update_hierarchy_gpu(s->rc, nullptr) : rdu_raycaster_update_hierarchy(s->rc)))
 		return st;
 	std::vector<DevTexture> tex;
 	for(const auto & t : s->textures)

@@ -2,6 +2,7 @@
 #pragma once
 #include "bvh.hpp"
 #include "capi_common.hpp"
+#include "gpubuild.hpp"
 #include "trace.hpp"
 #include "../../include/rdu.h"
 
@@ -15,6 +16,9 @@ struct rdu_raycaster {
 	rdu::DevBuffer nodes, tris, roots, wide;
 	bool uploaded = false;
 	int device = -1;
+	// GPU-built tree (rdu_raycaster_update_hierarchy_gpu): near-first queries only.
+	rdu::GpuBvh gpu;
+	bool gpuBuilt = false;
 	// Staging for host-pointer queries and global stack spill; guarded by `lock`.
 	mutable std::mutex lock;
 	mutable rdu::DevBuffer stageIn, stageOut, gstack, stats;

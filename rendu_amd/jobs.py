@@ -33,6 +33,7 @@ class Job:
     seed: int = 0
     estimator: int = S.ESTIMATOR_REFERENCE
     rr_start: int = 3
+    gpu_build: bool = False  # build the tree on the GPU instead of Rendu's host build
 
 
 @dataclass
@@ -50,7 +51,7 @@ class JobResult:
 
 def run_job(job: Job, keep: Optional[list] = None) -> JobResult:
     t0 = time.perf_counter()
-    pt = PathTracer(job.scene)
+    pt = PathTracer(job.scene, gpu_build=job.gpu_build)
     t1 = time.perf_counter()
     img = np.zeros((job.height, job.width, 3), np.float32)
     st = pt.render(job.camera, job.samples, job.depth, img, seed=job.seed, estimator=job.estimator,

@@ -36,10 +36,13 @@ class _BorrowedRaycaster:
 
 
 class PathTracer:
-    def __init__(self, scene: S.Scene):
+    def __init__(self, scene: S.Scene, gpu_build: bool = False):
+        """gpu_build: build the tree on the GPU (rdu_scene_set_gpu_build) instead of Rendu's host build."""
         h = ctypes.c_void_p()
         check(lib.rdu_scene_create(ctypes.byref(h)))
         self._h = h
+        if gpu_build:
+            check(lib.rdu_scene_set_gpu_build(h, 1))
         keep, tex, meshes, mats, objs, lights, bg = scene.descriptors()
         i = ctypes.c_int32()
         for t in tex:

@@ -92,6 +92,11 @@ class Raycaster:
         """Raycaster::updateHierarchy() (Raycaster.cpp:54-142)."""
         check(lib.rdu_raycaster_update_hierarchy(self._h))
 
+    def updateHierarchyGpu(self, stream: int = 0) -> None:
+        """Build another tree on the GPU instead (rdu_raycaster_update_hierarchy_gpu): same closest
+        hits except equal-t ties, no reference visiting order."""
+        check(lib.rdu_raycaster_update_hierarchy_gpu(self._h, ctypes.c_void_p(stream) if stream else None))
+
     # -- per-ray API (reference signatures) ---------------------------------------------------------
     def intersects(self, origin, direction, mini: float = 0.0001, maxi: float = 1e8) -> Hit:
         """Raycaster::intersects (Raycaster.hpp:65)."""
