@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--main-only", action="store_true", help="only the timed incoherent closest-hit passes")
     ap.add_argument("--refill", type=int, default=0,
                     help="fixed refill threshold instead of autotuning (profiling runs: one kernel configuration)")
+    ap.add_argument("--tree", choices=("rendu", "gpu"), default="rendu",
+                    help="tree of the timed queries: Rendu's host build (updateHierarchy) or the GPU linear BVH")
     ap.add_argument("--no-cache-experiment", action="store_true", help="skip the radiance-cache comparison")
     ap.add_argument("--cache-ref-spp", type=int, default=4096, help="samples of the reference frame")
     ap.add_argument("--profile-frame", action="store_true",
@@ -177,7 +179,11 @@ def main():
     rc = Raycaster()
     rc.addMesh(mesh)
     t0 = time.perf_counter()
-    rc.updateHierarchy()
+    if args.tree == "gpu":
+        rc.updateHierarchyGpu()
+        torch.cuda.synchronize()
+    else:
+        rc.updateHierarchy()
     build_s = time.perf_counter() - t0
     info = rc.info()
 
@@ -240,26 +246,33 @@ def main():
         "incoherent_closest_input_order_mrays": rate(timed(lambda: closest(d_inc, incoh=False), steps2, 1)[0], steps2),
         "incoherent_any_mrays": rate(timed(lambda: anyhit(d_inc), steps2, 1)[0], steps2),
         "coherent_any_mrays": rate(timed(lambda: anyhit(d_coh, incoh=False), steps2, 1)[0], steps2),
-        "incoherent_closest_reference_order_mrays": rate(timed(lambda: closest(d_inc, ref=True), 2, 1)[0], 2),
     }
+    if args.tree == "rendu":  # Rendu's visiting order exists only on Rendu's tree
+        extra["incoherent_closest_reference_order_mrays"] = rate(timed(lambda: closest(d_inc, ref=True), 2, 1)[0], 2)
     sub = np.ascontiguousarray(incoherent[: 1 << 20])
     nv, tt = rc.trace_stats(sub)
     extra["near_first_wide_nodes_per_ray"] = round(nv / sub.shape[0], 2)
     extra["near_first_triangle_tests_per_ray"] = round(tt / sub.shape[0], 2)
 
-    # The GPU-built tree (linear BVH) over the same soup: build time and the same query.
+    # The other tree over the same soup: its build time and the same query.
     rg = Raycaster()
     rg.addMesh(mesh)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    rg.updateHierarchyGpu()
+    if args.tree == "rendu":
+        rg.updateHierarchyGpu()
+    else:
+        rg.updateHierarchy()
     torch.cuda.synchronize()
-    extra["gpu_build_s"] = round(time.perf_counter() - t0, 4)
+    other_build = time.perf_counter() - t0
     rg.autotune(rays_ptr=d_inc.data_ptr(), n=N_HALF, incoherent=True, stream=sh)
     wall_g = timed(lambda: rg.intersect_device(d_inc.data_ptr(), N_HALF, d_hits.data_ptr(), sh, incoherent=True),
                    steps2, 1)[0]
-    extra["gpu_built_tree_incoherent_closest_mrays"] = rate(wall_g, steps2)
-    extra["gpu_built_tree_wide_nodes"] = rg.info()["wide_nodes"]
+    nv2, tt2 = rg.trace_stats(sub)
+    extra["other_tree"] = {"tree": "gpu linear BVH" if args.tree == "rendu" else "rendu (host build)",
+                           "build_s": round(other_build, 4), "incoherent_closest_mrays": rate(wall_g, steps2),
+                           "wide_nodes": rg.info()["wide_nodes"], "wide_nodes_per_ray": round(nv2 / sub.shape[0], 2),
+                           "triangle_tests_per_ray": round(tt2 / sub.shape[0], 2)}
     del rg
 
     # Small path-tracing workload on the GPU (samples/s).
@@ -345,6 +358,7 @@ def main():
             "bvh_depth": info["max_depth"],
             "wide_nodes": info["wide_nodes"],
             "visiting_order": "near-first, 4-wide, rays Morton-ordered in the timed region",
+            "tree": "rendu (host build, updateHierarchy)" if args.tree == "rendu" else "gpu linear BVH",
             "refill_min": tune["refill_min"],
         },
         "roofline": {

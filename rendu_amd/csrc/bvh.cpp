@@ -197,6 +197,27 @@ void HostScene::flatten(DeviceLayout & out) const {
 	// split at the median root centroid on the largest axis. A group's box contains its members'
 	// boxes, so it culls nothing that a member's own test would keep.
 	std::vector<HNode> ext(nodes);
+	// Near-first layout only: a 2-triangle leaf becomes a node over two 1-triangle leaves with the
+	// triangles' own boxes (Bounds.cpp:12-15), so that the 4-wide node culls each triangle.
+	const size_t baseCount = ext.size();
+	for(size_t i = 0; i < baseCount; ++i) {
+		if(!ext[i].leaf || ext[i].right != 2)
+			continue;
+		const uint64_t first = ext[i].left;
+		HNode a, b;
+		a.leaf = b.leaf = true;
+		a.left = first;
+		a.right = 1;
+		a.box = tris[first].box;
+		b.left = first + 1;
+		b.right = 1;
+		b.box = tris[first + 1].box;
+		ext.push_back(a);
+		ext.push_back(b);
+		ext[i].leaf = false;
+		ext[i].left = ext.size() - 2;
+		ext[i].right = ext.size() - 1;
+	}
 	uint64_t top = 0;
 	if(meshCount > 1) {
 		struct TWork {

@@ -263,8 +263,15 @@ int rdu_scene_commit(rdu_scene * s) {
 				o.model)))
 			return st;
 	}
-	if((st = s->gpuBuild ? rdu_raycaster_update_hierarchy_gpu(s->rc, nullptr) : rdu_raycaster_update_hierarchy(s->rc)))
+	if(s->gpuBuild) {
+		// Built on the scene's own stream, so that builds of different scenes overlap.
+		if(!s->stream && (st = hipCheck(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate")))
+			return st;
+		if((st = rdu_raycaster_update_hierarchy_gpu(s->rc, s->stream)))
+			return st;
+	} else if((st = rdu_raycaster_update_hierarchy(s->rc))) {
 		return st;
+	}
 	std::vector<DevTexture> tex;
 	for(const auto & t : s->textures)
 		tex.push_back({t.offset, int32_t(t.w), int32_t(t.h)});

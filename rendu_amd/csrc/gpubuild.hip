@@ -259,6 +259,8 @@ __global__ __launch_bounds__(kGbBlock) void gbCollapse(const uint2 * __restrict_
 	kids[0] = c0.x;
 	kids[1] = c0.y;
 	int nk = 2;
+	// Leaves stay single triangles: measured faster than merging sibling leaves into 2-triangle
+	// leaves (the 4-wide node then culls each triangle by its own box).
 	while(nk < 4) {
 		int best = -1;
 		float bestArea = -1.0f;
@@ -332,6 +334,41 @@ __global__ void gbSeedQueue(uint2 * q, uint32_t * count, uint32_t * wideCount) {
 	}
 }
 
+// One stream-ordered block for every temporary of a build: hipMallocAsync / hipFreeAsync order
+// with the build's stream, where a plain hipFree would wait for the whole device, so builds overlap
+// other streams' renders. Sub-allocations are 256-B aligned.
+class Arena {
+public:
+	explicit Arena(hipStream_t s) : s_(s) {}
+	Arena(const Arena &) = delete;
+	Arena & operator=(const Arena &) = delete;
+	~Arena() {
+		if(base_)
+			(void)hipFreeAsync(base_, s_);
+	}
+	size_t add(size_t bytes) {
+		const size_t o = size_;
+		size_ = (size_ + bytes + 255) & ~size_t(255);
+		return o;
+	}
+	int allocate() {
+		if(hipMallocAsync(&base_, size_ ? size_ : 256, s_) != hipSuccess) {
+			base_ = nullptr;
+			return fail(RDU_ERR_ALLOC, "GPU build: stream-ordered allocation failed");
+		}
+		return RDU_OK;
+	}
+	template<typename T>
+	T * at(size_t off) const {
+		return reinterpret_cast<T *>(static_cast<char *>(base_) + off);
+	}
+
+private:
+	hipStream_t s_;
+	void * base_ = nullptr;
+	size_t size_ = 0;
+};
+
 #define GB_CHECK(x)                                 \
 	do {                                            \
 		const hipError_t e_ = (x);                  \
@@ -357,33 +394,34 @@ int gpuBuildBvh(const HostScene & hs, GpuBvh & out, hipStream_t s) {
 		tri[5 * i + 3] = t.meshId;
 		tri[5 * i + 4] = t.localId;
 	}
-	DevBuffer dVtx, dTri, recIn, boxIn, cen, keys, keysAlt, vals, valsAlt, temp, leafBox, innerBox, childB, parI,
-		parL, flags, qA, qB, counts, bounds;
-	int st;
-	if((st = dVtx.upload(hs.vertices.data(), hs.vertices.size() * sizeof(f3))) ||
-		(st = dTri.upload(tri.data(), tri.size() * 4)) || (st = recIn.reserve(n * 48)) || (st = boxIn.reserve(n * 32)) ||
-		(st = cen.reserve(n * 16)) || (st = keys.reserve(n * 4)) || (st = keysAlt.reserve(n * 4)) ||
-		(st = vals.reserve(n * 4)) || (st = valsAlt.reserve(n * 4)) || (st = leafBox.reserve(n * 32)) ||
-		(st = bounds.reserve(12 * 4)) || (st = counts.reserve(4 * 4)) || (st = out.tris.reserve(n * 48)))
-		return st;
-	int * dB = static_cast<int *>(bounds.ptr);
-	hipLaunchKernelGGL(gbInitBounds, dim3(1), dim3(64), 0, s, dB);
-	hipLaunchKernelGGL(gbPrep, dim3(gbBlocks(n)), dim3(kGbBlock), 0, s, static_cast<const float *>(dVtx.ptr),
-		static_cast<const uint32_t *>(dTri.ptr), uint32_t(n), static_cast<float4 *>(recIn.ptr), static_cast<float4 *>(boxIn.ptr),
-		static_cast<float4 *>(cen.ptr), dB);
-	hipLaunchKernelGGL(gbMorton, dim3(gbBlocks(n)), dim3(kGbBlock), 0, s, static_cast<const float4 *>(cen.ptr), uint32_t(n),
-		dB, static_cast<uint32_t *>(keysAlt.ptr), static_cast<uint32_t *>(valsAlt.ptr));
-	GB_CHECK(hipGetLastError());
+	// Every temporary lives in one stream-ordered block (see Arena), laid out before it is allocated.
+	const size_t ni = n - 1;
 	size_t tb = 0;
 	GB_CHECK(rocprim::radix_sort_pairs(nullptr, tb, static_cast<uint32_t *>(nullptr), static_cast<uint32_t *>(nullptr),
 		static_cast<uint32_t *>(nullptr), static_cast<uint32_t *>(nullptr), n, 0, 30, s));
-	if((st = temp.reserve(tb)))
+	Arena ar(s);
+	const size_t oVtx = ar.add(hs.vertices.size() * sizeof(f3)), oTri = ar.add(tri.size() * 4), oRec = ar.add(n * 48),
+				 oBox = ar.add(n * 32), oCen = ar.add(n * 16), oKeys = ar.add(n * 4), oKeysAlt = ar.add(n * 4),
+				 oVals = ar.add(n * 4), oValsAlt = ar.add(n * 4), oTemp = ar.add(tb), oLeaf = ar.add(n * 32),
+				 oBounds = ar.add(12 * 4), oCounts = ar.add(4 * 4), oInner = ar.add(ni * 32), oChild = ar.add(ni * 8),
+				 oParI = ar.add(ni * 4), oParL = ar.add(n * 4), oFlags = ar.add(ni * 4), oQA = ar.add(ni * 8),
+				 oQB = ar.add(ni * 8);
+	int st;
+	if((st = ar.allocate()) || (st = out.tris.reserve(n * 48)) || (ni && (st = out.wide.reserve(ni * 128))))
 		return st;
-	GB_CHECK(rocprim::radix_sort_pairs(temp.ptr, tb, static_cast<uint32_t *>(keysAlt.ptr), static_cast<uint32_t *>(keys.ptr),
-		static_cast<uint32_t *>(valsAlt.ptr), static_cast<uint32_t *>(vals.ptr), n, 0, 30, s));
-	hipLaunchKernelGGL(gbGather, dim3(gbBlocks(n)), dim3(kGbBlock), 0, s, static_cast<const uint32_t *>(vals.ptr),
-		uint32_t(n), static_cast<const float4 *>(recIn.ptr), static_cast<const float4 *>(boxIn.ptr),
-		static_cast<float4 *>(out.tris.ptr), static_cast<float4 *>(leafBox.ptr));
+	GB_CHECK(hipMemcpyAsync(ar.at<void>(oVtx), hs.vertices.data(), hs.vertices.size() * sizeof(f3), hipMemcpyHostToDevice, s));
+	GB_CHECK(hipMemcpyAsync(ar.at<void>(oTri), tri.data(), tri.size() * 4, hipMemcpyHostToDevice, s));
+	int * dB = ar.at<int>(oBounds);
+	hipLaunchKernelGGL(gbInitBounds, dim3(1), dim3(64), 0, s, dB);
+	hipLaunchKernelGGL(gbPrep, dim3(gbBlocks(n)), dim3(kGbBlock), 0, s, ar.at<const float>(oVtx),
+		ar.at<const uint32_t>(oTri), uint32_t(n), ar.at<float4>(oRec), ar.at<float4>(oBox), ar.at<float4>(oCen), dB);
+	hipLaunchKernelGGL(gbMorton, dim3(gbBlocks(n)), dim3(kGbBlock), 0, s, ar.at<const float4>(oCen), uint32_t(n), dB,
+		ar.at<uint32_t>(oKeysAlt), ar.at<uint32_t>(oValsAlt));
+	GB_CHECK(hipGetLastError());
+	GB_CHECK(rocprim::radix_sort_pairs(ar.at<void>(oTemp), tb, ar.at<uint32_t>(oKeysAlt), ar.at<uint32_t>(oKeys),
+		ar.at<uint32_t>(oValsAlt), ar.at<uint32_t>(oVals), n, 0, 30, s));
+	hipLaunchKernelGGL(gbGather, dim3(gbBlocks(n)), dim3(kGbBlock), 0, s, ar.at<const uint32_t>(oVals), uint32_t(n),
+		ar.at<const float4>(oRec), ar.at<const float4>(oBox), static_cast<float4 *>(out.tris.ptr), ar.at<float4>(oLeaf));
 	GB_CHECK(hipGetLastError());
 	int bh[12];
 	GB_CHECK(hipMemcpyAsync(bh, dB, sizeof(bh), hipMemcpyDeviceToHost, s));
@@ -401,22 +439,19 @@ int gpuBuildBvh(const HostScene & hs, GpuBvh & out, hipStream_t s) {
 		out.innerCount = 0;
 		return 0;
 	}
-	const size_t ni = n - 1;
-	if((st = innerBox.reserve(ni * 32)) || (st = childB.reserve(ni * 8)) || (st = parI.reserve(ni * 4)) ||
-		(st = parL.reserve(n * 4)) || (st = flags.reserve(ni * 4)) || (st = qA.reserve(ni * 8)) || (st = qB.reserve(ni * 8)) ||
-		(st = out.wide.reserve(ni * 128)))
-		return st;
-	hipLaunchKernelGGL(gbKarras, dim3(gbBlocks(ni)), dim3(kGbBlock), 0, s, static_cast<const uint32_t *>(keys.ptr),
-		uint32_t(n), static_cast<int2 *>(childB.ptr), static_cast<uint32_t *>(parI.ptr), static_cast<uint32_t *>(parL.ptr));
-	hipLaunchKernelGGL(gbZero, dim3(gbBlocks(ni)), dim3(kGbBlock), 0, s, static_cast<uint32_t *>(flags.ptr), ni);
-	hipLaunchKernelGGL(gbBounds, dim3(gbBlocks(n)), dim3(kGbBlock), 0, s, uint32_t(n),
-		static_cast<const int2 *>(childB.ptr), static_cast<const uint32_t *>(parI.ptr), static_cast<const uint32_t *>(parL.ptr),
-		static_cast<const float4 *>(leafBox.ptr), static_cast<float4 *>(innerBox.ptr), static_cast<uint32_t *>(flags.ptr));
+	const int2 * childC = ar.at<const int2>(oChild);
+	const float4 * leafC = ar.at<const float4>(oLeaf);
+	const float4 * innerC = ar.at<const float4>(oInner);
+	hipLaunchKernelGGL(gbKarras, dim3(gbBlocks(ni)), dim3(kGbBlock), 0, s, ar.at<const uint32_t>(oKeys), uint32_t(n),
+		ar.at<int2>(oChild), ar.at<uint32_t>(oParI), ar.at<uint32_t>(oParL));
+	hipLaunchKernelGGL(gbZero, dim3(gbBlocks(ni)), dim3(kGbBlock), 0, s, ar.at<uint32_t>(oFlags), ni);
+	hipLaunchKernelGGL(gbBounds, dim3(gbBlocks(n)), dim3(kGbBlock), 0, s, uint32_t(n), childC, ar.at<const uint32_t>(oParI),
+		ar.at<const uint32_t>(oParL), leafC, ar.at<float4>(oInner), ar.at<uint32_t>(oFlags));
 	GB_CHECK(hipGetLastError());
 	// Collapse, one level per launch; the host reads each level's count (a few bytes) to stop.
-	uint32_t * dc = static_cast<uint32_t *>(counts.ptr); // [0], [1]: queue counts; [2]: wide nodes
-	hipLaunchKernelGGL(gbSeedQueue, dim3(1), dim3(64), 0, s, static_cast<uint2 *>(qA.ptr), dc, dc + 2);
-	uint2 * q[2] = {static_cast<uint2 *>(qA.ptr), static_cast<uint2 *>(qB.ptr)};
+	uint32_t * dc = ar.at<uint32_t>(oCounts); // [0], [1]: queue counts; [2]: wide nodes
+	hipLaunchKernelGGL(gbSeedQueue, dim3(1), dim3(64), 0, s, ar.at<uint2>(oQA), dc, dc + 2);
+	uint2 * q[2] = {ar.at<uint2>(oQA), ar.at<uint2>(oQB)};
 	uint32_t levels = 0, pending = 1;
 	int cur = 0;
 	while(pending > 0) {
@@ -424,8 +459,7 @@ int gpuBuildBvh(const HostScene & hs, GpuBvh & out, hipStream_t s) {
 			return fail(RDU_ERR_UNSUPPORTED, "GPU build: tree deeper than 4096 levels");
 		hipLaunchKernelGGL(gbZero, dim3(1), dim3(64), 0, s, dc + (1 - cur), size_t(1));
 		hipLaunchKernelGGL(gbCollapse, dim3(gbBlocks(pending)), dim3(kGbBlock), 0, s, q[cur], dc + cur, q[1 - cur],
-			dc + (1 - cur), dc + 2, static_cast<const int2 *>(childB.ptr), static_cast<const float4 *>(leafBox.ptr),
-			static_cast<const float4 *>(innerBox.ptr), static_cast<float4 *>(out.wide.ptr));
+			dc + (1 - cur), dc + 2, childC, leafC, innerC, static_cast<float4 *>(out.wide.ptr));
 		GB_CHECK(hipGetLastError());
 		GB_CHECK(hipMemcpyAsync(&pending, dc + (1 - cur), 4, hipMemcpyDeviceToHost, s));
 		GB_CHECK(hipStreamSynchronize(s));
@@ -435,7 +469,7 @@ int gpuBuildBvh(const HostScene & hs, GpuBvh & out, hipStream_t s) {
 	uint32_t wideCount = 0;
 	float4 root[2];
 	GB_CHECK(hipMemcpyAsync(&wideCount, dc + 2, 4, hipMemcpyDeviceToHost, s));
-	GB_CHECK(hipMemcpyAsync(root, innerBox.ptr, 32, hipMemcpyDeviceToHost, s));
+	GB_CHECK(hipMemcpyAsync(root, innerC, 32, hipMemcpyDeviceToHost, s));
 	GB_CHECK(hipStreamSynchronize(s));
 	out.wideCount = wideCount;
 	out.wideDepth = levels;
