@@ -141,16 +141,22 @@ def cache_experiment(pt, scn, cam, ref_spp):
             {"mode": best[0], **best[1], "time_ratio": round(best[1]["s"] / p64["s"], 3)}}
 
 
-def load_traffic():
-    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3 --pmc summary."""
+def load_pmc():
+    """The committed rocprofv3 --pmc summary of the dominant kernel (None when absent)."""
     path = os.path.join(ROOT, "profiles", "trace_closest_soup1m_pmc.json")
     if not os.path.exists(path):
         return None
     try:
         with open(path) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
-    except Exception:
+            return json.load(f)
+    except (OSError, ValueError):
         return None
+
+
+def load_traffic():
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3 --pmc summary."""
+    pmc = load_pmc()
+    return pmc.get("hbm_bytes_per_launch") if pmc else None
 
 
 def main():
@@ -253,6 +259,15 @@ def main():
     nv, tt = rc.trace_stats(sub)
     extra["near_first_wide_nodes_per_ray"] = round(nv / sub.shape[0], 2)
     extra["near_first_triangle_tests_per_ray"] = round(tt / sub.shape[0], 2)
+    # Demand bytes of this traversal order (112 B per 4-wide node fetched, 48 B per triangle
+    # record), against the fabric traffic and L2 hit rate of the committed rocprofv3 counters.
+    demand = (nv * 112 + tt * 48) / sub.shape[0]
+    extra["demand_bytes_per_ray"] = round(demand, 1)
+    extra["demand_gbs"] = round(demand * N_HALF / (trav_ms * 1e-3) / 1e9, 1)
+    pmc = load_pmc()
+    if pmc:
+        extra["fabric_gbs_from_counters"] = round(pmc["hbm_bytes_per_launch"] / (trav_ms * 1e-3) / 1e9, 1)
+        extra["l2_hit_rate_from_counters"] = pmc.get("tcc_hit_rate")
 
     # The other tree over the same soup: its build time and the same query.
     rg = Raycaster()
@@ -349,7 +364,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded soup and rays, generated in-process)",
+        "data": "synthetic (seeded soup and rays, generated in-process; seeds: soup 1, rays 2, "
+                "Cornell scene 3, single frame 11, jobs 0..63)",
         "config": {
             "workload": "ray-cast microbench: 1M-triangle soup, 2^23 incoherent closest-hit rays per step",
             "triangles": N_TRIS,
@@ -385,8 +401,16 @@ def main():
         sample = incoherent[: args.cpu_sample]
         c0 = time.perf_counter()
         for _ in range(args.cpu_passes):
-            ora.intersect(sample, "closest", threads=threads)
+            exp = ora.intersect(sample, "closest", threads=threads)
         cpu_s = time.perf_counter() - c0
+        # Hit-ID parity of the timed GPU results against the oracle on the same rays.
+        got = hits[: sample.shape[0]]
+        same = (got["hit"] == exp["hit"]) & (got["mesh_id"] == exp["mesh_id"]) & (got["local_id"] == exp["local_id"])
+        exact = same & (got["t"] == exp["t"]) & (got["u"] == exp["u"]) & (got["v"] == exp["v"])
+        t_ok = (got["hit"] == exp["hit"]) & ((exp["hit"] == 0) | (np.abs(got["t"] - exp["t"]) <= 1e-5 * np.abs(exp["t"])))
+        out["extra"]["parity_vs_oracle"] = {"rays": int(sample.shape[0]), "same_hit_id": round(float(same.mean()), 7),
+                                            "bit_exact_hit": round(float(exact.mean()), 7),
+                                            "t_within_1e-5": round(float(t_ok.mean()), 7)}
         out["cpu_baseline"] = {
             "value": round(args.cpu_passes * sample.shape[0] / cpu_s / 1e6, 4),
             "unit": "MRays/s",

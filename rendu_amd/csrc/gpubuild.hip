@@ -246,14 +246,9 @@ __device__ __forceinline__ float halfArea(float4 mn, float4 mx) {
 // One level of the 4-wide collapse: each queued (binary node, wide slot) takes its two children and
 // expands the internal child of largest surface area until it holds 4 (or only leaves remain),
 // writes the wide node and queues its internal children with fresh slots.
-__global__ __launch_bounds__(kGbBlock) void gbCollapse(const uint2 * __restrict__ qIn, const uint32_t * __restrict__ countIn,
-	uint2 * __restrict__ qOut, uint32_t * __restrict__ countOut, uint32_t * __restrict__ wideCount,
-	const int2 * __restrict__ child, const float4 * __restrict__ leafBox, const float4 * __restrict__ innerBox,
-	float4 * __restrict__ wide) {
-	const uint32_t i = blockIdx.x * kGbBlock + threadIdx.x;
-	if(i >= *countIn)
-		return;
-	const uint2 it = qIn[i];
+__device__ __forceinline__ void collapseOne(const uint2 it, uint2 * __restrict__ qOut, uint32_t * __restrict__ countOut,
+	uint32_t * __restrict__ wideCount, const int2 * __restrict__ child, const float4 * __restrict__ leafBox,
+	const float4 * __restrict__ innerBox, float4 * __restrict__ wide) {
 	int kids[4];
 	const int2 c0 = child[it.x];
 	kids[0] = c0.x;
@@ -318,6 +313,17 @@ __global__ __launch_bounds__(kGbBlock) void gbCollapse(const uint2 * __restrict_
 	o[6] = make_float4(__int_as_float(codes[0]), __int_as_float(codes[1]), __int_as_float(codes[2]),
 		__int_as_float(codes[3]));
 	o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Grid-stride over the level's queue, whose length is read on the device: levels can be launched
+// back to back without the host knowing their sizes.
+__global__ __launch_bounds__(kGbBlock) void gbCollapse(const uint2 * __restrict__ qIn, const uint32_t * __restrict__ countIn,
+	uint2 * __restrict__ qOut, uint32_t * __restrict__ countOut, uint32_t * __restrict__ wideCount,
+	const int2 * __restrict__ child, const float4 * __restrict__ leafBox, const float4 * __restrict__ innerBox,
+	float4 * __restrict__ wide) {
+	const uint32_t count = *countIn;
+	for(uint32_t i = blockIdx.x * kGbBlock + threadIdx.x; i < count; i += gridDim.x * kGbBlock)
+		collapseOne(qIn[i], qOut, countOut, wideCount, child, leafBox, innerBox, wide);
 }
 
 __global__ void gbZero(uint32_t * p, size_t n) {
@@ -454,17 +460,27 @@ int gpuBuildBvh(const HostScene & hs, GpuBvh & out, hipStream_t s) {
 	uint2 * q[2] = {ar.at<uint2>(oQA), ar.at<uint2>(oQB)};
 	uint32_t levels = 0, pending = 1;
 	int cur = 0;
+	// Levels go in batches of 16 launches with a fixed grid (grid-stride inside); the host reads the
+	// last level's count once per batch. Levels past the last non-empty one find an empty queue.
+	const unsigned grid = gbBlocks(ni) < 2048u ? gbBlocks(ni) : 2048u;
+	uint32_t sizes[16];
 	while(pending > 0) {
 		if(levels > 4096)
 			return fail(RDU_ERR_UNSUPPORTED, "GPU build: tree deeper than 4096 levels");
-		hipLaunchKernelGGL(gbZero, dim3(1), dim3(64), 0, s, dc + (1 - cur), size_t(1));
-		hipLaunchKernelGGL(gbCollapse, dim3(gbBlocks(pending)), dim3(kGbBlock), 0, s, q[cur], dc + cur, q[1 - cur],
-			dc + (1 - cur), dc + 2, childC, leafC, innerC, static_cast<float4 *>(out.wide.ptr));
+		for(int k = 0; k < 16; ++k) {
+			hipLaunchKernelGGL(gbZero, dim3(1), dim3(64), 0, s, dc + (1 - cur), size_t(1));
+			hipLaunchKernelGGL(gbCollapse, dim3(grid), dim3(kGbBlock), 0, s, q[cur], dc + cur, q[1 - cur], dc + (1 - cur),
+				dc + 2, childC, leafC, innerC, static_cast<float4 *>(out.wide.ptr));
+			GB_CHECK(hipMemcpyAsync(sizes + k, dc + (1 - cur), 4, hipMemcpyDeviceToHost, s));
+			cur = 1 - cur;
+		}
 		GB_CHECK(hipGetLastError());
-		GB_CHECK(hipMemcpyAsync(&pending, dc + (1 - cur), 4, hipMemcpyDeviceToHost, s));
 		GB_CHECK(hipStreamSynchronize(s));
-		++levels;
-		cur = 1 - cur;
+		// Depth = levels with a non-empty input queue.
+		for(int k = 0; k < 16 && pending > 0; ++k) {
+			++levels;
+			pending = sizes[k];
+		}
 	}
 	uint32_t wideCount = 0;
 	float4 root[2];
