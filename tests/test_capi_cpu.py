@@ -132,3 +132,43 @@ def test_queries_fail_loudly_without_a_gpu():
     assert e.value.code == _lib.RDU_ERR_HIP
     with pytest.raises(_lib.RduError):
         rc.intersects([0.5, 0.5, -1], [0, 0, 1])
+
+
+def test_extension_entry_points_reject_bad_arguments():
+    """Argument and state errors of the extension entry points are reported without a GPU."""
+    L = _lib.lib
+    c_void = ctypes.c_void_p
+    # Radiance cache: null arguments, bad learning rate.
+    lo = np.zeros(3, F32)
+    assert L.rdu_rcache_create(None, _lib.fptr(lo), _lib.fptr(lo), 0) == _lib.RDU_ERR_INVALID
+    assert L.rdu_rcache_set_learning_rate(None, ctypes.c_float(1e-3)) == _lib.RDU_ERR_INVALID
+    assert L.rdu_rcache_infer(None, None, 1, None, 0, None) == _lib.RDU_ERR_INVALID
+    assert L.rdu_rcache_train(None, None, None, 0, 1, 0, None, None) == _lib.RDU_ERR_INVALID
+    assert L.rdu_rcache_get_params(None, None, None, None) == _lib.RDU_ERR_INVALID
+    # Output step: components out of range, empty image is fine.
+    assert L.rdu_image_to_ldr(None, 4, 4, 5, None, 0, None) == _lib.RDU_ERR_INVALID
+    assert L.rdu_image_to_ldr(None, 0, 4, 3, None, 0, None) == _lib.RDU_OK
+    # Scene: GPU-build switch and cached render on a null / uncommitted scene.
+    assert L.rdu_scene_set_gpu_build(None, 1) == _lib.RDU_ERR_INVALID
+    h = c_void()
+    assert L.rdu_scene_create(ctypes.byref(h)) == _lib.RDU_OK
+    try:
+        assert L.rdu_scene_set_gpu_build(h, 1) == _lib.RDU_OK
+        assert L.rdu_pathtracer_render_cached(h, None, None, 1, 1, None, 1, 1, 3, None, None) == _lib.RDU_ERR_INVALID
+        stats = (ctypes.c_uint64 * 5)()
+        assert L.rdu_scene_last_cache_stats(h, ctypes.byref(stats)) == _lib.RDU_OK
+        n = ctypes.c_size_t(7)
+        assert L.rdu_scene_last_cache_pairs(h, None, None, 0, ctypes.byref(n)) == _lib.RDU_OK and n.value == 0
+    finally:
+        L.rdu_scene_destroy(h)
+    # Raycaster: GPU build twice / with no triangles, timing on a null handle.
+    rc = c_void()
+    assert L.rdu_raycaster_create(ctypes.byref(rc)) == _lib.RDU_OK
+    try:
+        assert L.rdu_raycaster_update_hierarchy_gpu(rc, None) == _lib.RDU_ERR_INVALID  # no triangles
+    finally:
+        L.rdu_raycaster_destroy(rc)
+    assert L.rdu_raycaster_update_hierarchy_gpu(None, None) == _lib.RDU_ERR_INVALID
+    assert L.rdu_raycaster_set_kernel_timing(None, 1) == _lib.RDU_ERR_INVALID
+    ms, k = ctypes.c_double(), ctypes.c_uint64()
+    assert L.rdu_raycaster_kernel_time(None, ctypes.byref(ms), ctypes.byref(k)) == _lib.RDU_ERR_INVALID
