@@ -253,6 +253,12 @@ def main():
         "incoherent_any_mrays": rate(timed(lambda: anyhit(d_inc), steps2, 1)[0], steps2),
         "coherent_any_mrays": rate(timed(lambda: anyhit(d_coh, incoh=False), steps2, 1)[0], steps2),
     }
+    # Host buffers in and out (PCIe-inclusive: 32 B per ray up, 24 B per hit down); not the metric.
+    rc.intersect_batch(incoherent[: 1 << 20], incoherent=True)
+    t0 = time.perf_counter()
+    for _ in range(2):
+        rc.intersect_batch(incoherent, incoherent=True)
+    extra["incoherent_closest_host_buffers_mrays"] = round(2 * N_HALF / (time.perf_counter() - t0) / 1e6, 2)
     if args.tree == "rendu":  # Rendu's visiting order exists only on Rendu's tree
         extra["incoherent_closest_reference_order_mrays"] = rate(timed(lambda: closest(d_inc, ref=True), 2, 1)[0], 2)
     sub = np.ascontiguousarray(incoherent[: 1 << 20])
@@ -305,6 +311,13 @@ def main():
     extra["cornell_256_16spp_depth5_msamples_per_s"] = round(best["paths"] / best["seconds"] / 1e6, 2)
     extra["cornell_256_16spp_depth5_ms"] = round(best["seconds"] * 1e3, 3)
     extra["cornell_rays_per_sample"] = round((best["extension_rays"] + best["shadow_rays"]) / best["paths"], 3)
+    del pt
+    # End to end (PathTracer construction = raycaster build + upload, then the render).
+    t0 = time.perf_counter()
+    pt = PathTracer(scn)
+    img[:] = 0
+    pt.render(cam_of(256, 256), 16, 5, img, seed=9)
+    extra["cornell_256_16spp_depth5_end_to_end_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
     del pt
 
     # Single frame: fBm terrain + 2000 icospheres (3.08M triangles), lat-long sky, 1920x1080,
@@ -411,6 +424,33 @@ def main():
         out["extra"]["parity_vs_oracle"] = {"rays": int(sample.shape[0]), "same_hit_id": round(float(same.mean()), 7),
                                             "bit_exact_hit": round(float(exact.mean()), 7),
                                             "t_within_1e-5": round(float(t_ok.mean()), 7)}
+        # The path tracer on the host cores (oracle), on bounded samples scaled explicitly: the
+        # Cornell workload at full size, and a 240 x 135 crop-equivalent of the single frame
+        # (same scene and camera, 8 spp) scaled to 1920 x 1080 x 64 spp.
+        from rendu_amd import scene as S
+        from rendu_amd import scenes
+        scn, cam_of = scenes.cornell_spheres(seed=3, subdivisions=4)
+        opt = orc.OraclePathTracer(scn)
+        im = np.zeros((256, 256, 3), np.float32)
+        c0 = time.perf_counter()
+        opt.render(S.camera_desc(cam_of(256, 256)), 16, 5, im, S.RenderOpts(1, 0, 3, 0), threads=threads)
+        cs = time.perf_counter() - c0
+        scn3, cam3 = scenes.terrain_spheres()
+        opt3 = orc.OraclePathTracer(scn3)
+        im3 = np.zeros((135, 240, 3), np.float32)
+        c0 = time.perf_counter()
+        opt3.render(S.camera_desc(cam3(240, 135)), 8, 8, im3, S.RenderOpts(2, S.ESTIMATOR_EXTENDED, 3, 0), threads=threads)
+        fs = time.perf_counter() - c0
+        rate3 = 240 * 135 * 8 / fs
+        out["extra"]["cpu_path_tracer"] = {
+            "threads": threads,
+            "cornell_256_16spp_depth5_s": round(cs, 3),
+            "cornell_msamples_per_s": round(256 * 256 * 16 / cs / 1e6, 4),
+            "frame_sample": "240x135, 8 spp, depth 8, RR 3 (same scene and camera)",
+            "frame_sample_s": round(fs, 3),
+            "frame_msamples_per_s": round(rate3 / 1e6, 4),
+            "frame_1080p_64spp_s_scaled": round(1920 * 1080 * 64 / rate3, 1),
+        }
         out["cpu_baseline"] = {
             "value": round(args.cpu_passes * sample.shape[0] / cpu_s / 1e6, 4),
             "unit": "MRays/s",
