@@ -319,6 +319,32 @@ def main():
     pt.render(cam_of(256, 256), 16, 5, img, seed=9)
     extra["cornell_256_16spp_depth5_end_to_end_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
     del pt
+    # HIP graph replay of render passes, eager against replayed: the Cornell workload in 16 passes
+    # of one sample each, and a progressive loop of 16 one-sample renders with new seeds (the way an
+    # interactive viewer calls render).
+    graph = {}
+    img1 = np.zeros((256, 256, 3), np.float32)
+    cam = cam_of(256, 256)
+    for mode in ("eager", "graph"):
+        ptg = PathTracer(scn)
+        ptg.set_graph(mode == "graph")
+        for _ in range(2):  # warm-up (the graph is captured here)
+            img[:] = 0
+            ptg.render(cam, 16, 5, img, seed=1, max_paths=256 * 256)
+        times = []
+        for rep in range(3):
+            img[:] = 0
+            st = ptg.render(cam, 16, 5, img, seed=1, max_paths=256 * 256)
+            times.append(st["seconds"])
+        graph[f"cornell_16_passes_{mode}_ms"] = round(min(times) * 1e3, 3)
+        graph[f"cornell_16_passes_{mode}_replayed"] = st["graph_passes"]
+        img1[:] = 0
+        t0 = time.perf_counter()
+        for k in range(16):
+            ptg.render(cam, 1, 5, img1, seed=100 + k)
+        graph[f"progressive_16x1spp_{mode}_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
+        del ptg
+    extra["graph_replay"] = graph
 
     # Single frame: fBm terrain + 2000 icospheres (3.08M triangles), lat-long sky, 1920x1080,
     # 64 spp, depth 8, Russian roulette from bounce 3.
@@ -340,6 +366,28 @@ def main():
     extra["frame_1080p_mrays_per_s"] = round(rays3 / frame_s / 1e6, 2)
     extra["frame_1080p_rays_per_sample"] = round(rays3 / st3["paths"], 3)
     extra["frame_scene_setup_s"] = round(setup3, 3)
+    # The pass autotuner on this frame (one 2-spp pass timed per candidate), then the frame with the
+    # chosen parameters, and with graph replay of its passes.
+    t0 = time.perf_counter()
+    tune3 = pt3.autotune(cam3(1920, 1080), 64, 8, 1920, 1080, seed=2, estimator=ESTIMATOR_EXTENDED, rr_start=3)
+    tune3_s = time.perf_counter() - t0
+    img3[:] = 0
+    w0 = time.perf_counter()
+    pt3.render(cam3(1920, 1080), 64, 8, img3, seed=2, estimator=ESTIMATOR_EXTENDED, rr_start=3)
+    tuned_s = time.perf_counter() - w0
+    pt3.set_graph(True)
+    img3[:] = 0
+    pt3.render(cam3(1920, 1080), 64, 8, img3, seed=3, estimator=ESTIMATOR_EXTENDED, rr_start=3)  # captures
+    img3[:] = 0
+    w0 = time.perf_counter()
+    stg = pt3.render(cam3(1920, 1080), 64, 8, img3, seed=2, estimator=ESTIMATOR_EXTENDED, rr_start=3)
+    graph_s = time.perf_counter() - w0
+    pt3.set_graph(False)
+    extra["frame_autotune"] = {k: tune3[k] for k in ("refill_min", "chunk", "blocks_per_cu", "candidates")}
+    extra["frame_autotune"].update({"pass_default_ms": round(tune3["default_ms"], 3),
+                                    "pass_best_ms": round(tune3["best_ms"], 3), "tuning_s": round(tune3_s, 3),
+                                    "frame_tuned_s": round(tuned_s, 4), "frame_tuned_graph_s": round(graph_s, 4),
+                                    "graph_passes": stg["graph_passes"], "passes": stg["passes"]})
     if not args.no_cache_experiment:
         extra["cache_1080p"] = cache_experiment(pt3, scn3, cam3(1920, 1080), args.cache_ref_spp)
     del pt3
@@ -389,6 +437,8 @@ def main():
             "visiting_order": "near-first, 4-wide, rays Morton-ordered in the timed region",
             "tree": "rendu (host build, updateHierarchy)" if args.tree == "rendu" else "gpu linear BVH",
             "refill_min": tune["refill_min"],
+            "chunk": tune.get("chunk", 0),
+            "blocks_per_cu": tune.get("blocks_per_cu", 0),
         },
         "roofline": {
             "bound": "hbm",

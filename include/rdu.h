@@ -256,22 +256,62 @@ typedef struct {
 	uint64_t extension_rays; /* closest-hit queries */
 	uint64_t shadow_rays;    /* visibility queries */
 	uint32_t samples;        /* spp after rounding */
+	uint32_t passes;         /* passes of at most max_paths paths */
+	uint32_t graph_passes;   /* passes replayed from a captured HIP graph */
+	uint32_t graph_captures; /* graphs captured by this render */
 } rdu_render_stats;
 int rdu_scene_last_render_stats(const rdu_scene* scene, rdu_render_stats* out);
 
-/* Autotuner (no reference counterpart). Times the near-first traversal on the caller's batch for
- * each candidate of the wavefront refill threshold (idle lanes a persistent wavefront waits for
- * before fetching new rays) and keeps the fastest for later queries of the same class
- * (RDU_INCOHERENT or not, closest or any hit) on this raycaster. flags as for the batch queries;
- * the batch is only read. Not thread-safe against concurrent queries on the same raycaster. */
+/* HIP graph replay of render passes (no reference counterpart). When enabled, a plain render's
+ * full passes (max_paths paths each; a shorter last pass runs eagerly) are captured once into a
+ * HIP graph and replayed: a pass whose launch configuration (camera frame, size, depth,
+ * estimator, buffers, traversal tuning) equals the previous full pass's is captured at that point
+ * and replayed for every later pass with the same configuration, within a render and across
+ * renders of the scene (also with another seed: progressive renders replay). Each pass reads its
+ * first sample index and the seed from device memory, so a replay renders exactly what the eager
+ * pass would. Cached renders always run eagerly. Off by default. */
+int rdu_scene_set_graph(rdu_scene* scene, int enable);
+
+
+/* Autotuner (no reference counterpart: Rendu's traversal has no launch parameters). Times the
+ * near-first persistent traversal on the caller's batch over candidate launch parameters by
+ * coordinate descent: the wavefront refill threshold (idle lanes a wavefront waits for before
+ * fetching new rays: 4..64), the ray tile (rays per interleaved run of a wavefront: 32..512) and
+ * the grid size (resident workgroups per CU: 1/4..3/4 of the occupancy limit). Keeps the fastest
+ * for later queries of the same class (RDU_INCOHERENT or not, closest or any hit) on this
+ * raycaster, and stores it in the process-wide tuning cache under the scene's statistics
+ * (triangle count, 4-wide tree depth, mesh count, each as a power-of-two bucket where a count;
+ * batch size bucket; class; host- or GPU-built tree). Batches of a class this raycaster has not
+ * tuned take the cache's entry for their statistics, if any. flags as for the batch queries; the
+ * batch is only read. Not thread-safe against concurrent queries on the same raycaster. */
 typedef struct {
-	uint32_t refill_min; /* chosen threshold */
-	float best_ms;       /* mean time of one batch with it */
-	float default_ms;    /* mean time with the untuned default */
-	uint32_t candidates; /* thresholds tried */
+	uint32_t refill_min;    /* chosen threshold (0 = default: 32 sorted batches, 16 input order) */
+	float best_ms;          /* mean time of one batch with the chosen parameters */
+	float default_ms;       /* mean time with the untuned defaults */
+	uint32_t candidates;    /* parameter sets timed */
+	uint32_t chunk;         /* chosen ray tile (0 = default: 64 sorted, contiguous runs input order) */
+	uint32_t blocks_per_cu; /* chosen workgroups per CU (0 = occupancy limit) */
+	uint32_t source;        /* rdu_raycaster_tuning: 0 defaults, 1 this raycaster, 2 tuning cache */
 } rdu_tune_report;
 int rdu_raycaster_autotune(rdu_raycaster* rc, const rdu_ray* rays, size_t n, int any_hit, uint32_t flags, void* stream,
 	rdu_tune_report* report);
+/* The launch parameters a batch of n rays of this class would use now, and where they come from
+ * (refill_min, chunk, blocks_per_cu, source; the times are 0). */
+int rdu_raycaster_tuning(const rdu_raycaster* rc, size_t n, int any_hit, uint32_t flags, rdu_tune_report* out);
+/* The process-wide tuning cache: save to / merge from a text file (a header line, then one entry
+ * per line), clear, count entries. A malformed file is rejected whole with RDU_ERR_INVALID. */
+int rdu_tune_cache_save(const char* path);
+int rdu_tune_cache_load(const char* path);
+int rdu_tune_cache_clear(void);
+int rdu_tune_cache_size(size_t* entries);
+
+/* Autotuner of render passes (no reference counterpart): times one pass of the given render
+ * configuration (the image is not touched) for each candidate of the traversal launch
+ * parameters, as rdu_raycaster_autotune does for batches, and keeps the fastest for this
+ * scene's later renders; it also enters the process-wide tuning cache under the scene's
+ * statistics and the pass size, where untuned scenes of the same class find it. */
+int rdu_scene_autotune(rdu_scene* scene, const rdu_camera_desc* camera, size_t samples, size_t depth, uint32_t width,
+	uint32_t height, const rdu_render_opts* opts, rdu_tune_report* report);
 
 /* Kernel timing (no reference counterpart; for measurement). While enabled, each batch query
  * records a pair of HIP events on its stream around the traversal kernel alone (not the ray

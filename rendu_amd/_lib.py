@@ -49,7 +49,8 @@ class _Info(ctypes.Structure):
 
 
 class TuneReport(ctypes.Structure):
-    _fields_ = [("refill_min", c_uint32), ("best_ms", c_float), ("default_ms", c_float), ("candidates", c_uint32)]
+    _fields_ = [("refill_min", c_uint32), ("best_ms", c_float), ("default_ms", c_float), ("candidates", c_uint32),
+                ("chunk", c_uint32), ("blocks_per_cu", c_uint32), ("source", c_uint32)]
 
 
 def _load() -> ctypes.CDLL:
@@ -65,6 +66,8 @@ def _load() -> ctypes.CDLL:
         "rdu_raycaster_update_hierarchy": (c_int, [c_void_p]),
         "rdu_raycaster_update_hierarchy_gpu": (c_int, [c_void_p, c_void_p]),
         "rdu_scene_set_gpu_build": (c_int, [c_void_p, c_int]),
+        "rdu_scene_set_graph": (c_int, [c_void_p, c_int]),
+        "rdu_scene_autotune": (c_int, [c_void_p, c_void_p, c_size_t, c_size_t, c_uint32, c_uint32, c_void_p, c_void_p]),
         "rdu_raycaster_intersect": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_uint32, c_void_p]),
         "rdu_raycaster_intersect_any": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_uint32, c_void_p]),
         "rdu_raycaster_intersect1": (c_int, [c_void_p, fp, fp, c_float, c_float, c_void_p]),
@@ -76,6 +79,11 @@ def _load() -> ctypes.CDLL:
         "rdu_raycaster_trace_stats": (c_int, [c_void_p, c_void_p, c_size_t, c_int, c_uint32,
                                               POINTER(c_uint64), POINTER(c_uint64)]),
         "rdu_raycaster_autotune": (c_int, [c_void_p, c_void_p, c_size_t, c_int, c_uint32, c_void_p, c_void_p]),
+        "rdu_raycaster_tuning": (c_int, [c_void_p, c_size_t, c_int, c_uint32, c_void_p]),
+        "rdu_tune_cache_save": (c_int, [c_char_p]),
+        "rdu_tune_cache_load": (c_int, [c_char_p]),
+        "rdu_tune_cache_clear": (c_int, []),
+        "rdu_tune_cache_size": (c_int, [POINTER(c_size_t)]),
         "rdu_raycaster_set_kernel_timing": (c_int, [c_void_p, c_int]),
         "rdu_raycaster_kernel_time": (c_int, [c_void_p, POINTER(ctypes.c_double), POINTER(c_uint64)]),
         "rdu_scene_create": (c_int, [POINTER(c_void_p)]),

@@ -47,6 +47,29 @@ uint32_t perRayFlags(const rdu_raycaster * rc) { return (rc && rc->gpuBuilt) ? 0
 // Incoherent batches below this size are traced in input order: the sort does not pay off.
 constexpr size_t kReorderMin = size_t(1) << 16;
 
+TuneKey tuneKey(const rdu_raycaster * rc, size_t n, uint32_t flags, bool any) {
+	const uint32_t cls = ((flags & RDU_INCOHERENT) ? kTuneIncoherent : 0u) | (any ? kTuneAny : 0u) |
+						 (rc->gpuBuilt ? kTuneGpuTree : 0u);
+	return makeTuneKey(rc->scene.tris.size(), rc->dev.wideDepth, rc->scene.meshCount, n, cls);
+}
+
+// Launch parameters of a batch: this raycaster's own tuning for the class, else the process-wide
+// cache's entry for the scene statistics, else the defaults. source: 0 default, 1 raycaster, 2 cache.
+TraceTuning tuningFor(const rdu_raycaster * rc, size_t n, uint32_t flags, bool any, uint32_t * source) {
+	const int c = (flags & RDU_INCOHERENT) ? 1 : 0, a = any ? 1 : 0;
+	TraceTuning t;
+	uint32_t src = 0;
+	if(rc->tunedSet[c][a]) {
+		t = rc->tuned[c][a];
+		src = 1;
+	} else if(tuneCacheLookup(tuneKey(rc, n, flags, any), t)) {
+		src = 2;
+	}
+	if(source)
+		*source = src;
+	return t;
+}
+
 template<bool ANY>
 int query(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, void * out, uint32_t flags, void * stream,
 	uint64_t * statsOut) {
@@ -62,7 +85,7 @@ int query(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, void * out, 
 	TraceOptions opt = traceOptions(flags);
 	if(rc->gpuBuilt && (opt.referenceOrder || !opt.wide))
 		return fail(RDU_ERR_STATE, "a GPU-built tree has no reference visiting order or binary layout");
-	opt.refillMin = rc->tunedRefill[(flags & RDU_INCOHERENT) ? 1 : 0][ANY ? 1 : 0];
+	applyTuning(opt, tuningFor(rc, n, flags, ANY, nullptr));
 	const size_t outBytes = ANY ? n : n * sizeof(rdu_hit);
 	std::lock_guard<std::mutex> guard(rc->lock);
 	if((st = setDevice(rc->device)))
@@ -398,55 +421,93 @@ int rdu_raycaster_autotune(rdu_raycaster * rc, const rdu_ray * rays, size_t n, i
 	if((st = outBuf.reserve(any_hit ? n : n * sizeof(rdu_hit))))
 		return st;
 	const uint32_t qflags = (flags & ~RDU_DEVICE_POINTERS) | RDU_DEVICE_POINTERS;
-	const uint32_t cls = (flags & RDU_INCOHERENT) ? 1 : 0;
-	const uint32_t candidates[] = {4, 8, 16, 24, 32, 40, 48, 64};
+	const int c = (flags & RDU_INCOHERENT) ? 1 : 0, a = any_hit ? 1 : 0;
 	hipEvent_t e0, e1;
-	if((st = hipCheck(hipEventCreate(&e0), "hipEventCreate")) || (st = hipCheck(hipEventCreate(&e1), "hipEventCreate")))
+	if((st = hipCheck(hipEventCreate(&e0), "hipEventCreate")))
 		return st;
-	float bestMs = 0.0f, defaultMs = 0.0f;
-	uint32_t best = 0;
-	for(uint32_t k = 0; k <= sizeof(candidates) / sizeof(candidates[0]); ++k) {
-		// k == 0: the untuned default; then each candidate.
-		rc->tunedRefill[cls][any_hit ? 1 : 0] = k == 0 ? 0 : candidates[k - 1];
-		float ms = 0.0f;
-		for(int rep = 0; rep < 3 && !st; ++rep) {
-			st = any_hit ? query<true>(rc, dRays, n, outBuf.ptr, qflags, s, nullptr)
-						 : query<false>(rc, dRays, n, outBuf.ptr, qflags, s, nullptr); // warm-up / first rep
-			if(st || rep == 0)
-				continue;
-			if((st = hipCheck(hipEventRecord(e0, s), "hipEventRecord")))
-				break;
-			st = any_hit ? query<true>(rc, dRays, n, outBuf.ptr, qflags, s, nullptr)
-						 : query<false>(rc, dRays, n, outBuf.ptr, qflags, s, nullptr);
-			if(!st)
-				st = hipCheck(hipEventRecord(e1, s), "hipEventRecord");
-			if(!st)
-				st = hipCheck(hipEventSynchronize(e1), "hipEventSynchronize");
-			float t = 0.0f;
-			if(!st)
-				st = hipCheck(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
-			ms += t;
-		}
-		if(st)
-			break;
-		if(k == 0)
-			defaultMs = ms;
-		else if(best == 0 || ms < bestMs) {
-			bestMs = ms;
-			best = candidates[k - 1];
-		}
+	if((st = hipCheck(hipEventCreate(&e1), "hipEventCreate"))) {
+		(void)hipEventDestroy(e0);
+		return st;
 	}
+	// One candidate: a warm-up query, then the mean of two timed ones.
+	auto timeOne = [&](const TraceTuning & t, float & ms) -> int {
+		rc->tuned[c][a] = t;
+		rc->tunedSet[c][a] = true;
+		ms = 0.0f;
+		int e = RDU_OK;
+		for(int rep = 0; rep < 3 && !e; ++rep) {
+			if(rep > 0 && (e = hipCheck(hipEventRecord(e0, s), "hipEventRecord")))
+				break;
+			e = any_hit ? query<true>(rc, dRays, n, outBuf.ptr, qflags, s, nullptr)
+						: query<false>(rc, dRays, n, outBuf.ptr, qflags, s, nullptr);
+			if(e || rep == 0)
+				continue;
+			float t1 = 0.0f;
+			if(!(e = hipCheck(hipEventRecord(e1, s), "hipEventRecord")) &&
+				!(e = hipCheck(hipEventSynchronize(e1), "hipEventSynchronize")) &&
+				!(e = hipCheck(hipEventElapsedTime(&t1, e0, e1), "hipEventElapsedTime")))
+				ms += 0.5f * t1;
+		}
+		return e;
+	};
+	TuneResult res;
+	st = tuneSearch(timeOne, persistentOccupancy(any_hit != 0), res);
 	(void)hipEventDestroy(e0);
 	(void)hipEventDestroy(e1);
-	rc->tunedRefill[cls][any_hit ? 1 : 0] = st ? 0 : best;
+	rc->tuned[c][a] = st ? TraceTuning{} : res.best;
+	rc->tunedSet[c][a] = st == RDU_OK;
 	if(st)
 		return st;
+	tuneCacheStore(tuneKey(rc, n, qflags, any_hit != 0), res.best);
 	if(report) {
-		report->refill_min = best;
-		report->best_ms = bestMs / 2.0f;
-		report->default_ms = defaultMs / 2.0f;
-		report->candidates = uint32_t(sizeof(candidates) / sizeof(candidates[0]));
+		report->refill_min = res.best.refillMin;
+		report->best_ms = res.bestMs;
+		report->default_ms = res.defaultMs;
+		report->candidates = res.timed;
+		report->chunk = res.best.chunk;
+		report->blocks_per_cu = res.best.blocksPerCu;
+		report->source = 1;
 	}
+	return RDU_OK;
+}
+
+int rdu_raycaster_tuning(const rdu_raycaster * rc, size_t n, int any_hit, uint32_t flags, rdu_tune_report * out) {
+	clearError();
+	if(!rc || !out)
+		return fail(RDU_ERR_INVALID, "null raycaster or report");
+	if(!rc->scene.built && !rc->gpuBuilt)
+		return fail(RDU_ERR_STATE, "tuning before updateHierarchy");
+	uint32_t src = 0;
+	const TraceTuning t = tuningFor(rc, n, flags, any_hit != 0, &src);
+	*out = rdu_tune_report{};
+	out->refill_min = t.refillMin;
+	out->chunk = t.chunk;
+	out->blocks_per_cu = t.blocksPerCu;
+	out->source = src;
+	return RDU_OK;
+}
+
+int rdu_tune_cache_save(const char * path) {
+	clearError();
+	return tuneCacheSave(path);
+}
+
+int rdu_tune_cache_load(const char * path) {
+	clearError();
+	return tuneCacheLoad(path);
+}
+
+int rdu_tune_cache_clear(void) {
+	clearError();
+	tuneCacheClear();
+	return RDU_OK;
+}
+
+int rdu_tune_cache_size(size_t * entries) {
+	clearError();
+	if(!entries)
+		return fail(RDU_ERR_INVALID, "null output");
+	*entries = tuneCacheSize();
 	return RDU_OK;
 }
 

@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 struct rdu_scene {
@@ -41,7 +42,27 @@ struct rdu_scene {
 	rdu_cache_stats lastCache{};
 	size_t lastTrainPairs = 0; // pairs left in cTrainQ / cTrainT by the last cached pass
 	hipStream_t stream = nullptr; // renders of this scene run here (created on first render)
+	// Graph replay of full passes (rdu_scene_set_graph): the captured pass, the launch
+	// configuration it was captured with, and that of the last eager full pass.
+	bool useGraph = false;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t graphExec = nullptr;
+	std::vector<unsigned char> graphKey, lastPassKey;
+	// Autotuned traversal parameters of this scene's passes (rdu_scene_autotune).
+	rdu::TraceTuning tuned;
+	bool tunedSet = false;
+	void dropGraph() {
+		if(graphExec)
+			(void)hipGraphExecDestroy(graphExec);
+		if(graph)
+			(void)hipGraphDestroy(graph);
+		graphExec = nullptr;
+		graph = nullptr;
+		graphKey.clear();
+		lastPassKey.clear();
+	}
 	~rdu_scene() {
+		dropGraph();
 		if(stream)
 			(void)hipStreamDestroy(stream);
 		if(rc)
@@ -115,6 +136,58 @@ int ensureCacheBuffers(rdu_scene * s, size_t P, size_t records) {
 		(st = s->cHoldQ.reserve(hold * 32)) || (st = s->cHoldT.reserve(hold * 16)) || (st = s->cCounts.reserve(kCacheSlots * 4)))
 		return st;
 	s->cacheCap = P;
+	return RDU_OK;
+}
+
+
+template<typename T>
+void appendBytes(std::vector<unsigned char> & k, const T & v) {
+	const unsigned char * p = reinterpret_cast<const unsigned char *>(&v);
+	k.insert(k.end(), p, p + sizeof(T));
+}
+
+// Everything a pass's launches depend on besides the committed scene (fixed after commit): two
+// passes with equal keys enqueue identical kernels with identical arguments.
+std::vector<unsigned char> passKey(const PtFrame & f, const PtOptions & po, const PtPaths & P, const PtScratch & scratch,
+	const float * dImage, uint32_t samplesInPass, uint32_t components, const TraceTuning & tune) {
+	std::vector<unsigned char> k;
+	appendBytes(k, f);
+	appendBytes(k, po);
+	appendBytes(k, P);
+	appendBytes(k, scratch.gstack);
+	appendBytes(k, dImage);
+	appendBytes(k, samplesInPass);
+	appendBytes(k, components);
+	appendBytes(k, tune);
+	return k;
+}
+
+TuneKey sceneTuneKey(const rdu_scene * s, size_t passPaths) {
+	const uint32_t cls = kTunePathPass | (s->rc->gpuBuilt ? kTuneGpuTree : 0u);
+	return makeTuneKey(s->rc->scene.tris.size(), s->rc->dev.wideDepth, s->rc->scene.meshCount, passPaths, cls);
+}
+
+// Captures what `enqueue` launches on `stream` into the scene's graph (replacing any previous one).
+int capturePass(rdu_scene * s, hipStream_t stream, const std::function<int()> & enqueue) {
+	s->dropGraph();
+	int st = hipCheck(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+	if(st)
+		return st;
+	const int e = enqueue();
+	hipGraph_t g = nullptr;
+	const hipError_t ee = hipStreamEndCapture(stream, &g);
+	if(e || ee != hipSuccess) {
+		if(g)
+			(void)hipGraphDestroy(g);
+		return hipCheck(e ? hipError_t(e) : ee, "pass capture");
+	}
+	hipGraphExec_t x = nullptr;
+	if((st = hipCheck(hipGraphInstantiate(&x, g, nullptr, nullptr, 0), "hipGraphInstantiate"))) {
+		(void)hipGraphDestroy(g);
+		return st;
+	}
+	s->graph = g;
+	s->graphExec = x;
 	return RDU_OK;
 }
 
@@ -319,16 +392,19 @@ int rdu_scene_commit(rdu_scene * s) {
 
 const rdu_raycaster * rdu_scene_raycaster(const rdu_scene * s) { return s ? s->rc : nullptr; }
 
+
+// tuneOut: instead of rendering, time one pass per candidate of the traversal launch parameters
+// (image unused) and keep the fastest for the scene.
 static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples, size_t depth, float * image,
 	uint32_t width, uint32_t height, uint32_t components, const rdu_render_opts * opts, rdu_rcache * cache,
-	const rdu_cache_opts * copts) {
+	const rdu_cache_opts * copts, TuneResult * tuneOut = nullptr) {
 	clearError();
 	const auto t0 = std::chrono::steady_clock::now();
 	if(!s)
 		return fail(RDU_ERR_INVALID, "[PathTracer] No scene available.");
 	if(!s->committed)
 		return fail(RDU_ERR_STATE, "scene not committed");
-	if(!cam || !image)
+	if(!cam || (!image && !tuneOut))
 		return fail(RDU_ERR_INVALID, "null camera or image");
 	if(components < 3)
 		return fail(RDU_ERR_INVALID, "[PathTracer] Expected a RGB image.");
@@ -356,7 +432,6 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 	PtFrame f;
 	f.width = width;
 	f.height = height;
-	f.seed = opt.seed;
 	f.eye = eye;
 	f.corner = center + imageDist * ((-widthScale) * right + heightScale * up);
 	f.dx = ((2.0f * widthScale) * imageDist) * right;
@@ -390,7 +465,7 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 	if(!s->stream && (st = hipCheck(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate")))
 		return st;
 	hipStream_t stream = s->stream;
-	if((st = hipCheck(hipMemcpyAsync(s->pImage.ptr, image, imgBytes, hipMemcpyHostToDevice, stream), "image upload")))
+	if(!tuneOut && (st = hipCheck(hipMemcpyAsync(s->pImage.ptr, image, imgBytes, hipMemcpyHostToDevice, stream), "image upload")))
 		return st;
 	if((st = hipCheck(hipMemsetAsync(s->pTotals.ptr, 0, 16, stream), "hipMemsetAsync")))
 		return st;
@@ -454,7 +529,60 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 		C.holdT = static_cast<float4 *>(s->cHoldT.ptr);
 		C.cells = rcacheCells(cache);
 	}
-	uint32_t pass = 0;
+	const size_t passPaths = pixels * perPass;
+	float * dImage = static_cast<float *>(s->pImage.ptr);
+	if((st = hipCheck(hipError_t(ptSetSampleBase(P.counts, 0, opt.seed, stream)), "sample base")))
+		return st;
+	if(tuneOut) {
+		const uint32_t n = uint32_t(perPass < spp ? perPass : spp);
+		hipEvent_t e0, e1;
+		if((st = hipCheck(hipEventCreate(&e0), "hipEventCreate")))
+			return st;
+		if((st = hipCheck(hipEventCreate(&e1), "hipEventCreate"))) {
+			(void)hipEventDestroy(e0);
+			return st;
+		}
+		// One candidate: a warm-up pass, then the mean of two timed ones.
+		auto timeOne = [&](const TraceTuning & t, float & ms) -> int {
+			ms = 0.0f;
+			int e = RDU_OK;
+			for(int rep = 0; rep < 3 && !e; ++rep) {
+				if((e = hipCheck(hipError_t(ptSetSampleBase(P.counts, 0, opt.seed, stream)), "sample base")))
+					break;
+				if(rep > 0 && (e = hipCheck(hipEventRecord(e0, stream), "hipEventRecord")))
+					break;
+				if((e = hipCheck(hipError_t(ptRenderPass(s->dev, s->rc->dev, f, P, n, po, dImage, components, scratch, nullptr,
+									 t, stream)),
+						"path tracing pass")))
+					break;
+				if(rep == 0)
+					continue;
+				float t1 = 0.0f;
+				if(!(e = hipCheck(hipEventRecord(e1, stream), "hipEventRecord")) &&
+					!(e = hipCheck(hipEventSynchronize(e1), "hipEventSynchronize")) &&
+					!(e = hipCheck(hipEventElapsedTime(&t1, e0, e1), "hipEventElapsedTime")))
+					ms += 0.5f * t1;
+			}
+			return e;
+		};
+		st = tuneSearch(timeOne, persistentOccupancy(false), *tuneOut);
+		(void)hipEventDestroy(e0);
+		(void)hipEventDestroy(e1);
+		if(st)
+			return st;
+		s->tuned = tuneOut->best;
+		s->tunedSet = true;
+		tuneCacheStore(sceneTuneKey(s, passPaths), tuneOut->best);
+		return hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+	}
+	// Traversal launch parameters: the scene's own tuning, else the tuning cache's entry for the
+	// scene's statistics and pass size, else the defaults.
+	TraceTuning tune;
+	if(s->tunedSet)
+		tune = s->tuned;
+	else
+		(void)tuneCacheLookup(sceneTuneKey(s, passPaths), tune);
+	uint32_t pass = 0, graphPasses = 0, graphCaptures = 0;
 	for(size_t base = 0; base < spp; base += perPass, ++pass) {
 		const size_t n = (spp - base) < perPass ? (spp - base) : perPass;
 		if(cache) {
@@ -463,10 +591,31 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 			C.trainX = k & 3u;
 			C.trainY = (k >> 2) & 3u;
 		}
-		if((st = hipCheck(hipError_t(ptRenderPass(s->dev, s->rc->dev, f, P, uint32_t(base), uint32_t(n), po,
-								  static_cast<float *>(s->pImage.ptr), components, scratch, cache ? &C : nullptr, stream)),
-				"path tracing pass")))
+		auto enqueue = [&]() -> int {
+			return ptRenderPass(s->dev, s->rc->dev, f, P, uint32_t(n), po, dImage, components, scratch, cache ? &C : nullptr,
+				tune, stream);
+		};
+		if(s->useGraph && !cache && n == perPass) {
+			// A configuration seen on the previous full pass is captured once, then replayed.
+			std::vector<unsigned char> key = passKey(f, po, P, scratch, dImage, uint32_t(n), components, tune);
+			if(!(s->graphExec && key == s->graphKey) && key == s->lastPassKey) {
+				if((st = capturePass(s, stream, enqueue)))
+					return st;
+				s->graphKey = key;
+				++graphCaptures;
+			}
+			if(s->graphExec && key == s->graphKey) {
+				if((st = hipCheck(hipGraphLaunch(s->graphExec, stream), "hipGraphLaunch")))
+					return st;
+				++graphPasses;
+			} else {
+				if((st = hipCheck(hipError_t(enqueue()), "path tracing pass")))
+					return st;
+				s->lastPassKey = std::move(key);
+			}
+		} else if((st = hipCheck(hipError_t(enqueue()), "path tracing pass"))) {
 			return st;
+		}
 		if(cache) {
 			uint32_t counts[kCacheSlots] = {0, 0, 0, 0};
 			if((st = hipCheck(hipMemcpyAsync(counts, C.counts, sizeof(counts), hipMemcpyDeviceToHost, stream), "counts")) ||
@@ -521,12 +670,44 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 	s->last.extension_rays = totals[0];
 	s->last.shadow_rays = totals[1];
 	s->last.samples = uint32_t(spp);
+	s->last.passes = pass;
+	s->last.graph_passes = graphPasses;
+	s->last.graph_captures = graphCaptures;
 	return RDU_OK;
 }
 
 int rdu_pathtracer_render(rdu_scene * s, const rdu_camera_desc * cam, size_t samples, size_t depth, float * image,
 	uint32_t width, uint32_t height, uint32_t components, const rdu_render_opts * opts) {
 	return renderImpl(s, cam, samples, depth, image, width, height, components, opts, nullptr, nullptr);
+}
+
+int rdu_scene_set_graph(rdu_scene * s, int enable) {
+	clearError();
+	if(!s)
+		return fail(RDU_ERR_INVALID, "null scene");
+	s->useGraph = enable != 0;
+	if(!s->useGraph)
+		s->dropGraph();
+	return RDU_OK;
+}
+
+int rdu_scene_autotune(rdu_scene * s, const rdu_camera_desc * cam, size_t samples, size_t depth, uint32_t width,
+	uint32_t height, const rdu_render_opts * opts, rdu_tune_report * report) {
+	TuneResult res;
+	const int st = renderImpl(s, cam, samples, depth, nullptr, width, height, 3, opts, nullptr, nullptr, &res);
+	if(st)
+		return st;
+	if(report) {
+		*report = rdu_tune_report{};
+		report->refill_min = res.best.refillMin;
+		report->chunk = res.best.chunk;
+		report->blocks_per_cu = res.best.blocksPerCu;
+		report->best_ms = res.bestMs;
+		report->default_ms = res.defaultMs;
+		report->candidates = res.timed;
+		report->source = 1;
+	}
+	return RDU_OK;
 }
 
 int rdu_pathtracer_render_cached(rdu_scene * s, rdu_rcache * cache, const rdu_camera_desc * cam, size_t samples,

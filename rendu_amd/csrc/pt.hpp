@@ -5,18 +5,30 @@
 
 namespace rdu {
 
-// Per-render constants: image size, random stream key, stratification grid and camera frame
-// (Camera::pixelShifts, src/engine/input/Camera.cpp:42-49).
+// Per-render constants: image size, stratification grid and camera frame
+// (Camera::pixelShifts, src/engine/input/Camera.cpp:42-49). The random stream key is a device
+// counter (kCountSeed).
 struct PtFrame {
 	uint32_t width = 0, height = 0;
-	uint32_t seed = 0;
 	int32_t cellX = 1;
 	float cellW = 1.0f, cellH = 1.0f;
 	f3 eye{0, 0, 0}, corner{0, 0, 0}, dx{0, 0, 0}, dy{0, 0, 0};
 };
 
-// Device counters (uint32 each).
-enum : int { kCountActive = 0, kCountNext = 1, kCountShadow = 2, kCountShadowNext = 3, kCountSlots = 4 };
+// Device counters (uint32 each). kCountBase is the first sample index of the running pass and
+// kCountBaseNext that of the next one: each pass reads its base from the device and advances it,
+// so a captured pass replays as the next pass with no host-side arguments changing. kCountSeed is
+// the render's random-stream key, read by the kernels from here for the same reason.
+enum : int {
+	kCountActive = 0,
+	kCountNext = 1,
+	kCountShadow = 2,
+	kCountShadowNext = 3,
+	kCountBase = 4,
+	kCountBaseNext = 5,
+	kCountSeed = 6,
+	kCountSlots = 7
+};
 
 // Path state, structure of arrays indexed by path id p = sample_in_pass * pixels + pixel.
 struct PtPaths {
@@ -83,9 +95,14 @@ struct PtCache {
 	float4 * cells = nullptr;          // kRcCells x {sum prediction, sum target, count, unused}: fine, then coarse
 };
 
-int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, const PtPaths & P, uint32_t sampleBase,
-	uint32_t samplesInPass, const PtOptions & opt, float * dImage, uint32_t components, PtScratch & scratch,
-	const PtCache * cache, hipStream_t stream);
+// One pass of samplesInPass samples per pixel, starting at the sample index the device holds in
+// kCountBaseNext, with the random key in kCountSeed (both set with ptSetSampleBase before the
+// first pass). Traversal launches use `tune`.
+// Enqueues only kernels (no copies, fills or host synchronisation), so it can be captured in a graph.
+int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, const PtPaths & P, uint32_t samplesInPass,
+	const PtOptions & opt, float * dImage, uint32_t components, PtScratch & scratch, const PtCache * cache,
+	const TraceTuning & tune, hipStream_t stream);
+int ptSetSampleBase(uint32_t * counts, uint32_t base, uint32_t seed, hipStream_t stream);
 int ptNormalizeImage(float * dImage, uint32_t pixels, uint32_t components, uint32_t spp, hipStream_t stream);
 
 } // namespace rdu

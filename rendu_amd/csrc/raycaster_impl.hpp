@@ -4,6 +4,7 @@
 #include "capi_common.hpp"
 #include "gpubuild.hpp"
 #include "trace.hpp"
+#include "tuning.hpp"
 #include "../../include/rdu.h"
 
 #include <mutex>
@@ -23,8 +24,10 @@ struct rdu_raycaster {
 	mutable std::mutex lock;
 	mutable rdu::DevBuffer stageIn, stageOut, gstack, stats;
 	mutable rdu::DevBuffer keys, keysAlt, perm, permAlt, sortTemp; // incoherent-batch ordering
-	// Autotuned refill threshold per [incoherent][any-hit] class (0 = not tuned).
-	uint32_t tunedRefill[2][2] = {{0, 0}, {0, 0}};
+	// Autotuned launch parameters per [incoherent][any-hit] class; classes this raycaster has not
+	// tuned take the process-wide tuning cache's entry for the scene's statistics, if any.
+	rdu::TraceTuning tuned[2][2];
+	bool tunedSet[2][2] = {{false, false}, {false, false}};
 	// Optional timing of the traversal kernel alone: event pairs recorded on the query's stream
 	// around each traversal launch, read back and summed by rdu_raycaster_kernel_time.
 	static constexpr size_t kMaxTimed = 4096;

@@ -382,7 +382,7 @@ uint32_t refillMin(const TraceOptions & opt) {
 }
 
 template<bool ANY, bool SPILL, bool STATS>
-PersistentGrid persistentGrid(size_t n);
+PersistentGrid persistentGrid(size_t n, uint32_t blocksPerCuCap);
 
 template<bool ANY>
 int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits, uint8_t * occ, const TraceOptions & opt,
@@ -412,12 +412,15 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 			return int(hipErrorInvalidValue);
 #define RDU_PERSIST(S, ST)                                                                                       \
 	{                                                                                                            \
-		const PersistentGrid g = persistentGrid<ANY, S, ST>(n);                                                  \
+		const PersistentGrid g = persistentGrid<ANY, S, ST>(n, opt.blocksPerCu);                                 \
 		const uint32_t waves = (g.blocks / kXcds) * (kBlock / 64);                                               \
 		const uint64_t slice = (uint64_t(n) + kXcds - 1) / kXcds;                                                \
-		/* sorted batches: interleaved 64-ray chunks; input order: one contiguous run per wavefront */          \
+		/* sorted batches: interleaved 64-ray chunks; input order: one contiguous run per wavefront; */        \
+		/* a tuned chunk overrides either */                                                                   \
 		uint64_t chunk = opt.spatialOrder ? 64 : (slice + waves - 1) / (waves ? waves : 1);                     \
 		chunk = chunk < 64 ? 64 : chunk;                                                                        \
+		if(opt.chunk >= 16 && opt.chunk <= (1u << 20))                                                          \
+			chunk = opt.chunk;                                                                                  \
 		hipLaunchKernelGGL((tracePersistent<ANY, S, ST>), dim3(g.blocks), dim3(kBlock), 0, stream, bvh, rays,      \
 			uint32_t(n), perm, opt.countDev, hits, occ, gstack, lds, kXcds, refillMin(opt), uint32_t(chunk), stats); \
 	}
@@ -690,7 +693,7 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 }
 
 template<bool ANY, bool SPILL, bool STATS>
-PersistentGrid persistentGrid(size_t n) {
+PersistentGrid persistentGrid(size_t n, uint32_t blocksPerCuCap) {
 	struct Occupancy {
 		int blocksPerCu = 1, cus = 1;
 	};
@@ -705,10 +708,11 @@ PersistentGrid persistentGrid(size_t n) {
 		o.cus = o.cus < 1 ? 1 : o.cus;
 		return o;
 	}();
-	const int blocksPerCu = occ.blocksPerCu, cus = occ.cus;
+	const int blocksPerCu = (blocksPerCuCap > 0 && int(blocksPerCuCap) < occ.blocksPerCu) ? int(blocksPerCuCap) : occ.blocksPerCu;
+	const int cus = occ.cus;
 	PersistentGrid g;
-	// One resident grid, trimmed for small batches (about one 64-ray batch per wavefront),
-	// always a whole number of blocks per die.
+	// One resident grid (optionally fewer workgroups per CU than fit), trimmed for small batches
+	// (about one 64-ray batch per wavefront), always a whole number of blocks per die.
 	size_t blocks = size_t(blocksPerCu) * size_t(cus);
 	const size_t needed = (n + kBlock - 1) / kBlock;
 	blocks = needed < blocks ? needed : blocks;
@@ -754,6 +758,13 @@ int ldsStackDepth() {
 	return (k >= 1 && k <= kLdsStack) ? k : kLdsStack;
 }
 
+int persistentOccupancy(bool any) {
+	int b = 0;
+	const hipError_t e = any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, tracePersistent<true, false, false>, kBlock, 0)
+							 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, tracePersistent<false, false, false>, kBlock, 0);
+	return (e != hipSuccess || b < 1) ? 1 : b;
+}
+
 int stackNeed(const DevBVH & bvh, const TraceOptions & opt) {
 	const bool wide = opt.wide && !opt.referenceOrder && bvh.wide != nullptr;
 	return wide ? 3 * int(bvh.wideDepth) + 2 : int(bvh.maxDepth) + 2;
@@ -765,7 +776,10 @@ size_t globalStackInts(const DevBVH & bvh, const TraceOptions & opt, size_t n) {
 	const int need = stackNeed(bvh, opt) - lds;
 	if(need <= 0)
 		return 0;
-	const size_t slots = ((n + kBlock - 1) / kBlock) * kBlock;
+	// One slot per lane of the widest grid a launch of n rays uses: the persistent grid rounds
+	// its block count up to a whole number per XCD, so it can exceed ceil(n / kBlock).
+	const size_t blocks = (n + kBlock - 1) / kBlock;
+	const size_t slots = ((blocks + kXcds - 1) / kXcds) * kXcds * kBlock;
 	return size_t(need) * slots;
 }
 

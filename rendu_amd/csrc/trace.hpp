@@ -78,7 +78,24 @@ struct TraceOptions {
 	const uint32_t * countDev = nullptr; // optional device count: only the first *countDev entries
 	bool spatialOrder = false; // perm is a spatial (Morton) order: interleave wavefronts finely
 	uint32_t refillMin = 0;    // idle lanes before a wavefront refills (0 = default for the order)
+	uint32_t chunk = 0;        // rays per interleaved run of a persistent wavefront (0 = default for the order)
+	uint32_t blocksPerCu = 0;  // resident workgroups per CU of the persistent grid (0 = occupancy limit)
 };
+
+// Launch parameters of the persistent traversal picked by the autotuner (0 = default).
+struct TraceTuning {
+	uint32_t refillMin = 0, chunk = 0, blocksPerCu = 0;
+	bool operator==(const TraceTuning & o) const {
+		return refillMin == o.refillMin && chunk == o.chunk && blocksPerCu == o.blocksPerCu;
+	}
+};
+inline void applyTuning(TraceOptions & o, const TraceTuning & t) {
+	o.refillMin = t.refillMin;
+	o.chunk = t.chunk;
+	o.blocksPerCu = t.blocksPerCu;
+}
+// Workgroups per CU the persistent closest-hit (any = false) or any-hit kernel can keep resident.
+int persistentOccupancy(bool any);
 
 // Launchers. rays: device rdu_ray array (float4 pairs); outputs are device pointers, indexed by
 // the original ray index. Return hipError_t as int.

@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 
 from . import scene as S
-from ._lib import check, fptr, lib
+from ._lib import TuneReport, check, fptr, lib
 
 
 class _BorrowedRaycaster:
@@ -80,6 +80,21 @@ class PathTracer:
         check(lib.rdu_pathtracer_render(self._h, ctypes.byref(cam), samples, depth, fptr(image), w, h, c,
                                         ctypes.byref(opts)))
         return self.last_stats()
+
+    def set_graph(self, enable: bool) -> None:
+        """Replay repeated full passes from a captured HIP graph (rdu_scene_set_graph)."""
+        check(lib.rdu_scene_set_graph(self._h, int(enable)))
+
+    def autotune(self, camera, samples: int, depth: int, width: int, height: int, seed: int = 0,
+                 estimator: int = S.ESTIMATOR_REFERENCE, rr_start: int = 3, max_paths: int = 0) -> dict:
+        """Time one pass of this render configuration per candidate of the traversal launch
+        parameters and keep the fastest for this scene's renders (rdu_scene_autotune)."""
+        rep = TuneReport()
+        opts = S.RenderOpts(seed, estimator, rr_start, max_paths)
+        cam = S.camera_desc(camera)
+        check(lib.rdu_scene_autotune(self._h, ctypes.byref(cam), samples, depth, width, height, ctypes.byref(opts),
+                                     ctypes.byref(rep)))
+        return {k: getattr(rep, k) for k, _ in TuneReport._fields_}
 
     def render_cached(self, cache, camera, samples: int, depth: int, image: np.ndarray, seed: int = 0,
                       estimator: int = S.ESTIMATOR_EXTENDED, rr_start: int = 3, max_paths: int = 0,

@@ -158,7 +158,9 @@ class Raycaster:
 
     def autotune(self, rays=None, n: int = 0, rays_ptr: int = 0, any_hit: bool = False, incoherent: bool = False,
                  stream: int = 0) -> dict:
-        """Pick the refill threshold for this class of batch by timing candidates (rdu_raycaster_autotune).
+        """Pick the persistent traversal's launch parameters (refill threshold, ray tile, workgroups
+        per CU) for this class of batch by timing candidates (rdu_raycaster_autotune); the winner
+        also enters the process-wide tuning cache under the scene's statistics.
         Pass host `rays` (rdu_ray records) or a device pointer `rays_ptr` with its count `n`."""
         rep = _lib.TuneReport()
         flags = _flags(False, incoherent)
@@ -171,6 +173,13 @@ class Raycaster:
         check(lib.rdu_raycaster_autotune(self._h, ptr, count, int(any_hit), flags,
                                          ctypes.c_void_p(stream) if stream else None, ctypes.byref(rep)))
         return {k: getattr(rep, k) for k, _ in _lib.TuneReport._fields_}
+
+    def tuning(self, n: int, any_hit: bool = False, incoherent: bool = False) -> dict:
+        """Launch parameters a batch of n rays of this class would use now, and their source
+        (0 defaults, 1 this raycaster's autotune, 2 the tuning cache)."""
+        rep = _lib.TuneReport()
+        check(lib.rdu_raycaster_tuning(self._h, n, int(any_hit), _flags(False, incoherent), ctypes.byref(rep)))
+        return {k: getattr(rep, k) for k in ("refill_min", "chunk", "blocks_per_cu", "source")}
 
     def set_kernel_timing(self, enable: bool) -> None:
         """Record HIP events around each traversal kernel launch (rdu_raycaster_set_kernel_timing)."""

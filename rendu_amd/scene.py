@@ -63,7 +63,8 @@ class RenderOpts(ctypes.Structure):
 
 class RenderStats(ctypes.Structure):
     _fields_ = [("seconds", ctypes.c_double), ("paths", ctypes.c_uint64), ("extension_rays", ctypes.c_uint64),
-                ("shadow_rays", ctypes.c_uint64), ("samples", c_uint32)]
+                ("shadow_rays", ctypes.c_uint64), ("samples", c_uint32), ("passes", c_uint32),
+                ("graph_passes", c_uint32), ("graph_captures", c_uint32)]
 
 
 class CacheOpts(ctypes.Structure):

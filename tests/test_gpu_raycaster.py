@@ -171,6 +171,19 @@ def test_spill_path_bit_exact(mixed, mixed_rays, monkeypatch):
     compare_near_first(ours.intersect_batch(mixed_rays), ref.intersect(mixed_rays, "closest"))
 
 
+@pytest.mark.parametrize("n", [1, 300, 1100])
+def test_spill_small_batch_fresh_raycaster(mixed_rays, monkeypatch, n):
+    """Small near-first batches on a raycaster that has not traced anything yet: the persistent grid
+    rounds its blocks up to a whole number per XCD (more lanes than ceil(n / 128) blocks), and the
+    spill stack is sized for that grid."""
+    ours, ref = build(mixed_scene())
+    monkeypatch.setenv("RDU_DEBUG_LDS_STACK", "1")
+    log_case(f"fresh raycaster, {n} rays, 1 LDS stack level")
+    rays = mixed_rays[:n]
+    compare_near_first(ours.intersect_batch(rays), ref.intersect(rays, "closest"))
+    assert np.array_equal(ours.intersect_any_batch(rays), ref.intersect(rays, "any")["hit"].astype(bool))
+
+
 def test_many_roots_and_tiny_meshes():
     meshes = [(workloads.triangle_soup(n, seed=100 + k, log_edge=-2.0), None)
               for k, n in enumerate([1, 2, 3, 4, 5, 7, 16, 33] * 25)]
