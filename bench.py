@@ -5,7 +5,8 @@ exp(-5 + 0.5 z), random rotation) with 2^23 incoherent rays (origins U[0,1]^3, d
 the sphere) and 2^23 coherent pinhole primaries (4096 x 2048), Rendu's default interval (1e-4, 1e8).
 One step = one closest-hit pass (rdu_raycaster_intersect, near-first order, RDU_INCOHERENT hint so
 the library orders the rays itself inside the timed region) over the 2^23 incoherent rays, rays
-and hits resident in HBM. The refill threshold is autotuned on the same batch during warm-up.
+and hits resident in HBM. The traversal's launch parameters (refill threshold, ray tile, workgroups
+per CU) are autotuned on the same batch during warm-up.
 
 Alongside: coherent closest hit, any hit, Rendu's exact visiting order, and the small path-tracing
 workload (Cornell box + 3 icospheres, 256x256, 16 spp, depth 5) in samples/s. The CPU baseline is
@@ -40,7 +41,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--main-only", action="store_true", help="only the timed incoherent closest-hit passes")
     ap.add_argument("--refill", type=int, default=0,
-                    help="fixed refill threshold instead of autotuning (profiling runs: one kernel configuration)")
+                    help="fixed launch parameters instead of autotuning (profiling runs: one kernel "
+                         "configuration): refill threshold, with --chunk and --blocks-per-cu")
+    ap.add_argument("--chunk", type=int, default=0, help="fixed ray tile with --refill (0 = default)")
+    ap.add_argument("--blocks-per-cu", type=int, default=0, help="fixed workgroups per CU with --refill (0 = all)")
     ap.add_argument("--tree", choices=("rendu", "gpu"), default="rendu",
                     help="tree of the timed queries: Rendu's host build (updateHierarchy) or the GPU linear BVH")
     ap.add_argument("--no-cache-experiment", action="store_true", help="skip the radiance-cache comparison")
@@ -90,7 +94,8 @@ def cache_experiment(pt, scn, cam, ref_spp):
     """Radiance cache on the single frame (1920 x 1080, depth 8, RR from bounce 3): RMSE against a
     ref_spp plain render, plain at 64 spp against cached renders (fresh cache each, online training
     included in the wall time) at 16 / 32 / 64 spp, with the trust test and with every vertex from
-    the termination bounce ending in the cache; also a cache warmed by one earlier 64-spp frame.
+    the termination bounce (2, 1, or the primary vertex) ending in the cache; also a cache warmed
+    by one earlier 64-spp frame.
     Reports the wall-time ratio of the fastest cached render whose RMSE reaches plain 64 spp."""
     from rendu_amd import scene as S
     from rendu_amd.rcache import RadianceCache
@@ -127,11 +132,12 @@ def cache_experiment(pt, scn, cam, ref_spp):
     runs["cached"] = [cached(s) for s in (16, 32, 64)]
     runs["cached_always"] = [cached(s, always_trust=True) for s in (16, 32, 64)]
     runs["cached_always_b1"] = [cached(s, always_trust=True, terminate_bounce=1) for s in (16, 32, 64)]
+    runs["cached_always_primary"] = [cached(s, always_trust=True, from_primary=True) for s in (16, 32)]
     warm = RadianceCache(lo, hi, seed=5)
     pt.render_cached(warm, cam, 64, 8, np.zeros((h, w, 3), np.float32), seed=3, estimator=ext, rr_start=3)
     runs["warm_cache"] = [cached(s, cache=warm) for s in (16, 32)]
     best = None
-    for key in ("cached", "cached_always", "cached_always_b1", "warm_cache"):
+    for key in ("cached", "cached_always", "cached_always_b1", "cached_always_primary", "warm_cache"):
         for r in runs[key]:
             if r["rmse"] <= p64["rmse"] and (best is None or r["s"] < best[1]["s"]):
                 best = (key, r)
@@ -225,8 +231,8 @@ def main():
         return wall, float(np.mean([a.elapsed_time(b) for a, b in ev]))
 
     if args.refill:
-        os.environ["RDU_REFILL_MIN"] = str(args.refill)
-        tune = {"refill_min": args.refill, "fixed": True}
+        rc.set_tuning(args.refill, args.chunk, args.blocks_per_cu, incoherent=True)
+        tune = {"refill_min": args.refill, "chunk": args.chunk, "blocks_per_cu": args.blocks_per_cu, "fixed": True}
     else:
         tune = rc.autotune(rays_ptr=d_inc.data_ptr(), n=N_HALF, incoherent=True, stream=sh)
     # The traversal kernel alone is timed with HIP events the library records on the launch stream.

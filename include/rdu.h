@@ -298,6 +298,9 @@ int rdu_raycaster_autotune(rdu_raycaster* rc, const rdu_ray* rays, size_t n, int
 /* The launch parameters a batch of n rays of this class would use now, and where they come from
  * (refill_min, chunk, blocks_per_cu, source; the times are 0). */
 int rdu_raycaster_tuning(const rdu_raycaster* rc, size_t n, int any_hit, uint32_t flags, rdu_tune_report* out);
+/* Sets this raycaster's launch parameters for a class without timing anything (refill_min 0..64,
+ * chunk 0 or 16..2^20, blocks_per_cu 0..64 from `params`; 0 = default; other fields ignored). */
+int rdu_raycaster_set_tuning(rdu_raycaster* rc, int any_hit, uint32_t flags, const rdu_tune_report* params);
 /* The process-wide tuning cache: save to / merge from a text file (a header line, then one entry
  * per line), clear, count entries. A malformed file is rejected whole with RDU_ERR_INVALID. */
 int rdu_tune_cache_save(const char* path);
@@ -388,6 +391,7 @@ typedef struct {
 	float min_count;
 	uint32_t frozen;
 	uint32_t always_trust;
+	uint32_t from_primary; /* 1: terminate from bounce 0 (the primary vertex); overrides terminate_bounce */
 } rdu_cache_opts;
 typedef struct {
 	uint64_t cache_queries; /* paths ended in the cache */

@@ -139,6 +139,8 @@ public:
 			_log);
 	}
 	const rdu_raycaster * raycaster() const { return rdu_scene_raycaster(_scene); }
+	// Extension (no Rendu counterpart): replay repeated full passes from a captured HIP graph.
+	void setGraph(bool enable) { report(rdu_scene_set_graph(_scene, enable ? 1 : 0), _log); }
 
 private:
 	rdu_scene * _scene;

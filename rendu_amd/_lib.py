@@ -80,6 +80,7 @@ def _load() -> ctypes.CDLL:
                                               POINTER(c_uint64), POINTER(c_uint64)]),
         "rdu_raycaster_autotune": (c_int, [c_void_p, c_void_p, c_size_t, c_int, c_uint32, c_void_p, c_void_p]),
         "rdu_raycaster_tuning": (c_int, [c_void_p, c_size_t, c_int, c_uint32, c_void_p]),
+        "rdu_raycaster_set_tuning": (c_int, [c_void_p, c_int, c_uint32, c_void_p]),
         "rdu_tune_cache_save": (c_int, [c_char_p]),
         "rdu_tune_cache_load": (c_int, [c_char_p]),
         "rdu_tune_cache_clear": (c_int, []),

@@ -487,6 +487,22 @@ int rdu_raycaster_tuning(const rdu_raycaster * rc, size_t n, int any_hit, uint32
 	return RDU_OK;
 }
 
+int rdu_raycaster_set_tuning(rdu_raycaster * rc, int any_hit, uint32_t flags, const rdu_tune_report * params) {
+	clearError();
+	if(!rc || !params)
+		return fail(RDU_ERR_INVALID, "null raycaster or parameters");
+	TraceTuning t;
+	t.refillMin = params->refill_min;
+	t.chunk = params->chunk;
+	t.blocksPerCu = params->blocks_per_cu;
+	if(!tuningValid(t))
+		return fail(RDU_ERR_INVALID, "launch parameters out of range");
+	const int c = (flags & RDU_INCOHERENT) ? 1 : 0, a = any_hit ? 1 : 0;
+	rc->tuned[c][a] = t;
+	rc->tunedSet[c][a] = true;
+	return RDU_OK;
+}
+
 int rdu_tune_cache_save(const char * path) {
 	clearError();
 	return tuneCacheSave(path);

@@ -497,7 +497,7 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 	if(cache) {
 		if(rcacheDevice(cache) != s->rc->device)
 			return fail(RDU_ERR_INVALID, "cache and scene live on different devices");
-		C.terminateBounce = int32_t(co.terminate_bounce ? co.terminate_bounce : 2u);
+		C.terminateBounce = co.from_primary ? 0 : int32_t(co.terminate_bounce ? co.terminate_bounce : 2u);
 		C.maxBias = co.max_bias > 0.0f ? co.max_bias : 0.2f;
 		C.minCount = co.min_count > 0.0f ? co.min_count : 8.0f;
 		C.alwaysTrust = co.always_trust ? 1u : 0u;

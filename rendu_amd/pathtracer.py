@@ -99,7 +99,8 @@ class PathTracer:
     def render_cached(self, cache, camera, samples: int, depth: int, image: np.ndarray, seed: int = 0,
                       estimator: int = S.ESTIMATOR_EXTENDED, rr_start: int = 3, max_paths: int = 0,
                       terminate_bounce: int = 0, train_steps: int = 0, train_batch: int = 0, max_bias: float = 0.0,
-                      min_count: float = 0.0, frozen: bool = False, always_trust: bool = False) -> dict:
+                      min_count: float = 0.0, frozen: bool = False, always_trust: bool = False,
+                      from_primary: bool = False) -> dict:
         """render() with a rendu_amd.rcache.RadianceCache (rdu_pathtracer_render_cached): paths end in
         the cache where it is trusted, one pixel in 16 trains it. Zero options take the defaults."""
         if image.dtype != np.float32 or not image.flags.c_contiguous or image.ndim != 3:
@@ -107,7 +108,7 @@ class PathTracer:
         h, w, c = image.shape
         opts = S.RenderOpts(seed, estimator, rr_start, max_paths)
         copts = S.CacheOpts(terminate_bounce, train_steps, train_batch, max_bias, min_count, int(frozen),
-                            int(always_trust))
+                            int(always_trust), int(from_primary))
         cam = S.camera_desc(camera)
         check(lib.rdu_pathtracer_render_cached(self._h, cache.handle, ctypes.byref(cam), samples, depth, fptr(image),
                                                w, h, c, ctypes.byref(opts), ctypes.byref(copts)))

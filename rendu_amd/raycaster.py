@@ -181,6 +181,12 @@ class Raycaster:
         check(lib.rdu_raycaster_tuning(self._h, n, int(any_hit), _flags(False, incoherent), ctypes.byref(rep)))
         return {k: getattr(rep, k) for k in ("refill_min", "chunk", "blocks_per_cu", "source")}
 
+    def set_tuning(self, refill_min: int = 0, chunk: int = 0, blocks_per_cu: int = 0, any_hit: bool = False,
+                   incoherent: bool = False) -> None:
+        """Fix this class's launch parameters without timing (0 = default; rdu_raycaster_set_tuning)."""
+        rep = _lib.TuneReport(refill_min, 0.0, 0.0, 0, chunk, blocks_per_cu, 0)
+        check(lib.rdu_raycaster_set_tuning(self._h, int(any_hit), _flags(False, incoherent), ctypes.byref(rep)))
+
     def set_kernel_timing(self, enable: bool) -> None:
         """Record HIP events around each traversal kernel launch (rdu_raycaster_set_kernel_timing)."""
         check(lib.rdu_raycaster_set_kernel_timing(self._h, int(enable)))

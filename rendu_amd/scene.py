@@ -70,7 +70,8 @@ class RenderStats(ctypes.Structure):
 class CacheOpts(ctypes.Structure):
     """rdu_cache_opts: zero fields take the library defaults."""
     _fields_ = [("terminate_bounce", c_uint32), ("train_steps", c_uint32), ("train_batch", c_uint32),
-                ("max_bias", c_float), ("min_count", c_float), ("frozen", c_uint32), ("always_trust", c_uint32)]
+                ("max_bias", c_float), ("min_count", c_float), ("frozen", c_uint32), ("always_trust", c_uint32),
+                ("from_primary", c_uint32)]
 
 
 class CacheStats(ctypes.Structure):

@@ -56,16 +56,21 @@ def test_fresh_cache_single_pass_is_plain():
     assert st["cache_queries"] == 0 and st["records"] > 0 and st["train_steps"] == 4
 
 
-def test_zero_cache_always_trusted_truncates_after_bounce_2():
+@pytest.mark.parametrize("term", [dict(), dict(terminate_bounce=1), dict(from_primary=True)])
+def test_zero_cache_always_trusted_truncates_after_the_termination_bounce(term):
+    """A zero cache trusted everywhere ends paths at their first non-emissive vertex from the
+    termination bounce t on (bounce 2 by default): the plain render of depth t + 1 (the scene has no
+    analytic lights, so the last traced vertex adds nothing in either)."""
     scn, cam_of, lo, hi = _cornell()
-    log_case("cached render, zero frozen cache == plain depth 3, 4 passes")
+    t = 0 if term.get("from_primary") else term.get("terminate_bounce", 2)
+    log_case(f"cached render, zero frozen cache == plain depth {t + 1}, 4 passes")
     pt = PathTracer(scn)
     w, h = 40, 32
     plain = np.zeros((h, w, 3), F32)
-    pt.render(cam_of(w, h), 16, 3, plain, seed=7, estimator=EXT, rr_start=3, max_paths=w * h * 4)
+    pt.render(cam_of(w, h), 16, t + 1, plain, seed=7, estimator=EXT, rr_start=3, max_paths=w * h * 4)
     got = np.zeros_like(plain)
     st = pt.render_cached(_constant_cache(lo, hi, 0.0), cam_of(w, h), 16, 8, got, seed=7, estimator=EXT, rr_start=3,
-                          max_paths=w * h * 4, frozen=True, always_trust=True)
+                          max_paths=w * h * 4, frozen=True, always_trust=True, **term)
     np.testing.assert_array_equal(got, plain)
     assert st["cache_queries"] > 0 and st["records"] == 0 and st["train_steps"] == 0
 
