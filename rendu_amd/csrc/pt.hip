@@ -502,6 +502,7 @@ int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, cons
 		TraceOptions to;
 		applyTuning(to, tn);
 		to.wide = true;
+		to.nearFar = b > 0; // camera rays are coherent, the later bounces are not
 		to.perm = P.queue[q];
 		to.countDev = P.counts + kCountActive;
 		int e = launchClosest(bvh, P.rays, nPaths, P.hits, to, scratch.gstack, nullptr, stream);

@@ -70,6 +70,25 @@ __device__ __forceinline__ bool slab6(const DevRay & r, float lx, float ly, floa
 	return entry <= vmin<EXACT>(furthest, maxi);
 }
 
+// Fast-path slab test on planes already ordered per axis by the sign of the reciprocal
+// direction (n = near plane, f = far plane): for finite operands (a - p) * inv is monotonic in a,
+// so the near plane's product is min(a, b) and the far plane's is max(a, b) of the reference.
+__device__ __forceinline__ bool slabNearFar(const DevRay & r, float nx, float ny, float nz, float fx, float fy, float fz,
+	float mini, float maxi, float & entry) {
+	const float ax = (nx - r.pos.x) * r.inv.x, bx = (fx - r.pos.x) * r.inv.x;
+	const float ay = (ny - r.pos.y) * r.inv.y, by = (fy - r.pos.y) * r.inv.y;
+	const float az = (nz - r.pos.z) * r.inv.z, bz = (fz - r.pos.z) * r.inv.z;
+	entry = __builtin_fmaxf(__builtin_fmaxf(ax, __builtin_fmaxf(ay, az)), mini);
+	return entry <= __builtin_fminf(__builtin_fminf(bx, __builtin_fminf(by, bz)), maxi);
+}
+
+// 16-byte load from a global-memory address held as an integer (keeps it a global, not a flat, load).
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ldGlobal(uintptr_t a) {
+	const f4v v = *reinterpret_cast<const __attribute__((address_space(1))) f4v *>(a);
+	return make_float4(v.x, v.y, v.z, v.w);
+}
+
 struct LaneStack {
 	int (*ls)[kBlock];
 	int * gs;
@@ -184,18 +203,11 @@ __device__ __forceinline__ void traverseBinary(const DevBVH & bvh, const DevRay 
 	}
 }
 
-// fp16 value not above e (round towards -inf; beyond the fp16 range: the largest finite half).
+// fp16 value not above e: round towards zero (v_cvt_pkrtz; beyond the fp16 range it gives the
+// largest finite half), which is towards -inf for e >= 0; negative entries (a negative tmin) are
+// stored as -inf, which never culls.
 __device__ __forceinline__ _Float16 roundDownHalf(float e) {
-	const float big = 65504.0f;
-	const float ec = e > big ? big : e;
-	_Float16 h = _Float16(ec);
-	if(float(h) > ec) {
-		// one fp16 ulp down (ec >= 0 here in practice; the bit step works for both signs)
-		unsigned short b = __builtin_bit_cast(unsigned short, h);
-		b = (b & 0x8000u) ? (unsigned short)(b + 1u) : (b == 0 ? (unsigned short)0x8001u : (unsigned short)(b - 1u));
-		h = __builtin_bit_cast(_Float16, b);
-	}
-	return h;
+	return e < 0.0f ? _Float16(-__builtin_inff()) : __builtin_amdgcn_cvt_pkrtz(e, e)[0];
 }
 
 __device__ __forceinline__ void cswap(float & a, int & ca, float & b, int & cb) {
@@ -353,7 +365,7 @@ __global__ __launch_bounds__(kBlock) void traceKernel(DevBVH bvh, const float4 *
 	}
 }
 
-template<bool ANY, bool SPILL, bool STATS>
+template<bool ANY, bool SPILL, bool STATS, bool NF>
 __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
 	const uint32_t * __restrict__ perm, const uint32_t * __restrict__ countDev, rdu_hit * __restrict__ hits,
 	uint8_t * __restrict__ occ, int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint32_t chunk,
@@ -381,7 +393,7 @@ uint32_t refillMin(const TraceOptions & opt) {
 	return opt.spatialOrder ? 32u : 16u;
 }
 
-template<bool ANY, bool SPILL, bool STATS>
+template<bool ANY, bool SPILL, bool STATS, bool NF>
 PersistentGrid persistentGrid(size_t n, uint32_t blocksPerCuCap);
 
 template<bool ANY>
@@ -410,9 +422,9 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 	if(persistent) {
 		if(n > 0xffffffffull)
 			return int(hipErrorInvalidValue);
-#define RDU_PERSIST(S, ST)                                                                                       \
+#define RDU_PERSIST(S, ST, NF)                                                                                   \
 	{                                                                                                            \
-		const PersistentGrid g = persistentGrid<ANY, S, ST>(n, opt.blocksPerCu);                                 \
+		const PersistentGrid g = persistentGrid<ANY, S, ST, NF>(n, opt.blocksPerCu);                             \
 		const uint32_t waves = (g.blocks / kXcds) * (kBlock / 64);                                               \
 		const uint64_t slice = (uint64_t(n) + kXcds - 1) / kXcds;                                                \
 		/* sorted batches: interleaved 64-ray chunks; input order: one contiguous run per wavefront; */        \
@@ -421,13 +433,26 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 		chunk = chunk < 64 ? 64 : chunk;                                                                        \
 		if(opt.chunk >= 16 && opt.chunk <= (1u << 20))                                                          \
 			chunk = opt.chunk;                                                                                  \
-		hipLaunchKernelGGL((tracePersistent<ANY, S, ST>), dim3(g.blocks), dim3(kBlock), 0, stream, bvh, rays,      \
+		hipLaunchKernelGGL((tracePersistent<ANY, S, ST, NF>), dim3(g.blocks), dim3(kBlock), 0, stream, bvh, rays,  \
 			uint32_t(n), perm, opt.countDev, hits, occ, gstack, lds, kXcds, refillMin(opt), uint32_t(chunk), stats); \
 	}
-		if(stats) {
-			if(spill) RDU_PERSIST(true, true) else RDU_PERSIST(false, true)
+		// Near/far plane fetch (closest hit only): fewer instructions per node, more registers (5
+		// waves per SIMD instead of 6); it pays on incoherent rays and costs on coherent ones.
+		bool nf = false;
+		if constexpr(!ANY)
+			nf = opt.nearFar;
+		if(nf) {
+			if(stats) {
+				if(spill) RDU_PERSIST(true, true, !ANY) else RDU_PERSIST(false, true, !ANY)
+			} else {
+				if(spill) RDU_PERSIST(true, false, !ANY) else RDU_PERSIST(false, false, !ANY)
+			}
 		} else {
-			if(spill) RDU_PERSIST(true, false) else RDU_PERSIST(false, false)
+			if(stats) {
+				if(spill) RDU_PERSIST(true, true, false) else RDU_PERSIST(false, true, false)
+			} else {
+				if(spill) RDU_PERSIST(true, false, false) else RDU_PERSIST(false, false, false)
+			}
 		}
 #undef RDU_PERSIST
 		return int(hipGetLastError());
@@ -477,7 +502,7 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 // so never above the true entry): a popped child whose entry already exceeds the current maxi
 // is dropped without being fetched, exactly as its own box test would drop it (the test fails
 // once entry > maxi). Levels spilled to global memory carry no entry and are always fetched.
-template<bool ANY, bool SPILL, bool STATS>
+template<bool ANY, bool SPILL, bool STATS, bool NF>
 __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
 	const uint32_t * __restrict__ perm, const uint32_t * __restrict__ countDev, rdu_hit * __restrict__ hits,
 	uint8_t * __restrict__ occ, int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint32_t chunk,
@@ -498,6 +523,7 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 
 	bool has = false, pend = false;
 	uint32_t ri = 0;
+	uint32_t ox = 0, oy = 32, oz = 64;
 	DevRay r = makeRay(mk(0, 0, 0), mk(0, 0, 1), 0.0f, 0.0f);
 	bool clean = true;
 	int c = 0, kk = 0;
@@ -547,6 +573,12 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 					const float4 rb = rays[2 * size_t(ri) + 1];
 					r = makeRay(xyz(ra), xyz(rb), ra.w, rb.w);
 					clean = finite3(r.inv) && finite3(r.pos);
+					// Byte offsets of each axis' near plane record within a node (x: 0/16, y: 32/48, z: 64/80).
+					if constexpr(NF) {
+						ox = r.inv.x < 0.0f ? 16u : 0u;
+						oy = r.inv.y < 0.0f ? 48u : 32u;
+						oz = r.inv.z < 0.0f ? 80u : 64u;
+					}
 					maxi = r.maxi;
 					best = -1;
 					bt = FLT_MAX;
@@ -579,12 +611,24 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 		if(has) {
 			if(c >= 0) {
 				const float4 * nd = bvh.wide + 8 * size_t(c);
-				a0 = nd[0];
-				a1 = nd[1];
-				a2 = nd[2];
-				a3 = nd[3];
-				a4 = nd[4];
-				a5 = nd[5];
+				// Near planes in a0, a2, a4 and far planes in a1, a3, a5 (nodes are 128-byte aligned,
+				// so the offsets combine with the address by OR and XOR).
+				if constexpr(NF) {
+					const uintptr_t b = reinterpret_cast<uintptr_t>(nd);
+					a0 = ldGlobal(b | ox);
+					a1 = ldGlobal(b | (ox ^ 16u));
+					a2 = ldGlobal(b | oy);
+					a3 = ldGlobal(b | (oy ^ 16u));
+					a4 = ldGlobal(b | oz);
+					a5 = ldGlobal(b | (oz ^ 16u));
+				} else {
+					a0 = nd[0];
+					a1 = nd[1];
+					a2 = nd[2];
+					a3 = nd[3];
+					a4 = nd[4];
+					a5 = nd[5];
+				}
 				a6 = nd[6];
 			} else {
 				const int ti = ((~c) >> 1) + kk;
@@ -605,24 +649,49 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 			float e0, e1, e2 = 0.0f, e3 = 0.0f;
 			bool h0, h1, h2, h3;
 			if(exact) {
+				// Back to (min, max) plane order for the reference's select semantics.
+				if(NF && ox) {
+					const float4 t = a0;
+					a0 = a1;
+					a1 = t;
+				}
+				if(NF && oy != 32u) {
+					const float4 t = a2;
+					a2 = a3;
+					a3 = t;
+				}
+				if(NF && oz != 64u) {
+					const float4 t = a4;
+					a4 = a5;
+					a5 = t;
+				}
 				h0 = slab6<true>(r, a0.x, a2.x, a4.x, a1.x, a3.x, a5.x, r.mini, maxi, e0);
 				h1 = slab6<true>(r, a0.y, a2.y, a4.y, a1.y, a3.y, a5.y, r.mini, maxi, e1);
 				h2 = (c2 != kWideEmpty) && slab6<true>(r, a0.z, a2.z, a4.z, a1.z, a3.z, a5.z, r.mini, maxi, e2);
 				h3 = (c3 != kWideEmpty) && slab6<true>(r, a0.w, a2.w, a4.w, a1.w, a3.w, a5.w, r.mini, maxi, e3);
 			} else {
-				h0 = slab6<false>(r, a0.x, a2.x, a4.x, a1.x, a3.x, a5.x, r.mini, maxi, e0);
-				h1 = slab6<false>(r, a0.y, a2.y, a4.y, a1.y, a3.y, a5.y, r.mini, maxi, e1);
-				h2 = (c2 != kWideEmpty) && slab6<false>(r, a0.z, a2.z, a4.z, a1.z, a3.z, a5.z, r.mini, maxi, e2);
-				h3 = (c3 != kWideEmpty) && slab6<false>(r, a0.w, a2.w, a4.w, a1.w, a3.w, a5.w, r.mini, maxi, e3);
+				if constexpr(NF) {
+					h0 = slabNearFar(r, a0.x, a2.x, a4.x, a1.x, a3.x, a5.x, r.mini, maxi, e0);
+					h1 = slabNearFar(r, a0.y, a2.y, a4.y, a1.y, a3.y, a5.y, r.mini, maxi, e1);
+					h2 = (c2 != kWideEmpty) && slabNearFar(r, a0.z, a2.z, a4.z, a1.z, a3.z, a5.z, r.mini, maxi, e2);
+					h3 = (c3 != kWideEmpty) && slabNearFar(r, a0.w, a2.w, a4.w, a1.w, a3.w, a5.w, r.mini, maxi, e3);
+				} else {
+					h0 = slab6<false>(r, a0.x, a2.x, a4.x, a1.x, a3.x, a5.x, r.mini, maxi, e0);
+					h1 = slab6<false>(r, a0.y, a2.y, a4.y, a1.y, a3.y, a5.y, r.mini, maxi, e1);
+					h2 = (c2 != kWideEmpty) && slab6<false>(r, a0.z, a2.z, a4.z, a1.z, a3.z, a5.z, r.mini, maxi, e2);
+					h3 = (c3 != kWideEmpty) && slab6<false>(r, a0.w, a2.w, a4.w, a1.w, a3.w, a5.w, r.mini, maxi, e3);
+				}
 			}
 			const int nh = int(h0) + int(h1) + int(h2) + int(h3);
 			if(nh == 0) {
 				next = true;
 			} else {
-				e0 = h0 ? __builtin_fminf(e0, FLT_MAX) : __builtin_inff();
-				e1 = h1 ? __builtin_fminf(e1, FLT_MAX) : __builtin_inff();
-				e2 = h2 ? __builtin_fminf(e2, FLT_MAX) : __builtin_inff();
-				e3 = h3 ? __builtin_fminf(e3, FLT_MAX) : __builtin_inff();
+				// A child that passed its test has a finite entry (mini <= entry <= maxi); the others
+				// sort last.
+				e0 = h0 ? e0 : __builtin_inff();
+				e1 = h1 ? e1 : __builtin_inff();
+				e2 = h2 ? e2 : __builtin_inff();
+				e3 = h3 ? e3 : __builtin_inff();
 				cswap(e0, c0, e1, c1);
 				cswap(e2, c2, e3, c3);
 				cswap(e0, c0, e2, c2);
@@ -692,7 +761,7 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 	}
 }
 
-template<bool ANY, bool SPILL, bool STATS>
+template<bool ANY, bool SPILL, bool STATS, bool NF>
 PersistentGrid persistentGrid(size_t n, uint32_t blocksPerCuCap) {
 	struct Occupancy {
 		int blocksPerCu = 1, cus = 1;
@@ -703,7 +772,7 @@ PersistentGrid persistentGrid(size_t n, uint32_t blocksPerCuCap) {
 		int dev = 0;
 		(void)hipGetDevice(&dev);
 		(void)hipDeviceGetAttribute(&o.cus, hipDeviceAttributeMultiprocessorCount, dev);
-		(void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocksPerCu, tracePersistent<ANY, SPILL, STATS>, kBlock, 0);
+		(void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocksPerCu, tracePersistent<ANY, SPILL, STATS, NF>, kBlock, 0);
 		o.blocksPerCu = o.blocksPerCu < 1 ? 1 : o.blocksPerCu;
 		o.cus = o.cus < 1 ? 1 : o.cus;
 		return o;
@@ -760,8 +829,8 @@ int ldsStackDepth() {
 
 int persistentOccupancy(bool any) {
 	int b = 0;
-	const hipError_t e = any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, tracePersistent<true, false, false>, kBlock, 0)
-							 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, tracePersistent<false, false, false>, kBlock, 0);
+	const hipError_t e = any ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, tracePersistent<true, false, false, false>, kBlock, 0)
+							 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, tracePersistent<false, false, false, false>, kBlock, 0);
 	return (e != hipSuccess || b < 1) ? 1 : b;
 }
 
