@@ -50,6 +50,19 @@ __device__ __forceinline__ float lane4(const s4 & v, int e) {
 	return bf2f(uint16_t(s));
 }
 
+// Stores one B-layout tile (features 16 t + 4 g + e of sample idx) into a feature-major bf16 buffer.
+__device__ __forceinline__ void storeTile(uint16_t * buf, int row0, int t, int g, uint32_t cap, uint32_t idx, s4 v) {
+	const size_t base = size_t(row0 + 16 * t + 4 * g) * cap + idx;
+	buf[base] = uint16_t(v.x);
+	buf[base + cap] = uint16_t(v.y);
+	buf[base + 2 * size_t(cap)] = uint16_t(v.z);
+	buf[base + 3 * size_t(cap)] = uint16_t(v.w);
+}
+// Bits e of a B-layout tile whose bf16 value is non-zero (post-ReLU activations are >= 0).
+__device__ __forceinline__ uint32_t nonzero4(s4 v) {
+	return (v.x != 0 ? 1u : 0u) | (v.y != 0 ? 2u : 0u) | (v.z != 0 ? 4u : 0u) | (v.w != 0 ? 8u : 0u);
+}
+
 __device__ __forceinline__ f4v4 mfma(s4 a, s4 b, f4v4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
 
 // A operand: W[mt*16 + m][kt*16 + 4g .. +3] (row-major [out][in], in = ld).
@@ -186,6 +199,9 @@ __device__ __forceinline__ void loadWeights(uint16_t * W, const uint16_t * w16) 
 // (scattered into the table gradients), activations and output gradients stored for the
 // weight-gradient GEMMs. Otherwise: forward only, radiance written to `out`.
 template<bool TRAIN>
+#ifdef RDU_EXP_RCW
+__attribute__((amdgpu_waves_per_eu(RDU_EXP_RCW, 8)))
+#endif
 __global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t * __restrict__ w16,
 	const uint16_t * __restrict__ t16, const float4 * __restrict__ queries, const float4 * __restrict__ targets,
 	const uint32_t * __restrict__ countDev, uint32_t n, float4 * __restrict__ out, RcTrainBuffers tb,
@@ -210,6 +226,11 @@ __global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t 
 		}
 		// Encoding: this lane computes levels 2g, 2g+1 (chunk 0), 8+2g, 9+2g (chunk 1) and the
 		// spherical harmonics 4g..4g+3 (chunk 2).
+		// Training stores each activation tile (for the weight gradients) as soon as it exists and
+		// keeps only a bit per element of whether it is non-zero (the ReLU derivative), so that
+		// activations do not stay live in registers through the backward pass.
+		const uint32_t cap = tb.capacity;
+		uint32_t active[6] = {0, 0, 0, 0, 0, 0}; // bit 4 mt + e: X[l][mt] element e > 0
 		s4 X[6][4];
 		{
 			float e[4];
@@ -221,6 +242,9 @@ __global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t 
 			X[0][1] = pack4(e);
 			shChunk(g, d[0], d[1], d[2], e);
 			X[0][2] = pack4(e);
+			if(TRAIN)
+				for(int t = 0; t < 3; ++t)
+					storeTile(tb.x, kXRow[0], t, g, cap, idx, X[0][t]);
 		}
 		// Hidden layers 0..4 (ReLU).
 #pragma unroll
@@ -236,6 +260,10 @@ __global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t 
 				const float r[4] = {acc.x > 0.0f ? acc.x : 0.0f, acc.y > 0.0f ? acc.y : 0.0f, acc.z > 0.0f ? acc.z : 0.0f,
 					acc.w > 0.0f ? acc.w : 0.0f};
 				X[l + 1][mt] = pack4(r);
+				if(TRAIN) {
+					storeTile(tb.x, kXRow[l + 1], mt, g, cap, idx, X[l + 1][mt]);
+					active[l + 1] |= nonzero4(X[l + 1][mt]) << (4 * mt);
+				}
 			}
 		}
 		// Output layer (rows 0..2 used).
@@ -263,6 +291,7 @@ __global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t 
 		}
 		s4 G[6][4]; // G[l][t]: gradient w.r.t. the pre-activation of layer l's output, tile t
 		G[5][0] = pack4(gy);
+		storeTile(tb.g, kGRow[5], 0, g, cap, idx, valid ? G[5][0] : s4{0, 0, 0, 0});
 		// Back through the output layer: dX5 = W5^T GY, then through each hidden layer.
 #pragma unroll
 		for(int l = 5; l >= 1; --l) {
@@ -278,8 +307,9 @@ __global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t 
 				float r[4];
 				const float a4[4] = {acc.x, acc.y, acc.z, acc.w};
 				for(int e = 0; e < 4; ++e)
-					r[e] = lane4(X[l][mt], e) > 0.0f ? a4[e] : 0.0f;
+					r[e] = ((active[l] >> (4 * mt + e)) & 1u) ? a4[e] : 0.0f;
 				G[l - 1][mt] = pack4(r);
+				storeTile(tb.g, kGRow[l - 1], mt, g, cap, idx, valid ? G[l - 1][mt] : s4{0, 0, 0, 0});
 			}
 		}
 		// dX0 = W0^T G0 for the 32 hash-grid features, scattered into the tables (the spherical
@@ -294,27 +324,8 @@ __global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t 
 				scatterLevel(grid, gt, l0 + 1, p, acc.z, acc.w);
 			}
 		}
-		// Store activations X0..X5 and gradients G0..G4 (of W0..W4's outputs) and GY (bf16,
-		// feature-major) for the weight gradients dW_l = G_l X_l^T.
-		const uint32_t cap = tb.capacity;
-		for(int l = 0; l < 6; ++l) {
-			const int tiles = l == 0 ? 3 : 4;
-			for(int t = 0; t < tiles; ++t)
-				for(int e = 0; e < 4; ++e) {
-					const int f = 16 * t + 4 * g + e;
-					const short xs = e == 0 ? X[l][t].x : (e == 1 ? X[l][t].y : (e == 2 ? X[l][t].z : X[l][t].w));
-					tb.x[size_t(kXRow[l] + f) * cap + idx] = uint16_t(xs);
-				}
-		}
-		for(int l = 0; l < 6; ++l) {
-			const int tiles = l == 5 ? 1 : 4;
-			for(int t = 0; t < tiles; ++t)
-				for(int e = 0; e < 4; ++e) {
-					const int f = 16 * t + 4 * g + e;
-					const short gs = e == 0 ? G[l][t].x : (e == 1 ? G[l][t].y : (e == 2 ? G[l][t].z : G[l][t].w));
-					tb.g[size_t(kGRow[l] + f) * cap + idx] = valid ? uint16_t(gs) : uint16_t(0);
-				}
-		}
+		// X0..X5 and G0..G4, GY (bf16, feature-major) are stored above for the weight gradients
+		// dW_l = G_l X_l^T.
 	}
 }
 

@@ -63,8 +63,25 @@ def run_job(job: Job, keep: Optional[list] = None) -> JobResult:
                      stats=st)
 
 
-def run_batch(jobs: List[Job], workers: int = 4, on_done: Optional[Callable[[JobResult], None]] = None):
-    """Run every job; returns (results in job order, wall seconds of the batch)."""
+def estimated_cost(job: Job) -> float:
+    """Rough seconds of one job on its own: the tree build (about 0.35 s per million triangles on
+    the host, 20x less on the GPU) plus the render (about 1e9 path vertices per second)."""
+    tris = sum(int(np.asarray(job.scene.meshes[o.mesh].indices).size) // 3 for o in job.scene.objects)
+    build = tris * (0.35e-6 / (20.0 if job.gpu_build else 1.0))
+    spp = 2 ** round(np.log2(max(job.samples, 1)))
+    return build + job.width * job.height * spp * job.depth / 1e9
+
+
+def run_batch(jobs: List[Job], workers: int = 4, on_done: Optional[Callable[[JobResult], None]] = None,
+              order: str = "input"):
+    """Run every job; returns (results in job order, wall seconds of the batch). order: "input"
+    starts jobs in list order; "lpt" starts the longest (estimated_cost) first, which shortens the
+    batch when a few large jobs would otherwise start last."""
+    if order not in ("input", "lpt"):
+        raise ValueError("order must be 'input' or 'lpt'")
+    seq = list(range(len(jobs)))
+    if order == "lpt":
+        seq.sort(key=lambda i: -estimated_cost(jobs[i]))
     keep: list = []
     lock = threading.Lock()
     results: List[Optional[JobResult]] = [None] * len(jobs)
@@ -80,7 +97,7 @@ def run_batch(jobs: List[Job], workers: int = 4, on_done: Optional[Callable[[Job
             on_done(r)
 
     with ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
-        list(ex.map(work, range(len(jobs))))
+        list(ex.map(work, seq))
     wall = time.perf_counter() - b0
     del keep
     return results, wall

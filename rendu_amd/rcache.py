@@ -54,7 +54,7 @@ class RadianceCache:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
+        if h is not None and h.value and _lib.lib is not None:  # the library may be gone at interpreter exit
             _lib.lib.rdu_rcache_destroy(h)
             self._h = None
 
