@@ -199,9 +199,6 @@ __device__ __forceinline__ void loadWeights(uint16_t * W, const uint16_t * w16) 
 // (scattered into the table gradients), activations and output gradients stored for the
 // weight-gradient GEMMs. Otherwise: forward only, radiance written to `out`.
 template<bool TRAIN>
-#ifdef RDU_EXP_RCW
-__attribute__((amdgpu_waves_per_eu(RDU_EXP_RCW, 8)))
-#endif
 __global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t * __restrict__ w16,
 	const uint16_t * __restrict__ t16, const float4 * __restrict__ queries, const float4 * __restrict__ targets,
 	const uint32_t * __restrict__ countDev, uint32_t n, float4 * __restrict__ out, RcTrainBuffers tb,
