@@ -147,6 +147,41 @@ def cache_experiment(pt, scn, cam, ref_spp):
             {"mode": best[0], **best[1], "time_ratio": round(best[1]["s"] / p64["s"], 3)}}
 
 
+def cache_experiment_cornell():
+    """Radiance cache on the closed Cornell box (256 x 256, depth 8, RR from bounce 3; most light is
+    indirect): RMSE against an 8192-spp plain render, plain at 64 / 128 spp against cached renders
+    at 64 spp (fresh cache each; 2 spp per pass so that the cache trains between passes)."""
+    from rendu_amd import scene as S
+    from rendu_amd import scenes
+    from rendu_amd.pathtracer import PathTracer
+    from rendu_amd.rcache import RadianceCache
+    ext = S.ESTIMATOR_EXTENDED
+    scn, cam_of = scenes.cornell_spheres(seed=3, subdivisions=4)
+    lo, hi = S.scene_bounds(scn)
+    pt = PathTracer(scn)
+    w = h = 256
+    cam = cam_of(w, h)
+    ref = np.zeros((h, w, 3), np.float32)
+    pt.render(cam, 8192, 8, ref, seed=1000, estimator=ext, rr_start=3)
+    rmse = lambda a: round(float(np.sqrt(np.mean((a.astype(np.float64) - ref) ** 2))), 6)
+    out = {"ref_spp": 8192}
+    for spp in (64, 128):
+        img = np.zeros((h, w, 3), np.float32)
+        t = time.perf_counter()
+        pt.render(cam, spp, 8, img, seed=7, estimator=ext, rr_start=3)
+        out[f"plain{spp}"] = {"s": round(time.perf_counter() - t, 4), "rmse": rmse(img)}
+    for name, kw in (("cached64_trust_test_b2", {}), ("cached64_always_b1", {"always_trust": True, "terminate_bounce": 1})):
+        img = np.zeros((h, w, 3), np.float32)
+        t = time.perf_counter()
+        st = pt.render_cached(RadianceCache(lo, hi, seed=5), cam, 64, 8, img, seed=7, estimator=ext, rr_start=3,
+                              max_paths=w * h * 2, **kw)
+        out[name] = {"s": round(time.perf_counter() - t, 4), "rmse": rmse(img), "cache_queries": st["cache_queries"],
+                     "train_steps": st["train_steps"]}
+    out["variance_ratio_plain64_over_cached64_always_b1"] = round(
+        (out["plain64"]["rmse"] / out["cached64_always_b1"]["rmse"]) ** 2, 3)
+    return out
+
+
 def load_pmc():
     """The committed rocprofv3 --pmc summary of the dominant kernel (None when absent)."""
     path = os.path.join(ROOT, "profiles", "trace_closest_soup1m_pmc.json")
@@ -396,6 +431,7 @@ def main():
                                     "graph_passes": stg["graph_passes"], "passes": stg["passes"]})
     if not args.no_cache_experiment:
         extra["cache_1080p"] = cache_experiment(pt3, scn3, cam3(1920, 1080), args.cache_ref_spp)
+        extra["cache_cornell_256"] = cache_experiment_cornell()
     del pt3
 
     # Batched jobs: 64 seeded scenes (1e4..5e5 triangles, 128^2..1024^2 pixels, 4..256 spp, depth 6),
