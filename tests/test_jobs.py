@@ -25,6 +25,40 @@ def test_random_job_distribution():
     assert a[2:] == b[2:] and len(a[0].objects) == len(b[0].objects)
 
 
+def test_estimated_cost_orders_by_work():
+    from rendu_amd import jobs as J
+    scn, cam, w, h, spp, depth = scenes.random_job(3)
+    small = J.Job(scn, cam(64, 64), 64, 64, 4, depth)
+    wide = J.Job(scn, cam(512, 512), 512, 512, 4, depth)
+    deep = J.Job(scn, cam(64, 64), 64, 64, 256, depth)
+    gpu = J.Job(scn, cam(64, 64), 64, 64, 4, depth, gpu_build=True)
+    assert J.estimated_cost(small) < J.estimated_cost(wide)
+    assert J.estimated_cost(small) < J.estimated_cost(deep)
+    assert J.estimated_cost(gpu) < J.estimated_cost(small)  # the GPU build is the cheaper build
+    # samples round to a power of two as the render does
+    odd = J.Job(scn, cam(64, 64), 64, 64, 5, depth)
+    assert J.estimated_cost(odd) == J.estimated_cost(small)
+    with pytest.raises(ValueError):
+        J.run_batch([], order="shortest")
+
+
+@pytest.mark.gpu
+def test_longest_first_batch_matches_sequential():
+    from conftest import log_case
+    from rendu_amd import jobs as J
+    log_case("5 small jobs, longest first, 4 workers vs alone")
+    js = []
+    for s in range(5):
+        scn, cam, w, h, spp, depth = scenes.random_job(200 + s)
+        side = 32 + 16 * s
+        js.append(J.Job(scn, cam(side, side), side, side, min(spp, 8), depth, seed=s))
+    alone = [J.run_job(j).image for j in js]
+    res, wall = J.run_batch(js, workers=4, order="lpt")
+    assert [r.index for r in res] == list(range(5))
+    for r, a in zip(res, alone):
+        np.testing.assert_array_equal(r.image, a)
+
+
 @pytest.mark.gpu
 def test_concurrent_jobs_match_sequential():
     from conftest import log_case
