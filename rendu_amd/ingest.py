@@ -31,7 +31,7 @@ import numpy as np
 
 from . import scene as S
 from .camera import Camera
-from .output import read_png
+from .output import read_exr, read_png
 from .workloads import MeshData
 
 F32 = np.float32
@@ -268,8 +268,20 @@ class Resources:
 
 
 def _read_image(path: str, flip: bool) -> np.ndarray:
+    if path.lower().endswith(".exr"):
+        # Image::loadHDR (Image.cpp:335-420): channels picked by name R, G, B, A (missing colour
+        # channels 0, missing alpha 1), half read as float, rows flipped like the LDR images.
+        with open(path, "rb") as f:
+            chans = read_exr(f.read())
+        h, w = next(iter(chans.values())).shape
+        out = np.zeros((h, w, 4), F32)
+        out[..., 3] = 1.0
+        for k, name in enumerate("RGBA"):
+            if name in chans:
+                out[..., k] = chans[name]
+        return out[::-1].copy() if flip else out
     if not path.lower().endswith(".png"):
-        raise NotImplementedError(f"only PNG textures are decoded ({os.path.basename(path)})")
+        raise NotImplementedError(f"only PNG and uncompressed EXR textures are decoded ({os.path.basename(path)})")
     with open(path, "rb") as f:
         px = read_png(f.read())
     if px.ndim == 2:
@@ -401,7 +413,15 @@ def load_scene(path: str, roots: Sequence[str]):
                     faces = texture(res, n, cube=True)
                     scn.background = S.Background(S.BG_SKYBOX, cube=tuple(scn.add_texture(f) for f in faces))
                 elif p.key == "sun":
-                    raise NotImplementedError("atmosphere backgrounds need Rendu's precomputed scattering table")
+                    # Scene.cpp:191-200 + Sky::decode (Sky.cpp:7-20): the precomputed scattering
+                    # table as the background texture, the sun direction normalised (zero refused).
+                    sun = _vec(p, 3)
+                    if not np.any(sun):
+                        raise ValueError("Invalid null sun direction.")
+                    sun = sun / np.float32(np.sqrt(np.float32(np.dot(sun, sun))))
+                    lut = texture(res, "scattering-precomputed")
+                    scn.background = S.Background(S.BG_ATMOSPHERE, texture=tex_id(lut),
+                                                  sun_direction=tuple(float(v) for v in sun))
         elif tok.key == "camera":
             for p in tok.elements:
                 if p.key in ("position", "center", "up"):

@@ -135,8 +135,10 @@ def exr_bytes(image: np.ndarray, ignore_alpha: bool = True) -> bytes:
 
 
 def read_exr(data: bytes) -> dict:
-    """Reader for the files exr_bytes writes: {channel name: (h, w) float32}."""
-    assert data[:4] == b"\x76\x2f\x31\x01"
+    """Reader of single-part scanline EXR files without compression and with half channels (what
+    exr_bytes writes, and the layout of Rendu's precomputed tables): {channel name: (h, w) float32}."""
+    if data[:4] != b"\x76\x2f\x31\x01":
+        raise ValueError("not an OpenEXR file")
     pos, attrs = 8, {}
     while data[pos] != 0:
         e = data.index(b"\0", pos)
@@ -146,6 +148,8 @@ def read_exr(data: bytes) -> dict:
         attrs[name] = data[e2 + 5:e2 + 5 + size]
         pos = e2 + 5 + size
     pos += 1
+    if attrs.get("compression", b"\0")[:1] != b"\0":
+        raise NotImplementedError("compressed EXR files are not decoded")
     x0, y0, x1, y1 = struct.unpack("<iiii", attrs["dataWindow"])
     w, h = x1 - x0 + 1, y1 - y0 + 1
     names, cl = [], attrs["channels"]
@@ -153,6 +157,8 @@ def read_exr(data: bytes) -> dict:
     while cl[p] != 0:
         e = cl.index(b"\0", p)
         names.append(cl[p:e].decode())
+        if struct.unpack("<i", cl[e + 1:e + 5])[0] != 1:
+            raise NotImplementedError("only half-float EXR channels are decoded")
         p = e + 1 + 16
     offsets = struct.unpack(f"<{h}Q", data[pos:pos + 8 * h])
     out = {n: np.zeros((h, w), np.float32) for n in names}

@@ -92,6 +92,74 @@ def test_cornellbox_scene_loads():
     assert np.all(np.abs(grey[..., :3] * 255 - np.round(grey[..., :3] * 255)) < 1e-4)
 
 
+COMMON = os.path.join(os.path.dirname(__file__), "golden", "common")
+SUN = "-0.287348 0.000000 0.957826"  # the sun of Rendu's pbrdemo/cornellbox-reflections.scene
+
+
+def atmosphere_scene(tmp_path):
+    """Rendu's Cornell box with the background of its cornellbox-reflections scene (an atmosphere
+    with that scene's sun), so the sky is seen through the open front."""
+    text = open(os.path.join(GOLD, "cornellbox.scene")).read()
+    text = text.replace("\tcolor: 0.000000 0.000000 0.000000 \n", f"\tsun: {SUN} \n", 1)
+    assert f"sun: {SUN}" in text
+    p = tmp_path / "cornellbox-sky.scene"
+    p.write_text(text)
+    return ingest.load_scene(str(p), [GOLD, COMMON])
+
+
+def test_rendu_scattering_table_reads():
+    """Rendu's precomputed scattering table (resources/common, copied as data): an uncompressed
+    512 x 512 half-float EXR stored A, B, G, R; Image::loadHDR picks channels by name."""
+    from rendu_amd.output import read_exr
+    path = os.path.join(COMMON, "scattering-precomputed.exr")
+    chans = read_exr(open(path, "rb").read())
+    assert sorted(chans) == ["A", "B", "G", "R"]
+    img = ingest.texture(ingest.Resources([COMMON]), "scattering-precomputed")
+    assert img.shape == (512, 512, 4) and img.dtype == np.float32 and np.isfinite(img).all()
+    for k, name in enumerate("RGBA"):
+        np.testing.assert_array_equal(img[..., k], chans[name])
+    # A transmittance-like table: every entry in [0, 1], alpha 1, and not constant.
+    assert img[..., :3].min() >= 0.0 and img[..., :3].max() <= 1.0 and np.all(img[..., 3] == 1.0)
+    assert img[..., :3].std() > 0.05
+    # half values exactly (the file holds halves)
+    assert np.array_equal(img, img.astype(np.float16).astype(np.float32))
+
+
+def test_atmosphere_background_ingested(tmp_path):
+    scn, cam_of = atmosphere_scene(tmp_path)
+    b = scn.background
+    assert b.mode == S.BG_ATMOSPHERE
+    sun = np.array([float(v) for v in SUN.split()], np.float32)
+    np.testing.assert_allclose(b.sun_direction, sun / np.linalg.norm(sun), rtol=1e-6)
+    assert scn.textures[b.texture].shape == (512, 512, 4)
+    bad = tmp_path / "null-sun.scene"
+    bad.write_text(open(os.path.join(GOLD, "cornellbox.scene")).read().replace(
+        "\tcolor: 0.000000 0.000000 0.000000 \n", "\tsun: 0 0 0 \n", 1))
+    with pytest.raises(ValueError):
+        ingest.load_scene(str(bad), [GOLD, COMMON])
+
+
+@pytest.mark.gpu
+def test_atmosphere_scene_with_rendus_table_gpu_vs_oracle(tmp_path):
+    from conftest import log_case
+    from oracle import oracle as orc
+    from rendu_amd.pathtracer import PathTracer
+    scn, cam_of = atmosphere_scene(tmp_path)
+    log_case("Rendu cornellbox.scene with the reflections scene's atmosphere, 48x48 16spp depth5")
+    cam = cam_of(48, 48)
+    got = np.zeros((48, 48, 3), np.float32)
+    PathTracer(scn).render(cam, 16, 5, got, seed=2)
+    exp = np.zeros_like(got)
+    orc.OraclePathTracer(scn).render(S.camera_desc(cam), 16, 5, exp, S.RenderOpts(2, 0, 3, 0))
+    dark = np.zeros_like(got)
+    scn.background = S.Background(S.BG_COLOR, (0.0, 0.0, 0.0))
+    PathTracer(scn).render(cam, 16, 5, dark, seed=2)
+    assert exp.mean() > dark.mean() + 1e-3  # the sky lights the box
+    d = np.abs(got.astype(np.float64) - exp)
+    assert (d <= 1e-4 * np.abs(exp) + 1e-6).mean() >= 0.97
+    assert d.mean() <= 1e-3 * np.abs(exp).mean()
+
+
 @pytest.mark.gpu
 def test_cornellbox_scene_gpu_vs_oracle():
     from conftest import log_case
