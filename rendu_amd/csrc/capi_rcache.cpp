@@ -15,6 +15,7 @@ struct rdu_rcache {
 	rdu::RcTrainBuffers tb;
 	rdu::RcAdam adam;
 	rdu::DevBuffer w32, w16, gw, mw, vw, t32, t16, gt, mt, vt, tx, tg, tloss, stagePos, stageTgt, stageOut;
+	rdu::DevBuffer rKey, rKeyAlt, rUniq, rCount, rVal, rValAlt, rUniqVal, rTemp; // table-gradient records
 	rdu::DevBuffer cells; // per coarse cell trust statistics of the path tracer's cache mode
 	int device = -1;
 };
@@ -28,13 +29,27 @@ int ensureTrain(rdu_rcache * c, uint32_t n) {
 	if(cap <= c->tb.capacity)
 		return RDU_OK;
 	int st;
+	const size_t recs = size_t(cap) * kRcRecordsPerSample;
+	const size_t tempBytes = rcReduceTempBytes(uint32_t(recs));
 	if((st = c->tx.reserve(size_t(kRcIn + 5 * kRcWidth) * cap * 2)) ||
 		(st = c->tg.reserve(size_t(5 * kRcWidth + kRcOutPad) * cap * 2)) ||
-		(st = c->tloss.reserve(size_t(cap) * 4)))
+		(st = c->tloss.reserve(size_t(cap) * 4)) || (st = c->rKey.reserve(recs * 4)) ||
+		(st = c->rKeyAlt.reserve(recs * 4)) || (st = c->rUniq.reserve(recs * 4)) || (st = c->rCount.reserve(4)) ||
+		(st = c->rVal.reserve(recs * sizeof(RcGradPair))) || (st = c->rValAlt.reserve(recs * sizeof(RcGradPair))) ||
+		(st = c->rUniqVal.reserve(recs * sizeof(RcGradPair))) || (st = c->rTemp.reserve(tempBytes)))
 		return st;
 	c->tb.x = static_cast<uint16_t *>(c->tx.ptr);
 	c->tb.g = static_cast<uint16_t *>(c->tg.ptr);
 	c->tb.loss = static_cast<float *>(c->tloss.ptr);
+	c->tb.recKey = static_cast<uint32_t *>(c->rKey.ptr);
+	c->tb.recKeyAlt = static_cast<uint32_t *>(c->rKeyAlt.ptr);
+	c->tb.uniqKey = static_cast<uint32_t *>(c->rUniq.ptr);
+	c->tb.uniqCount = static_cast<uint32_t *>(c->rCount.ptr);
+	c->tb.recVal = static_cast<RcGradPair *>(c->rVal.ptr);
+	c->tb.recValAlt = static_cast<RcGradPair *>(c->rValAlt.ptr);
+	c->tb.uniqVal = static_cast<RcGradPair *>(c->rUniqVal.ptr);
+	c->tb.temp = c->rTemp.ptr;
+	c->tb.tempBytes = tempBytes;
 	c->tb.capacity = cap;
 	return RDU_OK;
 }

@@ -11,6 +11,9 @@
 // between layers. The backward pass uses the same structure with transposed weight tiles.
 #include "rcache.hpp"
 
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce_by_key.hpp>
+
 namespace rdu {
 
 namespace {
@@ -129,15 +132,35 @@ __device__ __forceinline__ void encodeLevel(const RcGrid & grid, const uint16_t 
 	}
 }
 
-__device__ __forceinline__ void scatterLevel(const RcGrid & grid, float * gt, int level, const float p[3], float g0, float g1) {
-	float * gl = gt + size_t(level) * kRcTable * kRcFeatures;
+// Table-gradient records of one sample and level: record (sample, level, corner) holds the
+// entry's key and the corner-weighted gradients of its two features.
+__device__ __forceinline__ void recordLevel(const RcGrid & grid, const RcTrainBuffers & tb, uint32_t sample, int level,
+	const float p[3], float g0, float g1) {
+	const size_t base = (size_t(sample) * kRcLevels + size_t(level)) * 8;
 	for(int c = 0; c < 8; ++c) {
 		uint32_t idx;
 		float w;
 		cornerSetup(grid, level, p, c, idx, w);
-		atomicAdd(gl + 2 * idx, w * g0);
-		atomicAdd(gl + 2 * idx + 1, w * g1);
+		tb.recKey[base + c] = uint32_t(level) * kRcTable + idx;
+		tb.recVal[base + c] = RcGradPair{w * g0, w * g1};
 	}
+}
+
+struct GradPairSum {
+	__device__ __forceinline__ RcGradPair operator()(const RcGradPair & x, const RcGradPair & y) const {
+		return RcGradPair{x.a + y.a, x.b + y.b};
+	}
+};
+
+// gt[key] += per-key sums (keys are unique, so no two threads touch one entry).
+__global__ void rcApplyTableGrad(const uint32_t * __restrict__ keys, const RcGradPair * __restrict__ sums,
+	const uint32_t * __restrict__ count, float * __restrict__ gt) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= *count)
+		return;
+	const size_t k = keys[i];
+	gt[2 * k] += sums[i].a;
+	gt[2 * k + 1] += sums[i].b;
 }
 
 // Real spherical harmonics of degrees 0..3 of a unit direction, coefficients 4 * chunk .. + 3
@@ -317,8 +340,8 @@ __global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t 
 				acc = mfma(loadAT(W, kRcIn, mt, kt, lane), G[0][kt], acc);
 			if(valid) {
 				const int l0 = 8 * mt + 2 * g;
-				scatterLevel(grid, gt, l0, p, acc.x, acc.y);
-				scatterLevel(grid, gt, l0 + 1, p, acc.z, acc.w);
+				recordLevel(grid, tb, idx, l0, p, acc.x, acc.y);
+				recordLevel(grid, tb, idx, l0 + 1, p, acc.z, acc.w);
 			}
 		}
 		// X0..X5 and G0..G4, GY (bf16, feature-major) are stored above for the weight gradients
@@ -410,6 +433,26 @@ __global__ void rcCopy(const float * w, uint16_t * w16, size_t n) {
 		w16[i] = BF16 ? f2bf(w[i]) : f2h(w[i]);
 }
 
+// Sums the table-gradient records of n samples per entry into gt: radix sort by key (stable), then
+// a deterministic reduction by key, then one add per touched entry.
+int reduceTableGradients(RcParams & p, RcTrainBuffers & tb, uint32_t n, hipStream_t stream) {
+	const uint32_t recs = n * kRcRecordsPerSample;
+	rocprim::double_buffer<uint32_t> keys(tb.recKey, tb.recKeyAlt);
+	rocprim::double_buffer<RcGradPair> vals(tb.recVal, tb.recValAlt);
+	size_t bytes = tb.tempBytes;
+	hipError_t e = rocprim::radix_sort_pairs(tb.temp, bytes, keys, vals, recs, 0, kRcLog2Table + 4, stream);
+	if(e != hipSuccess)
+		return int(e);
+	bytes = tb.tempBytes;
+	e = rocprim::deterministic_reduce_by_key(tb.temp, bytes, keys.current(), vals.current(), size_t(recs), tb.uniqKey,
+		tb.uniqVal, tb.uniqCount, GradPairSum(), rocprim::equal_to<uint32_t>(), stream);
+	if(e != hipSuccess)
+		return int(e);
+	hipLaunchKernelGGL(rcApplyTableGrad, dim3((recs + 255) / 256), dim3(256), 0, stream, tb.uniqKey, tb.uniqVal,
+		tb.uniqCount, p.gt);
+	return int(hipGetLastError());
+}
+
 unsigned fusedBlocks(uint32_t n) {
 	const uint32_t groups = (n + 15u) / 16u;
 	uint32_t blocks = (groups + 3u) / 4u;
@@ -426,6 +469,17 @@ int weightGradients(RcParams & p, RcTrainBuffers & tb, uint32_t n, hipStream_t s
 }
 
 } // namespace
+
+size_t rcReduceTempBytes(uint32_t records) {
+	size_t a = 0, b = 0;
+	rocprim::double_buffer<uint32_t> keys(nullptr, nullptr);
+	rocprim::double_buffer<RcGradPair> vals(nullptr, nullptr);
+	(void)rocprim::radix_sort_pairs(nullptr, a, keys, vals, records, 0, kRcLog2Table + 4);
+	(void)rocprim::deterministic_reduce_by_key(nullptr, b, static_cast<const uint32_t *>(nullptr),
+		static_cast<const RcGradPair *>(nullptr), size_t(records), static_cast<uint32_t *>(nullptr),
+		static_cast<RcGradPair *>(nullptr), static_cast<uint32_t *>(nullptr), GradPairSum(), rocprim::equal_to<uint32_t>());
+	return a > b ? a : b;
+}
 
 int rcInfer(const RcGrid & grid, const RcParams & p, const float4 * queries, const uint32_t * countDev, uint32_t n,
 	float4 * out, hipStream_t stream) {
@@ -446,6 +500,8 @@ int rcGradients(const RcGrid & grid, RcParams & p, RcTrainBuffers & tb, const fl
 		targets, nullptr, n, nullptr, tb, p.gt, lossScale);
 	int e = int(hipGetLastError());
 	if(e)
+		return e;
+	if((e = reduceTableGradients(p, tb, n, stream)))
 		return e;
 	return weightGradients(p, tb, n, stream);
 }

@@ -52,12 +52,27 @@ struct RcParams {
 
 // Per-sample activations and output gradients of a training batch (bf16, feature-major
 // [feature][sample]), used by the weight-gradient GEMMs.
+// Two feature gradients of one table entry.
+struct RcGradPair {
+	float a, b;
+};
+
 struct RcTrainBuffers {
 	uint16_t * x = nullptr; // X0 (48 rows) then X1..X5 (64 rows each): 368 rows
 	uint16_t * g = nullptr; // G0..G4 (64 rows each, pre-activation gradients) then GY (16 rows): 336 rows
 	float * loss = nullptr; // per-sample loss (for monitoring)
 	uint32_t capacity = 0;
+	// Table-gradient records, one per (sample, level, corner): key = level * kRcTable + entry and
+	// the two weighted feature gradients. They are sorted by key and summed per key (in a fixed
+	// order, so the table gradients are run-to-run deterministic) instead of being added with atomics.
+	uint32_t * recKey = nullptr, * recKeyAlt = nullptr, * uniqKey = nullptr, * uniqCount = nullptr;
+	RcGradPair * recVal = nullptr, * recValAlt = nullptr, * uniqVal = nullptr;
+	void * temp = nullptr;
+	size_t tempBytes = 0;
 };
+constexpr uint32_t kRcRecordsPerSample = kRcLevels * 8;
+// Scratch bytes the sort and the reduction need for `records` records.
+size_t rcReduceTempBytes(uint32_t records);
 
 struct RcAdam {
 	// lr 3e-3: one-sample path-tracing targets are sparse and noisy; 1e-2 overshoots on them.
