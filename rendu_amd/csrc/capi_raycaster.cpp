@@ -5,6 +5,8 @@
 #include "raycaster_impl.hpp"
 
 #include <cfloat>
+#include <cstdlib>
+#include <new>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -173,6 +175,86 @@ int query(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, void * out, 
 	return RDU_OK;
 }
 
+// Per-thread resources of the per-ray entry points, one set per device: a stream, a pinned
+// host-mapped record (the ray, then the result) that the kernel reads and writes in place, and a
+// spill stack. Created on a thread's first per-ray call on a device and kept for the life of the
+// process (a stream and a few KB per thread and device, deliberately not freed at thread exit,
+// when the HIP runtime may already be gone), so that per-ray calls from many host threads need
+// no lock and no copies and run concurrently on the GPU.
+struct PerRayCtx {
+	hipStream_t stream = nullptr;
+	unsigned char * host = nullptr; // rdu_ray at 0, rdu_hit at 32, occlusion byte at 56
+	unsigned char * dev = nullptr;  // device address of `host`
+	int * gstack = nullptr;
+	size_t gstackInts = 0;
+};
+constexpr int kMaxPerRayDevices = 16;
+thread_local PerRayCtx * tlsPerRay[kMaxPerRayDevices] = {};
+
+int perRayCtx(int device, PerRayCtx ** out) {
+	if(device < 0 || device >= kMaxPerRayDevices)
+		return fail(RDU_ERR_UNSUPPORTED, "per-ray queries support devices 0..15");
+	if(!tlsPerRay[device]) {
+		PerRayCtx c;
+		int st;
+		if((st = hipCheck(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate")))
+			return st;
+		if((st = hipCheck(hipHostMalloc(reinterpret_cast<void **>(&c.host), 64, hipHostMallocMapped), "hipHostMalloc"))) {
+			(void)hipStreamDestroy(c.stream);
+			return st;
+		}
+		if((st = hipCheck(hipHostGetDevicePointer(reinterpret_cast<void **>(&c.dev), c.host, 0), "hipHostGetDevicePointer"))) {
+			(void)hipHostFree(c.host);
+			(void)hipStreamDestroy(c.stream);
+			return st;
+		}
+		tlsPerRay[device] = new(std::nothrow) PerRayCtx(c);
+		if(!tlsPerRay[device])
+			return fail(RDU_ERR_ALLOC, "host allocation failed");
+	}
+	*out = tlsPerRay[device];
+	return RDU_OK;
+}
+
+// One ray, synchronous, on the calling thread's own stream and pinned record (no raycaster lock).
+template<bool ANY>
+int queryOne(const rdu_raycaster * rc, const rdu_ray & ray, rdu_hit * hit, uint8_t * occ) {
+	int st = checkBuilt(rc);
+	if(st)
+		return st;
+	TraceOptions opt = traceOptions(perRayFlags(rc));
+	if((st = setDevice(rc->device)))
+		return st;
+	PerRayCtx * c = nullptr;
+	if((st = perRayCtx(rc->device, &c)))
+		return st;
+	const size_t ints = globalStackInts(rc->dev, opt, 1);
+	if(ints > c->gstackInts) {
+		if(c->gstack)
+			(void)hipFree(c->gstack);
+		c->gstack = nullptr;
+		c->gstackInts = 0;
+		if(std::getenv("RDU_FAIL_ALLOC") || hipMalloc(reinterpret_cast<void **>(&c->gstack), ints * sizeof(int)) != hipSuccess) {
+			(void)hipGetLastError();
+			c->gstack = nullptr;
+			return fail(RDU_ERR_ALLOC, "device allocation failed (per-ray spill stack)");
+		}
+		c->gstackInts = ints;
+	}
+	std::memcpy(c->host, &ray, sizeof(rdu_ray));
+	const float4 * dRay = reinterpret_cast<const float4 *>(c->dev);
+	const int e = ANY ? launchAny(rc->dev, dRay, 1, c->dev + 56, opt, c->gstack, nullptr, c->stream)
+					  : launchClosest(rc->dev, dRay, 1, c->dev + 32, opt, c->gstack, nullptr, c->stream);
+	if((st = hipCheck(hipError_t(e), "trace kernel launch")) ||
+		(st = hipCheck(hipStreamSynchronize(c->stream), "hipStreamSynchronize")))
+		return st;
+	if(ANY)
+		*occ = c->host[56];
+	else
+		std::memcpy(hit, c->host + 32, sizeof(rdu_hit));
+	return RDU_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -309,7 +391,7 @@ int rdu_raycaster_intersect1(const rdu_raycaster * rc, const float o[3], const f
 	if(!o || !d || !out)
 		return fail(RDU_ERR_INVALID, "null argument");
 	const rdu_ray r = {o[0], o[1], o[2], mini, d[0], d[1], d[2], maxi};
-	return query<false>(rc, &r, 1, out, perRayFlags(rc), nullptr, nullptr);
+	return queryOne<false>(rc, r, out, nullptr);
 }
 
 int rdu_raycaster_intersect_any1(const rdu_raycaster * rc, const float o[3], const float d[3], float mini, float maxi,
@@ -319,7 +401,7 @@ int rdu_raycaster_intersect_any1(const rdu_raycaster * rc, const float o[3], con
 		return fail(RDU_ERR_INVALID, "null argument");
 	const rdu_ray r = {o[0], o[1], o[2], mini, d[0], d[1], d[2], maxi};
 	uint8_t occ = 0;
-	const int st = query<true>(rc, &r, 1, &occ, perRayFlags(rc), nullptr, nullptr);
+	const int st = queryOne<true>(rc, r, nullptr, &occ);
 	*occluded = occ;
 	return st;
 }

@@ -157,6 +157,36 @@ def test_per_ray_api(mixed):
         assert vis == (not bool(ref.intersect(seg, "any")["hit"][0]))
 
 
+def test_per_ray_api_from_many_threads(mixed):
+    """Per-ray calls from 8 host threads at once (each thread has its own stream and pinned record):
+    every result equals the oracle's, as from one thread."""
+    import threading
+    ours, ref = mixed
+    log_case("per-ray intersects from 8 threads")
+    rays = workloads.soup_rays(0, 8 * 96, seed=19)
+    exp = ref.intersect(rays, "closest")
+    got = [None] * rays.shape[0]
+    occ = [None] * rays.shape[0]
+
+    def work(k):
+        for i in range(k, rays.shape[0], 8):
+            o = [rays["ox"][i], rays["oy"][i], rays["oz"][i]]
+            d = [rays["dx"][i], rays["dy"][i], rays["dz"][i]]
+            got[i] = ours.intersects(o, d)
+            occ[i] = ours.intersectsAny(o, d)
+
+    ths = [threading.Thread(target=work, args=(k,)) for k in range(8)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    for i, h in enumerate(got):
+        assert h.hit == bool(exp["hit"][i]) and occ[i] == bool(exp["hit"][i])
+        if h.hit:
+            assert (np.float32(h.dist), np.float32(h.u), np.float32(h.v)) == (exp["t"][i], exp["u"][i], exp["v"][i])
+            assert (h.meshId, h.localId) == (exp["mesh_id"][i], exp["local_id"][i])
+
+
 def test_spill_path_bit_exact(mixed, mixed_rays, monkeypatch):
     ours, ref = mixed
     monkeypatch.setenv("RDU_DEBUG_LDS_STACK", "3")
