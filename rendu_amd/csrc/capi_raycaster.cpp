@@ -89,6 +89,7 @@ int query(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, void * out, 
 		return fail(RDU_ERR_STATE, "a GPU-built tree has no reference visiting order or binary layout");
 	applyTuning(opt, tuningFor(rc, n, flags, ANY, nullptr));
 	opt.nearFar = (flags & RDU_INCOHERENT) != 0;
+	opt.unorderedAny = (flags & RDU_INCOHERENT) == 0;
 	const size_t outBytes = ANY ? n : n * sizeof(rdu_hit);
 	std::lock_guard<std::mutex> guard(rc->lock);
 	if((st = setDevice(rc->device)))

@@ -365,7 +365,9 @@ __global__ __launch_bounds__(kBlock) void traceKernel(DevBVH bvh, const float4 *
 	}
 }
 
-template<bool ANY, bool SPILL, bool STATS, bool NF>
+// ALT selects the kernel's variant: for closest hit the near/far plane fetch, for any hit the
+// unsorted visiting order.
+template<bool ANY, bool SPILL, bool STATS, bool ALT>
 __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
 	const uint32_t * __restrict__ perm, const uint32_t * __restrict__ countDev, rdu_hit * __restrict__ hits,
 	uint8_t * __restrict__ occ, int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint32_t chunk,
@@ -393,7 +395,7 @@ uint32_t refillMin(const TraceOptions & opt) {
 	return opt.spatialOrder ? 32u : 16u;
 }
 
-template<bool ANY, bool SPILL, bool STATS, bool NF>
+template<bool ANY, bool SPILL, bool STATS, bool ALT>
 PersistentGrid persistentGrid(size_t n, uint32_t blocksPerCuCap);
 
 template<bool ANY>
@@ -422,9 +424,9 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 	if(persistent) {
 		if(n > 0xffffffffull)
 			return int(hipErrorInvalidValue);
-#define RDU_PERSIST(S, ST, NF)                                                                                   \
+#define RDU_PERSIST(S, ST, ALT)                                                                                  \
 	{                                                                                                            \
-		const PersistentGrid g = persistentGrid<ANY, S, ST, NF>(n, opt.blocksPerCu);                             \
+		const PersistentGrid g = persistentGrid<ANY, S, ST, ALT>(n, opt.blocksPerCu);                            \
 		const uint32_t waves = (g.blocks / kXcds) * (kBlock / 64);                                               \
 		const uint64_t slice = (uint64_t(n) + kXcds - 1) / kXcds;                                                \
 		/* sorted batches: interleaved 64-ray chunks; input order: one contiguous run per wavefront; */        \
@@ -433,19 +435,18 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 		chunk = chunk < 64 ? 64 : chunk;                                                                        \
 		if(opt.chunk >= 16 && opt.chunk <= (1u << 20))                                                          \
 			chunk = opt.chunk;                                                                                  \
-		hipLaunchKernelGGL((tracePersistent<ANY, S, ST, NF>), dim3(g.blocks), dim3(kBlock), 0, stream, bvh, rays,  \
+		hipLaunchKernelGGL((tracePersistent<ANY, S, ST, ALT>), dim3(g.blocks), dim3(kBlock), 0, stream, bvh, rays,  \
 			uint32_t(n), perm, opt.countDev, hits, occ, gstack, lds, kXcds, refillMin(opt), uint32_t(chunk), stats); \
 	}
-		// Near/far plane fetch (closest hit only): fewer instructions per node, more registers (5
-		// waves per SIMD instead of 6); it pays on incoherent rays and costs on coherent ones.
-		bool nf = false;
-		if constexpr(!ANY)
-			nf = opt.nearFar;
-		if(nf) {
+		// Kernel variant: closest hit with the near/far plane fetch (fewer instructions per node,
+		// more registers: 5 waves per SIMD instead of 6; pays on incoherent rays), any hit without
+		// the near-first sort (pays on coherent rays).
+		const bool alt = ANY ? opt.unorderedAny : opt.nearFar;
+		if(alt) {
 			if(stats) {
-				if(spill) RDU_PERSIST(true, true, !ANY) else RDU_PERSIST(false, true, !ANY)
+				if(spill) RDU_PERSIST(true, true, true) else RDU_PERSIST(false, true, true)
 			} else {
-				if(spill) RDU_PERSIST(true, false, !ANY) else RDU_PERSIST(false, false, !ANY)
+				if(spill) RDU_PERSIST(true, false, true) else RDU_PERSIST(false, false, true)
 			}
 		} else {
 			if(stats) {
@@ -502,11 +503,13 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 // so never above the true entry): a popped child whose entry already exceeds the current maxi
 // is dropped without being fetched, exactly as its own box test would drop it (the test fails
 // once entry > maxi). Levels spilled to global memory carry no entry and are always fetched.
-template<bool ANY, bool SPILL, bool STATS, bool NF>
+template<bool ANY, bool SPILL, bool STATS, bool ALT>
 __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
 	const uint32_t * __restrict__ perm, const uint32_t * __restrict__ countDev, rdu_hit * __restrict__ hits,
 	uint8_t * __restrict__ occ, int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint32_t chunk,
 	uint64_t * stats) {
+	constexpr bool NF = ALT && !ANY;      // near/far plane fetch
+	constexpr bool UNSORTED = ALT && ANY; // children in node order
 	__shared__ int ls[kWideLds][kBlock];
 	__shared__ _Float16 le[ANY ? 1 : kWideLds][kBlock];
 	if(countDev)
@@ -685,6 +688,19 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 			const int nh = int(h0) + int(h1) + int(h2) + int(h3);
 			if(nh == 0) {
 				next = true;
+			} else if constexpr(UNSORTED) {
+				// Any hit: the answer does not depend on the visiting order, so no sorting: go on
+				// with the first child that passed and push the others (cheaper per node; near-first
+				// finds a hit sooner on incoherent rays, this pays on coherent ones).
+				const int first = h0 ? 0 : (h1 ? 1 : (h2 ? 2 : 3));
+				if(h1 && first != 1)
+					st.push<SPILL>(c1);
+				if(h2 && first != 2)
+					st.push<SPILL>(c2);
+				if(h3 && first != 3)
+					st.push<SPILL>(c3);
+				c = first == 0 ? c0 : (first == 1 ? c1 : (first == 2 ? c2 : c3));
+				kk = 0;
 			} else {
 				// A child that passed its test has a finite entry (mini <= entry <= maxi); the others
 				// sort last.
@@ -761,7 +777,7 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 	}
 }
 
-template<bool ANY, bool SPILL, bool STATS, bool NF>
+template<bool ANY, bool SPILL, bool STATS, bool ALT>
 PersistentGrid persistentGrid(size_t n, uint32_t blocksPerCuCap) {
 	struct Occupancy {
 		int blocksPerCu = 1, cus = 1;
@@ -772,7 +788,7 @@ PersistentGrid persistentGrid(size_t n, uint32_t blocksPerCuCap) {
 		int dev = 0;
 		(void)hipGetDevice(&dev);
 		(void)hipDeviceGetAttribute(&o.cus, hipDeviceAttributeMultiprocessorCount, dev);
-		(void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocksPerCu, tracePersistent<ANY, SPILL, STATS, NF>, kBlock, 0);
+		(void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocksPerCu, tracePersistent<ANY, SPILL, STATS, ALT>, kBlock, 0);
 		o.blocksPerCu = o.blocksPerCu < 1 ? 1 : o.blocksPerCu;
 		o.cus = o.cus < 1 ? 1 : o.cus;
 		return o;

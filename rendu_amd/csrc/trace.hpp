@@ -82,6 +82,8 @@ struct TraceOptions {
 	uint32_t blocksPerCu = 0;  // resident workgroups per CU of the persistent grid (0 = occupancy limit)
 	bool nearFar = false;      // closest hit: fetch node planes ordered near/far by the ray's direction signs
 	                           // (fewer instructions per node, lower occupancy: for incoherent rays)
+	bool unorderedAny = false; // any hit: visit the children that pass in node order, no near-first sort
+	                           // (for coherent rays)
 };
 
 // Launch parameters of the persistent traversal picked by the autotuner (0 = default).
