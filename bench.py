@@ -147,6 +147,48 @@ def cache_experiment(pt, scn, cam, ref_spp):
             {"mode": best[0], **best[1], "time_ratio": round(best[1]["s"] / p64["s"], 3)}}
 
 
+def cache_kernels():
+    """Radiance-cache kernels alone (HIP events on the launch stream): inference over 2^20 queries and
+    training steps of 16,384 pairs (forward, backward, sorted table-gradient reduction, weight
+    gradients, Adam), against the dense bf16 MFMA peak. Issued MLP work: 40,960 FLOP per query
+    forward (48x64 + 4x64x64 + 16x64 multiply-adds), about 3x that per training pair."""
+    import torch
+    from rendu_amd.rcache import RadianceCache, make_queries
+    rng = np.random.default_rng(4)
+    cache = RadianceCache([0, 0, 0], [1, 1, 1], seed=3)
+    nq = 1 << 20
+    q = torch.from_numpy(make_queries(rng.random((nq, 3)).astype(np.float32))).cuda()
+    out = torch.empty((nq, 4), dtype=torch.float32, device="cuda")
+    nt = 16384
+    cent = rng.random((64, 3))
+    pos = (cent[rng.integers(0, 64, nt)] + 0.02 * rng.standard_normal((nt, 3))).clip(0, 1).astype(np.float32)
+    qt = torch.from_numpy(make_queries(pos)).cuda()
+    tt = torch.from_numpy(rng.random((nt, 4)).astype(np.float32)).cuda()
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+
+    def timed(fn, reps):
+        fn()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    inf_ms = timed(lambda: cache.infer_device(q.data_ptr(), nq, out.data_ptr(), stream=sh), 20)
+    tr_ms = timed(lambda: cache.train_device(qt.data_ptr(), tt.data_ptr(), nt, steps=1, stream=sh), 20)
+    flop_q = 2 * (48 * 64 + 4 * 64 * 64 + 16 * 64)
+    inf_tf = nq * flop_q / (inf_ms * 1e-3) / 1e12
+    tr_tf = nt * 3 * flop_q / (tr_ms * 1e-3) / 1e12
+    return {"infer_2e20_ms": round(inf_ms, 4), "infer_gqueries_per_s": round(nq / (inf_ms * 1e-3) / 1e9, 3),
+            "infer_tflops_issued": round(inf_tf, 2), "train_step_16k_ms": round(tr_ms, 4),
+            "train_mpairs_per_s": round(nt / (tr_ms * 1e-3) / 1e6, 2), "train_tflops_issued": round(tr_tf, 2),
+            "bf16_dense_peak_tflops": 2500.0, "infer_frac_of_peak": round(inf_tf / 2500.0, 4),
+            "table_gather_bytes_per_query": 512}
+
+
 def cache_experiment_cornell():
     """Radiance cache on the closed Cornell box (256 x 256, depth 8, RR from bounce 3; most light is
     indirect): RMSE against an 8192-spp plain render, plain at 64 / 128 spp against cached renders
@@ -432,6 +474,7 @@ def main():
     if not args.no_cache_experiment:
         extra["cache_1080p"] = cache_experiment(pt3, scn3, cam3(1920, 1080), args.cache_ref_spp)
         extra["cache_cornell_256"] = cache_experiment_cornell()
+    extra["cache_kernels"] = cache_kernels()
     del pt3
 
     # Batched jobs: 64 seeded scenes (1e4..5e5 triangles, 128^2..1024^2 pixels, 4..256 spp, depth 6),
