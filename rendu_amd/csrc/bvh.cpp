@@ -2,8 +2,10 @@
 #include "bvh.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <limits>
+#include <thread>
 
 namespace rdu {
 
@@ -84,58 +86,144 @@ void HostScene::addMesh(const float * xyz, size_t nv, const uint32_t * idx, size
 	++meshCount;
 }
 
-void HostScene::build() {
+namespace {
+
+// One iteration of Raycaster::updateHierarchy's loop (Raycaster.cpp:54-142) on a node covering
+// tris[begin, begin + count): its box; a leaf below 3 triangles; else the split of the range (mean
+// of the centroids on the largest axis and std::partition, or the count / 2 nth_element fallback).
+// Returns whether the node is internal, with `split` triangles going left.
+bool splitNode(std::vector<HTri> & tris, HNode & node, uint64_t begin, uint64_t count, uint64_t & split) {
+	HBox global = tris[begin].box;
+	for(uint64_t t = 1; t < count; ++t)
+		mergeInto(global, tris[begin + t].box);
+	node.box = global;
+	if(count < 3) {
+		node.leaf = true;
+		node.left = begin;
+		node.right = count;
+		return false;
+	}
+	node.leaf = false;
+	split = 0;
+	const f3 s = global.mx - global.mn;
+	const int axis = (s.x >= s.y && s.x >= s.z) ? 0 : (s.y >= s.z ? 1 : 2);
+	HTri * first = tris.data() + begin;
+	HTri * last = first + count;
+	if(count >= 5) {
+		// Sequential float sum in triangle order, then one division (Raycaster.cpp:102-106).
+		float mean = 0.0f;
+		for(uint64_t t = 0; t < count; ++t)
+			mean += comp(first[t].c, axis);
+		mean /= float(count);
+		HTri * mid = std::partition(first, last, [mean, axis](const HTri & a) { return comp(a.c, axis) < mean; });
+		split = uint64_t(mid - first);
+	}
+	if(split == 0 || split == count || count < 5) {
+		split = count / 2;
+		std::nth_element(first, first + split, last,
+			[axis](const HTri & a, const HTri & b) { return comp(a.c, axis) < comp(b.c, axis); });
+	}
+	return true;
+}
+
+// Subtrees at least this large are built on their own thread (while spawning depth is left).
+constexpr uint64_t kParallelMin = 32768;
+
+// Appends `sub` (a subtree list: its root at sub[0], descendants after, child links local to sub)
+// to `out`: sub[0] goes to out[at], the descendants to the end of out, links remapped.
+void stitch(std::vector<HNode> & out, uint64_t at, const std::vector<HNode> & sub) {
+	const uint64_t base = out.size(); // where sub[1] lands
+	auto remap = [base](HNode n) {
+		if(!n.leaf) {
+			n.left = base + (n.left - 1);
+			n.right = base + (n.right - 1);
+		}
+		return n;
+	};
+	out[at] = remap(sub[0]);
+	for(size_t i = 1; i < sub.size(); ++i)
+		out.push_back(remap(sub[i]));
+}
+
+// Builds the subtree whose root record is out[0] (covering tris[begin, begin + count)), appending
+// its descendants to `out` in exactly the order Rendu's single LIFO loop allocates them: a node's
+// two children together, then everything below the right child (pushed last, so popped first),
+// then everything below the left child. Large subtrees split into threads for `spawn` more
+// levels; their lists are stitched in that same order, so the result does not depend on the
+// number of threads.
+void buildSubtree(std::vector<HTri> & tris, std::vector<HNode> & out, uint64_t begin, uint64_t count, int spawn) {
 	struct Work {
 		uint64_t id, begin, count;
 	};
+	if(spawn > 0 && count >= kParallelMin) {
+		uint64_t split = 0;
+		if(!splitNode(tris, out[0], begin, count, split))
+			return;
+		out.resize(3); // left child at 1, right child at 2
+		out[0].left = 1;
+		out[0].right = 2;
+		std::vector<HNode> right(1), left(1);
+		right.reserve(2 * (count - split));
+		left.reserve(2 * split);
+		std::thread worker([&] { buildSubtree(tris, right, begin + split, count - split, spawn - 1); });
+		buildSubtree(tris, left, begin, split, spawn - 1);
+		worker.join();
+		stitch(out, 2, right);
+		stitch(out, 1, left);
+		return;
+	}
 	std::vector<Work> stack;
-	stack.reserve(64 + meshCount);
-	for(uint64_t r = 0; r < meshCount; ++r)
-		stack.push_back({r, nodes[r].left, nodes[r].right});
-	nodes.reserve(nodes.size() + 2 * tris.size());
-
+	stack.push_back({0, begin, count});
 	while(!stack.empty()) {
 		const Work cur = stack.back();
 		stack.pop_back();
-		const uint64_t begin = cur.begin, count = cur.count;
-		HBox global = tris[begin].box;
-		for(uint64_t t = 1; t < count; ++t)
-			mergeInto(global, tris[begin + t].box);
-		nodes[cur.id].box = global;
-		if(count < 3) {
-			nodes[cur.id].leaf = true;
-			nodes[cur.id].left = begin;
-			nodes[cur.id].right = count;
-			continue;
-		}
-		nodes[cur.id].leaf = false;
 		uint64_t split = 0;
-		const f3 s = global.mx - global.mn;
-		const int axis = (s.x >= s.y && s.x >= s.z) ? 0 : (s.y >= s.z ? 1 : 2);
-		HTri * first = tris.data() + begin;
-		HTri * last = first + count;
-		if(count >= 5) {
-			// Sequential float sum in triangle order, then one division (Raycaster.cpp:102-106).
-			float mean = 0.0f;
-			for(uint64_t t = 0; t < count; ++t)
-				mean += comp(first[t].c, axis);
-			mean /= float(count);
-			HTri * mid = std::partition(first, last, [mean, axis](const HTri & a) { return comp(a.c, axis) < mean; });
-			split = uint64_t(mid - first);
+		if(!splitNode(tris, out[cur.id], cur.begin, cur.count, split))
+			continue;
+		out.emplace_back();
+		const uint64_t leftPos = out.size() - 1;
+		out[cur.id].left = leftPos;
+		stack.push_back({leftPos, cur.begin, split});
+		out.emplace_back();
+		const uint64_t rightPos = out.size() - 1;
+		out[cur.id].right = rightPos;
+		stack.push_back({rightPos, cur.begin + split, cur.count - split});
+	}
+}
+
+} // namespace
+
+void HostScene::build() {
+	// Raycaster.cpp:54-142: one LIFO loop seeded with the mesh roots 0..M-1, so the last mesh's
+	// subtree is built (and its nodes allocated) first. The meshes' triangle ranges are disjoint:
+	// each subtree is built on its own (on a pool of threads), then the lists are stitched in the
+	// loop's order (root M-1 first), which reproduces its node array exactly.
+	const uint64_t m = meshCount;
+	std::vector<std::vector<HNode>> subs(m);
+	// Threads only pay off for big scenes (starting one costs tens of microseconds).
+	const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+	const unsigned workers = tris.size() < kParallelMin ? 1u : unsigned(std::min<uint64_t>(hw, m));
+	const int spawn = 4; // up to 16 threads inside a mesh, only for subtrees of kParallelMin or more
+	std::atomic<uint64_t> next{0};
+	auto run = [&] {
+		for(uint64_t i = next.fetch_add(1); i < m; i = next.fetch_add(1)) {
+			const uint64_t r = m - 1 - i; // the loop's order: biggest index first
+			subs[r].assign(1, nodes[r]);
+			subs[r].reserve(2 * nodes[r].right);
+			buildSubtree(tris, subs[r], nodes[r].left, nodes[r].right, spawn);
 		}
-		if(split == 0 || split == count || count < 5) {
-			split = count / 2;
-			std::nth_element(first, first + split, last,
-				[axis](const HTri & a, const HTri & b) { return comp(a.c, axis) < comp(b.c, axis); });
-		}
-		nodes.emplace_back();
-		const uint64_t leftPos = nodes.size() - 1;
-		nodes[cur.id].left = leftPos;
-		stack.push_back({leftPos, begin, split});
-		nodes.emplace_back();
-		const uint64_t rightPos = nodes.size() - 1;
-		nodes[cur.id].right = rightPos;
-		stack.push_back({rightPos, begin + split, count - split});
+	};
+	std::vector<std::thread> pool;
+	for(unsigned t = 1; t < workers; ++t)
+		pool.emplace_back(run);
+	run();
+	for(std::thread & t : pool)
+		t.join();
+	nodes.reserve(nodes.size() + 2 * tris.size());
+	for(uint64_t i = 0; i < m; ++i) {
+		const uint64_t r = m - 1 - i;
+		stitch(nodes, r, subs[r]);
+		std::vector<HNode>().swap(subs[r]);
 	}
 	built = true;
 }
@@ -275,6 +363,7 @@ void HostScene::flatten(DeviceLayout & out) const {
 		uint32_t depth;
 	};
 	std::vector<WItem> wst;
+	out.wide.reserve((ext.size() / 2 + 1) * 32);
 	if(ext[top].leaf) {
 		out.wideTopCode = leafCode(ext[top].left, ext[top].right);
 	} else {
@@ -286,11 +375,12 @@ void HostScene::flatten(DeviceLayout & out) const {
 		const WItem it = wst.back();
 		wst.pop_back();
 		out.wideDepth = std::max(out.wideDepth, it.depth);
-		std::vector<uint64_t> kids = {ext[it.id].left, ext[it.id].right};
-		while(kids.size() < 4) {
+		uint64_t kids[4] = {ext[it.id].left, ext[it.id].right, 0, 0};
+		size_t nk = 2;
+		while(nk < 4) {
 			int best = -1;
 			float bestArea = -1.0f;
-			for(size_t k = 0; k < kids.size(); ++k)
+			for(size_t k = 0; k < nk; ++k)
 				if(!ext[kids[k]].leaf && area(kids[k]) > bestArea) {
 					best = int(k);
 					bestArea = area(kids[k]);
@@ -298,13 +388,17 @@ void HostScene::flatten(DeviceLayout & out) const {
 			if(best < 0)
 				break;
 			const uint64_t e = kids[size_t(best)];
+			for(size_t k = nk; k > size_t(best) + 1; --k) // open a slot after `best`
+				kids[k] = kids[k - 1];
 			kids[size_t(best)] = ext[e].left;
-			kids.insert(kids.begin() + best + 1, ext[e].right);
+			kids[size_t(best) + 1] = ext[e].right;
+			++nk;
 		}
 		int32_t codes[4] = {kWideEmpty, kWideEmpty, kWideEmpty, kWideEmpty};
 		float * o = out.wide.data() + size_t(it.slot) * 32;
-		std::vector<WItem> children;
-		for(size_t k = 0; k < kids.size(); ++k) {
+		WItem children[4];
+		size_t nc = 0;
+		for(size_t k = 0; k < nk; ++k) {
 			const HNode & c = ext[kids[k]];
 			o[0 + k] = c.box.mn.x;
 			o[4 + k] = c.box.mx.x;
@@ -316,15 +410,15 @@ void HostScene::flatten(DeviceLayout & out) const {
 				codes[k] = leafCode(c.left, c.right);
 			} else {
 				codes[k] = int32_t(out.wideCount++);
-				children.push_back({kids[k], uint32_t(codes[k]), it.depth + 1});
+				children[nc++] = {kids[k], uint32_t(codes[k]), it.depth + 1};
 			}
 		}
 		out.wide.resize(size_t(out.wideCount) * 32, 0.0f);
 		o = out.wide.data() + size_t(it.slot) * 32;
 		for(int k = 0; k < 4; ++k)
 			o[24 + k] = fromBits(uint32_t(codes[k]));
-		for(auto c = children.rbegin(); c != children.rend(); ++c)
-			wst.push_back(*c);
+		for(size_t k = nc; k > 0; --k)
+			wst.push_back(children[k - 1]);
 	}
 
 	out.roots.assign(size_t(meshCount) * 8, 0.0f);

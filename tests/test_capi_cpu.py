@@ -72,12 +72,18 @@ def _meshes_mixed():
     return out
 
 
-@pytest.mark.parametrize("case", ["soup", "mixed", "tiny"])
+@pytest.mark.parametrize("case", ["soup", "mixed", "tiny", "big", "big-mixed"])
 def test_host_tree_equals_oracle_tree(case):
+    """The host build equals the oracle's node for node. The big cases pass the size above which
+    the build splits subtrees over threads (32768 triangles), within one mesh and across meshes."""
     if case == "soup":
         meshes = [(workloads.triangle_soup(20000, seed=3), None)]
     elif case == "mixed":
         meshes = _meshes_mixed()
+    elif case == "big":
+        meshes = [(workloads.triangle_soup(300000, seed=4), None)]
+    elif case == "big-mixed":
+        meshes = [(workloads.triangle_soup(n, seed=n), None) for n in (70000, 7, 40000, 90000, 3)] + _meshes_mixed()
     else:
         meshes = [(workloads.triangle_soup(n, seed=n), None) for n in (1, 2, 3, 4, 5, 9)]
     ours, ref = _scene_pair(meshes)
