@@ -120,6 +120,33 @@ def test_state_and_argument_errors():
     assert e.value.code == _lib.RDU_ERR_STATE
 
 
+def test_tuning_and_graph_argument_errors():
+    import ctypes
+    lib = _lib.lib
+    rc = Raycaster()
+    rep = _lib.TuneReport()
+    # Tuning queries need a built tree; explicit parameters are range-checked.
+    assert lib.rdu_raycaster_tuning(rc._h, 1024, 0, 0, ctypes.byref(rep)) == _lib.RDU_ERR_STATE
+    rc.addMesh(workloads.triangle_soup(100, seed=1))
+    _update(rc)
+    assert lib.rdu_raycaster_tuning(rc._h, 1024, 0, 0, None) == _lib.RDU_ERR_INVALID
+    rc.set_tuning(24, 128, 2, incoherent=True)
+    t = rc.tuning(1 << 20, incoherent=True)
+    assert (t["refill_min"], t["chunk"], t["blocks_per_cu"], t["source"]) == (24, 128, 2, 1)
+    assert rc.tuning(1 << 20)["source"] == 0  # another class
+    for bad in ((65, 0, 0), (16, 8, 0), (16, 0, 65)):
+        with pytest.raises(_lib.RduError) as e:
+            rc.set_tuning(*bad)
+        assert e.value.code == _lib.RDU_ERR_INVALID
+    assert lib.rdu_raycaster_set_tuning(None, 0, 0, ctypes.byref(rep)) == _lib.RDU_ERR_INVALID
+    # Scene-level entry points on a null handle.
+    assert lib.rdu_scene_set_graph(None, 1) == _lib.RDU_ERR_INVALID
+    assert lib.rdu_scene_autotune(None, None, 16, 5, 64, 64, None, None) == _lib.RDU_ERR_INVALID
+    assert lib.rdu_tune_cache_size(None) == _lib.RDU_ERR_INVALID
+    assert lib.rdu_tune_cache_save(None) == _lib.RDU_ERR_INVALID
+    assert lib.rdu_tune_cache_load(None) == _lib.RDU_ERR_INVALID
+
+
 def test_zero_triangle_mesh_rejected():
     rc = Raycaster()
     rc.addMesh(workloads.MeshData(np.zeros((3, 3), F32), np.zeros(0, np.uint32)))
