@@ -178,3 +178,23 @@ def test_graph_replays_across_seeds_and_recaptures_on_a_new_frame():
     img = np.zeros((32, 32, 3), F32)
     st = pt.render(sizes[(32, 32)], 8, 4, img, seed=1, max_paths=32 * 32 * 2)
     assert np.array_equal(img, refs[(32, 32, 1)]) and st["graph_passes"] == 0
+
+
+def test_graph_replay_masked_walk_and_lights():
+    """A scene with alpha cut-outs (the closest-hit shadow walk, several kernels per layer), three
+    analytic lights and a lat-long background replays bit for bit."""
+    from test_gpu_pathtracer import feature_scene
+    from rendu_amd import scene as S
+    scn, cam_of = feature_scene(S.BG_LATLONG)
+    w, h = 40, 30
+    cam = cam_of(w, h)
+    ref = np.zeros((h, w, 3), F32)
+    PathTracer(scn).render(cam, 8, 5, ref, seed=3, max_paths=w * h * 2)
+    pt = PathTracer(scn)
+    pt.set_graph(True)
+    for k in range(3):
+        log_case(f"graph replay feature scene render {k}")
+        img = np.zeros((h, w, 3), F32)
+        st = pt.render(cam, 8, 5, img, seed=3, max_paths=w * h * 2)
+        assert np.array_equal(img, ref)
+        assert st["graph_passes"] == (3 if k == 0 else 4) and st["shadow_rays"] > 0
