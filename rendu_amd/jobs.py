@@ -49,18 +49,29 @@ class JobResult:
     stats: dict = field(default_factory=dict)
 
 
-def run_job(job: Job, keep: Optional[list] = None) -> JobResult:
+def _build(job: Job):
     t0 = time.perf_counter()
     pt = PathTracer(job.scene, gpu_build=job.gpu_build)
-    t1 = time.perf_counter()
+    return pt, t0, time.perf_counter()
+
+
+def _render(job: Job, pt: PathTracer, t0: float, t1: float) -> JobResult:
+    """Render a built job; t0 / t1 are its build's start and end."""
     img = np.zeros((job.height, job.width, 3), np.float32)
+    r0 = time.perf_counter()
     st = pt.render(job.camera, job.samples, job.depth, img, seed=job.seed, estimator=job.estimator,
                    rr_start=job.rr_start)
     t2 = time.perf_counter()
+    return JobResult(-1, img, t2 - t0, t1 - t0, t2 - r0, st["paths"], st["extension_rays"] + st["shadow_rays"],
+                     stats=st)
+
+
+def run_job(job: Job, keep: Optional[list] = None) -> JobResult:
+    pt, t0, t1 = _build(job)
+    r = _render(job, pt, t0, t1)
     if keep is not None:
         keep.append(pt)
-    return JobResult(-1, img, t2 - t0, t1 - t0, t2 - t1, st["paths"], st["extension_rays"] + st["shadow_rays"],
-                     stats=st)
+    return r
 
 
 def estimated_cost(job: Job) -> float:
@@ -73,12 +84,19 @@ def estimated_cost(job: Job) -> float:
 
 
 def run_batch(jobs: List[Job], workers: int = 4, on_done: Optional[Callable[[JobResult], None]] = None,
-              order: str = "input"):
+              order: str = "input", build_workers: int = 0):
     """Run every job; returns (results in job order, wall seconds of the batch). order: "input"
     starts jobs in list order; "lpt" starts the longest (estimated_cost) first, which shortens the
-    batch when a few large jobs would otherwise start last."""
+    batch when a few large jobs would otherwise start last.
+
+    build_workers = 0: each of the `workers` threads builds and then renders its job. build_workers
+    > 0: the builds run in a pool of their own, in the same order, ahead of the renders (each
+    render waits for its own build only), so the GPU starts after the first build instead of
+    after `workers` concurrent ones, and later builds overlap earlier renders."""
     if order not in ("input", "lpt"):
         raise ValueError("order must be 'input' or 'lpt'")
+    if build_workers < 0:
+        raise ValueError("build_workers must be >= 0")
     seq = list(range(len(jobs)))
     if order == "lpt":
         seq.sort(key=lambda i: -estimated_cost(jobs[i]))
@@ -86,20 +104,34 @@ def run_batch(jobs: List[Job], workers: int = 4, on_done: Optional[Callable[[Job
     lock = threading.Lock()
     results: List[Optional[JobResult]] = [None] * len(jobs)
     b0 = time.perf_counter()
+    bex = ThreadPoolExecutor(max_workers=build_workers) if build_workers > 0 else None
+    builds = {i: bex.submit(_build, jobs[i]) for i in seq} if bex else {}
 
     def work(i):
-        s = time.perf_counter() - b0
-        r = run_job(jobs[i], keep)
+        if bex:
+            pt, t0, t1 = builds[i].result()
+            r = _render(jobs[i], pt, t0, t1)
+            s = t0 - b0
+        else:
+            s = time.perf_counter() - b0
+            pt, t0, t1 = _build(jobs[i])
+            r = _render(jobs[i], pt, t0, t1)
+            with lock:
+                keep.append(pt)
         r.index, r.started_s = i, s
         with lock:
             results[i] = r
         if on_done:
             on_done(r)
 
-    with ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
-        list(ex.map(work, seq))
+    try:
+        with ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
+            list(ex.map(work, seq))
+    finally:
+        if bex:
+            bex.shutdown(wait=True)
     wall = time.perf_counter() - b0
-    del keep
+    del keep, builds
     return results, wall
 
 

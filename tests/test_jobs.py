@@ -40,6 +40,8 @@ def test_estimated_cost_orders_by_work():
     assert J.estimated_cost(odd) == J.estimated_cost(small)
     with pytest.raises(ValueError):
         J.run_batch([], order="shortest")
+    with pytest.raises(ValueError):
+        J.run_batch([], build_workers=-1)
 
 
 @pytest.mark.gpu
@@ -57,6 +59,24 @@ def test_longest_first_batch_matches_sequential():
     assert [r.index for r in res] == list(range(5))
     for r, a in zip(res, alone):
         np.testing.assert_array_equal(r.image, a)
+
+
+@pytest.mark.gpu
+def test_pipelined_builds_match_sequential():
+    from conftest import log_case
+    from rendu_amd import jobs as J
+    log_case("5 small jobs, builds in their own pool of 2 ahead of 4 render threads, vs alone")
+    js = []
+    for s in range(5):
+        scn, cam, w, h, spp, depth = scenes.random_job(300 + s)
+        side = 40 + 8 * s
+        js.append(J.Job(scn, cam(side, side), side, side, min(spp, 8), depth, seed=s))
+    alone = [J.run_job(j).image for j in js]
+    res, wall = J.run_batch(js, workers=4, order="lpt", build_workers=2)
+    assert [r.index for r in res] == list(range(5))
+    for r, a in zip(res, alone):
+        np.testing.assert_array_equal(r.image, a)
+        assert r.latency_s >= r.build_s + r.render_s - 1e-6
 
 
 @pytest.mark.gpu
