@@ -1,8 +1,10 @@
 #include "capi_common.hpp"
 #include "../../include/rdu.h"
 
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 namespace rdu {
 
@@ -12,6 +14,30 @@ thread_local std::string tlError;
 bool allocFaultInjected() {
 	const char * v = std::getenv("RDU_FAIL_ALLOC");
 	return v && std::strcmp(v, "1") == 0;
+}
+
+// Library buffers come from the current device's default stream-ordered pool, whose release
+// threshold is raised once per device so that freed blocks stay in the pool for the next
+// allocation instead of going back to the driver. Every driver allocation is a kernel call that
+// serialises with the other threads' driver calls (and has been seen to take over a second), so
+// scenes built and dropped one after another, or from several threads, reuse memory without them.
+int poolReady(int device) {
+	static std::mutex m;
+	static uint64_t ready = 0;
+	if(device < 0 || device >= 64)
+		return RDU_OK; // no pool set-up: plain pool behaviour
+	std::lock_guard<std::mutex> g(m);
+	if((ready >> device) & 1u)
+		return RDU_OK;
+	hipMemPool_t pool = nullptr;
+	int st = hipCheck(hipDeviceGetDefaultMemPool(&pool, device), "hipDeviceGetDefaultMemPool");
+	if(st)
+		return st;
+	uint64_t keep = UINT64_MAX;
+	if((st = hipCheck(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep), "hipMemPoolSetAttribute")))
+		return st;
+	ready |= uint64_t(1) << device;
+	return RDU_OK;
 }
 } // namespace
 
@@ -45,8 +71,13 @@ int currentDevice(int * device) {
 DevBuffer::~DevBuffer() { release(); }
 
 void DevBuffer::release() {
-	if(ptr)
-		(void)hipFree(ptr);
+	if(ptr) {
+		// As hipFree would: wait until no stream can still use the block, then return it (to the
+		// pool, where the next allocation of any stream may take it at once).
+		(void)hipDeviceSynchronize();
+		(void)hipFreeAsync(ptr, nullptr);
+		(void)hipStreamSynchronize(nullptr);
+	}
 	ptr = nullptr;
 	bytes = 0;
 }
@@ -57,11 +88,20 @@ int DevBuffer::reserve(size_t n) {
 	release();
 	if(allocFaultInjected())
 		return fail(RDU_ERR_ALLOC, "device allocation failed (RDU_FAIL_ALLOC=1)");
+	int device = 0, st;
+	if((st = hipCheck(hipGetDevice(&device), "hipGetDevice")) || (st = poolReady(device)))
+		return st;
 	const size_t want = n < 256 ? 256 : n;
-	const hipError_t e = hipMalloc(&ptr, want);
+	// Allocated in the null stream's order, which is complete once that stream has drained: the
+	// block is then usable from every stream, like a hipMalloc result.
+	hipError_t e = hipMallocAsync(&ptr, want, nullptr);
+	if(e == hipSuccess)
+		e = hipStreamSynchronize(nullptr);
 	if(e != hipSuccess) {
+		if(ptr)
+			(void)hipFreeAsync(ptr, nullptr);
 		ptr = nullptr;
-		return hipCheck(e, "hipMalloc");
+		return hipCheck(e, "hipMallocAsync");
 	}
 	bytes = want;
 	return RDU_OK;

@@ -7,7 +7,80 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <memory>
+#include <mutex>
 #include <vector>
+
+namespace rdu {
+
+// Working memory of one render of the wavefront path tracer: path state, ray and shadow queues,
+// counters and the device image. It belongs to a render, not to a scene: a render leases a set
+// from a per-process pool and gives it back when it returns, so the process holds one set per
+// render that ran at the same time (each grown to the largest pass it served and kept for reuse)
+// instead of one per scene. Batches of many scenes then reuse memory instead of asking the driver
+// for hundreds of megabytes per scene.
+struct RenderPaths {
+	int device = 0;
+	DevBuffer pRays, pHits, pAtten, pColor, pQ0, pQ1, pCounts, pTotals, pSRays, pSContrib, pSQ0, pSQ1, pOcc, pSHits,
+		pGstack, pImage;
+	size_t pathCap = 0;
+};
+
+namespace {
+std::mutex pathsMutex;
+std::vector<std::unique_ptr<RenderPaths>> pathsAll; // every set ever made (process lifetime)
+std::vector<RenderPaths *> pathsFree;
+
+// A free set of `device` for passes of P paths: the smallest one that already holds P paths,
+// else the largest (which then grows), else a new one.
+RenderPaths * leasePaths(int device, size_t P) {
+	std::lock_guard<std::mutex> g(pathsMutex);
+	size_t pick = pathsFree.size();
+	for(size_t i = 0; i < pathsFree.size(); ++i) {
+		const RenderPaths * c = pathsFree[i];
+		if(c->device != device)
+			continue;
+		if(pick == pathsFree.size()) {
+			pick = i;
+			continue;
+		}
+		const RenderPaths * b = pathsFree[pick];
+		const bool cFits = c->pathCap >= P, bFits = b->pathCap >= P;
+		if((cFits && (!bFits || c->pathCap < b->pathCap)) || (!cFits && !bFits && c->pathCap > b->pathCap))
+			pick = i;
+	}
+	if(pick < pathsFree.size()) {
+		RenderPaths * r = pathsFree[pick];
+		pathsFree.erase(pathsFree.begin() + std::ptrdiff_t(pick));
+		return r;
+	}
+	pathsAll.push_back(std::make_unique<RenderPaths>());
+	pathsAll.back()->device = device;
+	return pathsAll.back().get();
+}
+
+// Holds a leased set for the duration of a render. On return the render's stream is drained first
+// (already the case after a successful render; an error may leave work in flight).
+class PathsLease {
+public:
+	PathsLease(int device, size_t P, hipStream_t stream) : _r(leasePaths(device, P)), _stream(stream) {}
+	~PathsLease() {
+		(void)hipStreamSynchronize(_stream);
+		std::lock_guard<std::mutex> g(pathsMutex);
+		pathsFree.push_back(_r);
+	}
+	PathsLease(const PathsLease &) = delete;
+	PathsLease & operator=(const PathsLease &) = delete;
+	RenderPaths * operator->() const { return _r; }
+	RenderPaths & operator*() const { return *_r; }
+
+private:
+	RenderPaths * _r;
+	hipStream_t _stream;
+};
+} // namespace
+
+} // namespace rdu
 
 struct rdu_scene {
 	struct Tex {
@@ -31,10 +104,6 @@ struct rdu_scene {
 	rdu_raycaster * rc = nullptr;
 	rdu::DevScene dev;
 	rdu::DevBuffer dTexels, dTextures, dNrm, dTan, dBit, dUv, dIdx, dMeshes, dObjects, dMaterials, dLights;
-	// Path buffers, grown on demand.
-	rdu::DevBuffer pRays, pHits, pAtten, pColor, pQ0, pQ1, pCounts, pTotals, pSRays, pSContrib, pSQ0, pSQ1, pOcc,
-		pSHits, pGstack, pImage;
-	size_t pathCap = 0;
 	// Radiance-cache mode buffers, grown on demand.
 	rdu::DevBuffer cQuery, cQAtten, cQPath, cPred, cRec, cRecPath, cTrainQ, cTrainT, cHoldQ, cHoldT, cCounts;
 	size_t cacheCap = 0;
@@ -104,22 +173,24 @@ int uploadVec(DevBuffer & b, const std::vector<T> & v) {
 	return b.upload(v.data(), v.size() * sizeof(T));
 }
 
-int ensurePaths(rdu_scene * s, size_t P) {
-	if(P <= s->pathCap)
-		return RDU_OK;
+// Sizes a leased set for passes of P paths over this scene's tree (the spill stack depends on the
+// tree's depth, so it is checked on every render).
+int ensurePaths(RenderPaths & R, const rdu_scene * s, size_t P) {
 	int st;
-	if((st = s->pRays.reserve(P * 32)) || (st = s->pHits.reserve(P * sizeof(rdu_hit))) || (st = s->pAtten.reserve(P * 16)) ||
-		(st = s->pColor.reserve(P * 16)) || (st = s->pQ0.reserve(P * 4)) || (st = s->pQ1.reserve(P * 4)) ||
-		(st = s->pCounts.reserve(kCountSlots * 4)) || (st = s->pTotals.reserve(2 * 8)) || (st = s->pSRays.reserve(P * 32)) ||
-		(st = s->pSContrib.reserve(P * 16)) || (st = s->pSQ0.reserve(P * 4)) || (st = s->pSQ1.reserve(P * 4)) ||
-		(st = s->pOcc.reserve(P)) || (st = s->pSHits.reserve(P * sizeof(rdu_hit))))
-		return st;
+	if(P > R.pathCap) {
+		if((st = R.pRays.reserve(P * 32)) || (st = R.pHits.reserve(P * sizeof(rdu_hit))) || (st = R.pAtten.reserve(P * 16)) ||
+			(st = R.pColor.reserve(P * 16)) || (st = R.pQ0.reserve(P * 4)) || (st = R.pQ1.reserve(P * 4)) ||
+			(st = R.pCounts.reserve(kCountSlots * 4)) || (st = R.pTotals.reserve(2 * 8)) || (st = R.pSRays.reserve(P * 32)) ||
+			(st = R.pSContrib.reserve(P * 16)) || (st = R.pSQ0.reserve(P * 4)) || (st = R.pSQ1.reserve(P * 4)) ||
+			(st = R.pOcc.reserve(P)) || (st = R.pSHits.reserve(P * sizeof(rdu_hit))))
+			return st;
+		R.pathCap = P;
+	}
 	TraceOptions o;
 	o.wide = true;
 	const size_t g = globalStackInts(s->rc->dev, o, P);
-	if(g && (st = s->pGstack.reserve(g * sizeof(int))))
+	if(g && (st = R.pGstack.reserve(g * sizeof(int))))
 		return st;
-	s->pathCap = P;
 	return RDU_OK;
 }
 
@@ -456,36 +527,37 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 	int st = setDevice(s->rc->device);
 	if(st)
 		return st;
-	if((st = ensurePaths(s, pixels * perPass)))
-		return st;
-	const size_t imgBytes = pixels * components * sizeof(float);
-	if((st = s->pImage.reserve(imgBytes)))
-		return st;
 	// A stream per scene: renders of different scenes from different host threads overlap on the GPU.
 	if(!s->stream && (st = hipCheck(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate")))
 		return st;
 	hipStream_t stream = s->stream;
-	if(!tuneOut && (st = hipCheck(hipMemcpyAsync(s->pImage.ptr, image, imgBytes, hipMemcpyHostToDevice, stream), "image upload")))
+	PathsLease R(s->rc->device, pixels * perPass, stream);
+	if((st = ensurePaths(*R, s, pixels * perPass)))
 		return st;
-	if((st = hipCheck(hipMemsetAsync(s->pTotals.ptr, 0, 16, stream), "hipMemsetAsync")))
+	const size_t imgBytes = pixels * components * sizeof(float);
+	if((st = R->pImage.reserve(imgBytes)))
+		return st;
+	if(!tuneOut && (st = hipCheck(hipMemcpyAsync(R->pImage.ptr, image, imgBytes, hipMemcpyHostToDevice, stream), "image upload")))
+		return st;
+	if((st = hipCheck(hipMemsetAsync(R->pTotals.ptr, 0, 16, stream), "hipMemsetAsync")))
 		return st;
 	PtPaths P;
-	P.rays = static_cast<float4 *>(s->pRays.ptr);
-	P.hits = static_cast<rdu_hit *>(s->pHits.ptr);
-	P.atten = static_cast<float4 *>(s->pAtten.ptr);
-	P.color = static_cast<float4 *>(s->pColor.ptr);
-	P.queue[0] = static_cast<uint32_t *>(s->pQ0.ptr);
-	P.queue[1] = static_cast<uint32_t *>(s->pQ1.ptr);
-	P.counts = static_cast<uint32_t *>(s->pCounts.ptr);
-	P.totals = static_cast<unsigned long long *>(s->pTotals.ptr);
-	P.shadowRays = static_cast<float4 *>(s->pSRays.ptr);
-	P.shadowContrib = static_cast<float4 *>(s->pSContrib.ptr);
-	P.shadowQueue[0] = static_cast<uint32_t *>(s->pSQ0.ptr);
-	P.shadowQueue[1] = static_cast<uint32_t *>(s->pSQ1.ptr);
-	P.occluded = static_cast<uint8_t *>(s->pOcc.ptr);
-	P.shadowHits = static_cast<rdu_hit *>(s->pSHits.ptr);
+	P.rays = static_cast<float4 *>(R->pRays.ptr);
+	P.hits = static_cast<rdu_hit *>(R->pHits.ptr);
+	P.atten = static_cast<float4 *>(R->pAtten.ptr);
+	P.color = static_cast<float4 *>(R->pColor.ptr);
+	P.queue[0] = static_cast<uint32_t *>(R->pQ0.ptr);
+	P.queue[1] = static_cast<uint32_t *>(R->pQ1.ptr);
+	P.counts = static_cast<uint32_t *>(R->pCounts.ptr);
+	P.totals = static_cast<unsigned long long *>(R->pTotals.ptr);
+	P.shadowRays = static_cast<float4 *>(R->pSRays.ptr);
+	P.shadowContrib = static_cast<float4 *>(R->pSContrib.ptr);
+	P.shadowQueue[0] = static_cast<uint32_t *>(R->pSQ0.ptr);
+	P.shadowQueue[1] = static_cast<uint32_t *>(R->pSQ1.ptr);
+	P.occluded = static_cast<uint8_t *>(R->pOcc.ptr);
+	P.shadowHits = static_cast<rdu_hit *>(R->pSHits.ptr);
 	PtScratch scratch;
-	scratch.gstack = static_cast<int *>(s->pGstack.ptr);
+	scratch.gstack = static_cast<int *>(R->pGstack.ptr);
 	// Radiance-cache mode set-up.
 	PtCache C;
 	rdu_cache_opts co{};
@@ -530,7 +602,7 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 		C.cells = rcacheCells(cache);
 	}
 	const size_t passPaths = pixels * perPass;
-	float * dImage = static_cast<float *>(s->pImage.ptr);
+	float * dImage = static_cast<float *>(R->pImage.ptr);
 	if((st = hipCheck(hipError_t(ptSetSampleBase(P.counts, 0, opt.seed, stream)), "sample base")))
 		return st;
 	if(tuneOut) {
@@ -654,14 +726,14 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 		}
 	}
 	s->lastCache = cstats;
-	if((st = hipCheck(hipError_t(ptNormalizeImage(static_cast<float *>(s->pImage.ptr), uint32_t(pixels), components,
+	if((st = hipCheck(hipError_t(ptNormalizeImage(static_cast<float *>(R->pImage.ptr), uint32_t(pixels), components,
 							  uint32_t(spp), stream)),
 			"normalise")))
 		return st;
-	if((st = hipCheck(hipMemcpyAsync(image, s->pImage.ptr, imgBytes, hipMemcpyDeviceToHost, stream), "image download")))
+	if((st = hipCheck(hipMemcpyAsync(image, R->pImage.ptr, imgBytes, hipMemcpyDeviceToHost, stream), "image download")))
 		return st;
 	unsigned long long totals[2] = {0, 0};
-	if((st = hipCheck(hipMemcpyAsync(totals, s->pTotals.ptr, 16, hipMemcpyDeviceToHost, stream), "stats download")) ||
+	if((st = hipCheck(hipMemcpyAsync(totals, R->pTotals.ptr, 16, hipMemcpyDeviceToHost, stream), "stats download")) ||
 		(st = hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize")))
 		return st;
 	const auto t1 = std::chrono::steady_clock::now();

@@ -112,3 +112,35 @@ def test_small_job_matches_oracle():
     d = np.abs(got.astype(np.float64) - exp)
     assert (d <= 1e-4 * np.abs(exp) + 1e-6).mean() >= 0.98
     assert d.mean() <= 1e-3 * max(np.abs(exp).mean(), 1e-6)
+
+
+@pytest.mark.gpu
+def test_render_working_sets_move_between_scenes():
+    """A render's working memory comes from a pool shared by all scenes: a scene rendered after
+    others (smaller and larger images, a deeper or shallower tree, graph replay on) gets the same
+    image as the first time."""
+    from conftest import log_case
+    from rendu_amd.pathtracer import PathTracer
+    log_case("two scenes alternating renders over shared working sets, one with graph replay")
+    scn_a, cam_a, _, _, _, depth_a = scenes.random_job(11)
+    scn_b, cam_b = scenes.cornell_spheres(seed=3, subdivisions=2)
+    a, b = PathTracer(scn_a), PathTracer(scn_b)
+    b.set_graph(True)
+
+    def render(pt, cam, w, h, spp, depth, seed):
+        img = np.zeros((h, w, 3), np.float32)
+        pt.render(cam(w, h), spp, depth, img, seed=seed, max_paths=w * h * 2)
+        return img
+
+    a1 = render(a, cam_a, 96, 96, 8, depth_a, 1)
+    b1 = render(b, cam_b, 40, 40, 8, 5, 2)
+    a2 = render(a, cam_a, 96, 96, 8, depth_a, 1)
+    b2 = render(b, cam_b, 40, 40, 8, 5, 2)
+    b3 = render(b, cam_b, 40, 40, 8, 5, 2)
+    a3 = render(a, cam_a, 160, 160, 4, depth_a, 1)  # a larger pass: the shared set grows
+    b4 = render(b, cam_b, 40, 40, 8, 5, 2)
+    np.testing.assert_array_equal(a1, a2)
+    for img in (b2, b3, b4):
+        np.testing.assert_array_equal(img, b1)
+    assert a3.mean() > 0
+    assert b.last_stats()["graph_passes"] > 0
