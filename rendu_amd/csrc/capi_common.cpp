@@ -72,14 +72,21 @@ DevBuffer::~DevBuffer() { release(); }
 
 void DevBuffer::release() {
 	if(ptr) {
-		// As hipFree would: wait until no stream can still use the block, then return it (to the
-		// pool, where the next allocation of any stream may take it at once).
+		// As hipFree would: wait until no stream of the block's device can still use it, then
+		// return it (to that device's pool, where the next allocation may take it at once).
+		int current = -1;
+		(void)hipGetDevice(&current);
+		if(device >= 0 && device != current)
+			(void)hipSetDevice(device);
 		(void)hipDeviceSynchronize();
 		(void)hipFreeAsync(ptr, nullptr);
 		(void)hipStreamSynchronize(nullptr);
+		if(device >= 0 && device != current && current >= 0)
+			(void)hipSetDevice(current);
 	}
 	ptr = nullptr;
 	bytes = 0;
+	device = -1;
 }
 
 int DevBuffer::reserve(size_t n) {
@@ -88,8 +95,8 @@ int DevBuffer::reserve(size_t n) {
 	release();
 	if(allocFaultInjected())
 		return fail(RDU_ERR_ALLOC, "device allocation failed (RDU_FAIL_ALLOC=1)");
-	int device = 0, st;
-	if((st = hipCheck(hipGetDevice(&device), "hipGetDevice")) || (st = poolReady(device)))
+	int dev = 0, st;
+	if((st = hipCheck(hipGetDevice(&dev), "hipGetDevice")) || (st = poolReady(dev)))
 		return st;
 	const size_t want = n < 256 ? 256 : n;
 	// Allocated in the null stream's order, which is complete once that stream has drained: the
@@ -104,6 +111,7 @@ int DevBuffer::reserve(size_t n) {
 		return hipCheck(e, "hipMallocAsync");
 	}
 	bytes = want;
+	device = dev;
 	return RDU_OK;
 }
 

@@ -18,6 +18,7 @@ int currentDevice(int * device);
 struct DevBuffer {
 	void * ptr = nullptr;
 	size_t bytes = 0;
+	int device = -1; // device of `ptr` (the current device when it was allocated)
 	DevBuffer() = default;
 	DevBuffer(const DevBuffer &) = delete;
 	DevBuffer & operator=(const DevBuffer &) = delete;
