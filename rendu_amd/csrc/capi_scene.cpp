@@ -28,7 +28,9 @@ struct RenderPaths {
 
 namespace {
 std::mutex pathsMutex;
-std::vector<std::unique_ptr<RenderPaths>> pathsAll; // every set ever made (process lifetime)
+// Every set ever made. Never destroyed: the sets hold device memory until the process ends, and
+// freeing it from a static destructor could run after the HIP runtime has shut down.
+std::vector<std::unique_ptr<RenderPaths>> & pathsAll = *new std::vector<std::unique_ptr<RenderPaths>>();
 std::vector<RenderPaths *> pathsFree;
 
 // A free set of `device` for passes of P paths: the smallest one that already holds P paths,
