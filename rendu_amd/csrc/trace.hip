@@ -366,8 +366,8 @@ __global__ __launch_bounds__(kBlock) void traceKernel(DevBVH bvh, const float4 *
 }
 
 // ALT selects the kernel's variant: for closest hit the near/far plane fetch, for any hit the
-// unsorted visiting order.
-template<bool ANY, bool SPILL, bool STATS, bool ALT>
+// unsorted visiting order; NFA selects the near/far plane fetch for (sorted) any hit.
+template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA = false>
 __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
 	const uint32_t * __restrict__ perm, const uint32_t * __restrict__ countDev, rdu_hit * __restrict__ hits,
 	uint8_t * __restrict__ occ, int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint32_t chunk,
@@ -395,7 +395,7 @@ uint32_t refillMin(const TraceOptions & opt) {
 	return opt.spatialOrder ? 32u : 16u;
 }
 
-template<bool ANY, bool SPILL, bool STATS, bool ALT>
+template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA>
 PersistentGrid persistentGrid(size_t n, uint32_t blocksPerCuCap);
 
 template<bool ANY>
@@ -424,9 +424,9 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 	if(persistent) {
 		if(n > 0xffffffffull)
 			return int(hipErrorInvalidValue);
-#define RDU_PERSIST(S, ST, ALT)                                                                                  \
+#define RDU_PERSIST(S, ST, ALT, NFA)                                                                               \
 	{                                                                                                            \
-		const PersistentGrid g = persistentGrid<ANY, S, ST, ALT>(n, opt.blocksPerCu);                            \
+		const PersistentGrid g = persistentGrid<ANY, S, ST, ALT, NFA>(n, opt.blocksPerCu);                            \
 		const uint32_t waves = (g.blocks / kXcds) * (kBlock / 64);                                               \
 		const uint64_t slice = (uint64_t(n) + kXcds - 1) / kXcds;                                                \
 		/* sorted batches: interleaved 64-ray chunks; input order: one contiguous run per wavefront; */        \
@@ -435,24 +435,31 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 		chunk = chunk < 64 ? 64 : chunk;                                                                        \
 		if(opt.chunk >= 16 && opt.chunk <= (1u << 20))                                                          \
 			chunk = opt.chunk;                                                                                  \
-		hipLaunchKernelGGL((tracePersistent<ANY, S, ST, ALT>), dim3(g.blocks), dim3(kBlock), 0, stream, bvh, rays,  \
+		hipLaunchKernelGGL((tracePersistent<ANY, S, ST, ALT, NFA>), dim3(g.blocks), dim3(kBlock), 0, stream, bvh, rays,  \
 			uint32_t(n), perm, opt.countDev, hits, occ, gstack, lds, kXcds, refillMin(opt), uint32_t(chunk), stats); \
 	}
-		// Kernel variant: closest hit with the near/far plane fetch (fewer instructions per node,
-		// more registers: 5 waves per SIMD instead of 6; pays on incoherent rays), any hit without
-		// the near-first sort (pays on coherent rays).
+		// Kernel variant: the near/far plane fetch (fewer instructions per node, more registers:
+		// 5 waves per SIMD instead of 6 for closest hit and of 7 for any hit; pays on incoherent rays), or
+		// for any hit on coherent rays the unsorted visiting order.
 		const bool alt = ANY ? opt.unorderedAny : opt.nearFar;
+		const bool nfa = ANY && !opt.unorderedAny && opt.nearFar;
 		if(alt) {
 			if(stats) {
-				if(spill) RDU_PERSIST(true, true, true) else RDU_PERSIST(false, true, true)
+				if(spill) RDU_PERSIST(true, true, true, false) else RDU_PERSIST(false, true, true, false)
 			} else {
-				if(spill) RDU_PERSIST(true, false, true) else RDU_PERSIST(false, false, true)
+				if(spill) RDU_PERSIST(true, false, true, false) else RDU_PERSIST(false, false, true, false)
+			}
+		} else if(nfa) {
+			if(stats) {
+				if(spill) RDU_PERSIST(true, true, false, true) else RDU_PERSIST(false, true, false, true)
+			} else {
+				if(spill) RDU_PERSIST(true, false, false, true) else RDU_PERSIST(false, false, false, true)
 			}
 		} else {
 			if(stats) {
-				if(spill) RDU_PERSIST(true, true, false) else RDU_PERSIST(false, true, false)
+				if(spill) RDU_PERSIST(true, true, false, false) else RDU_PERSIST(false, true, false, false)
 			} else {
-				if(spill) RDU_PERSIST(true, false, false) else RDU_PERSIST(false, false, false)
+				if(spill) RDU_PERSIST(true, false, false, false) else RDU_PERSIST(false, false, false, false)
 			}
 		}
 #undef RDU_PERSIST
@@ -503,12 +510,12 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 // so never above the true entry): a popped child whose entry already exceeds the current maxi
 // is dropped without being fetched, exactly as its own box test would drop it (the test fails
 // once entry > maxi). Levels spilled to global memory carry no entry and are always fetched.
-template<bool ANY, bool SPILL, bool STATS, bool ALT>
+template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA>
 __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
 	const uint32_t * __restrict__ perm, const uint32_t * __restrict__ countDev, rdu_hit * __restrict__ hits,
 	uint8_t * __restrict__ occ, int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint32_t chunk,
 	uint64_t * stats) {
-	constexpr bool NF = ALT && !ANY;      // near/far plane fetch
+	constexpr bool NF = (ALT && !ANY) || (ANY && NFA && !ALT); // near/far plane fetch
 	constexpr bool UNSORTED = ALT && ANY; // children in node order
 	__shared__ int ls[kWideLds][kBlock];
 	__shared__ _Float16 le[ANY ? 1 : kWideLds][kBlock];
@@ -777,7 +784,7 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 	}
 }
 
-template<bool ANY, bool SPILL, bool STATS, bool ALT>
+template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA>
 PersistentGrid persistentGrid(size_t n, uint32_t blocksPerCuCap) {
 	struct Occupancy {
 		int blocksPerCu = 1, cus = 1;
@@ -788,7 +795,7 @@ PersistentGrid persistentGrid(size_t n, uint32_t blocksPerCuCap) {
 		int dev = 0;
 		(void)hipGetDevice(&dev);
 		(void)hipDeviceGetAttribute(&o.cus, hipDeviceAttributeMultiprocessorCount, dev);
-		(void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocksPerCu, tracePersistent<ANY, SPILL, STATS, ALT>, kBlock, 0);
+		(void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocksPerCu, tracePersistent<ANY, SPILL, STATS, ALT, NFA>, kBlock, 0);
 		o.blocksPerCu = o.blocksPerCu < 1 ? 1 : o.blocksPerCu;
 		o.cus = o.cus < 1 ? 1 : o.cus;
 		return o;

@@ -80,8 +80,9 @@ struct TraceOptions {
 	uint32_t refillMin = 0;    // idle lanes before a wavefront refills (0 = default for the order)
 	uint32_t chunk = 0;        // rays per interleaved run of a persistent wavefront (0 = default for the order)
 	uint32_t blocksPerCu = 0;  // resident workgroups per CU of the persistent grid (0 = occupancy limit)
-	bool nearFar = false;      // closest hit: fetch node planes ordered near/far by the ray's direction signs
-	                           // (fewer instructions per node, lower occupancy: for incoherent rays)
+	bool nearFar = false;      // fetch node planes ordered near/far by the ray's direction signs (fewer
+	                           // instructions per node, lower occupancy: for incoherent rays); any hit
+	                           // uses it when unorderedAny is off
 	bool unorderedAny = false; // any hit: visit the children that pass in node order, no near-first sort
 	                           // (for coherent rays)
 };
