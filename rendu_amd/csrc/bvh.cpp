@@ -228,6 +228,30 @@ void HostScene::build() {
 	built = true;
 }
 
+namespace {
+// body(begin, end) over [0, n): on up to 16 threads (contiguous chunks) for large n, else inline.
+// Each index is written by exactly one chunk, so the result does not depend on the threads.
+template<typename F>
+void parallelFor(size_t n, const F & body) {
+	const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+	const size_t workers = n < kParallelMin ? 1 : std::min<size_t>(hw, 16);
+	if(workers <= 1) {
+		body(size_t(0), n);
+		return;
+	}
+	const size_t step = (n + workers - 1) / workers;
+	std::vector<std::thread> pool;
+	for(size_t w = 1; w < workers; ++w) {
+		const size_t b = w * step, e = std::min(n, b + step);
+		if(b < e)
+			pool.emplace_back([&body, b, e] { body(b, e); });
+	}
+	body(size_t(0), std::min(n, step));
+	for(std::thread & t : pool)
+		t.join();
+}
+} // namespace
+
 void HostScene::flatten(DeviceLayout & out) const {
 	// Depth-first preorder numbering of internal nodes (left subtree first) so that a parent and
 	// its near subtree share cache lines as traversal descends.
@@ -258,35 +282,45 @@ void HostScene::flatten(DeviceLayout & out) const {
 	out.internalCount = uint32_t(order.size());
 	out.maxDepth = maxDepth;
 	out.nodes.assign(order.size() * 16, 0.0f);
-	for(size_t i = 0; i < order.size(); ++i) {
-		const HNode & n = nodes[order[i]];
-		const HBox & L = nodes[n.left].box;
-		const HBox & R = nodes[n.right].box;
-		float * o = out.nodes.data() + 16 * i;
-		const float v[12] = {L.mn.x, L.mn.y, L.mn.z, L.mx.x, L.mx.y, L.mx.z, R.mn.x, R.mn.y, R.mn.z, R.mx.x, R.mx.y, R.mx.z};
-		std::memcpy(o, v, sizeof(v));
-		o[12] = fromBits(uint32_t(code(n.left)));
-		o[13] = fromBits(uint32_t(code(n.right)));
-	}
+	parallelFor(order.size(), [&](size_t begin, size_t end) {
+		for(size_t i = begin; i < end; ++i) {
+			const HNode & n = nodes[order[i]];
+			const HBox & L = nodes[n.left].box;
+			const HBox & R = nodes[n.right].box;
+			float * o = out.nodes.data() + 16 * i;
+			const float v[12] = {L.mn.x, L.mn.y, L.mn.z, L.mx.x, L.mx.y, L.mx.z, R.mn.x, R.mn.y, R.mn.z, R.mx.x, R.mx.y, R.mx.z};
+			std::memcpy(o, v, sizeof(v));
+			o[12] = fromBits(uint32_t(code(n.left)));
+			o[13] = fromBits(uint32_t(code(n.right)));
+		}
+	});
 	out.tris.assign(tris.size() * 12, 0.0f);
-	for(size_t i = 0; i < tris.size(); ++i) {
-		const HTri & t = tris[i];
-		const f3 v0 = vertices[t.v0];
-		const f3 e1 = vertices[t.v1] - v0; // Raycaster.cpp:235
-		const f3 e2 = vertices[t.v2] - v0; // Raycaster.cpp:236
-		float * o = out.tris.data() + 12 * i;
-		const float v[9] = {v0.x, v0.y, v0.z, e1.x, e1.y, e1.z, e2.x, e2.y, e2.z};
-		std::memcpy(o, v, sizeof(v));
-		o[9] = fromBits(t.meshId);
-		o[10] = fromBits(t.localId);
-	}
+	parallelFor(tris.size(), [&](size_t begin, size_t end) {
+		for(size_t i = begin; i < end; ++i) {
+			const HTri & t = tris[i];
+			const f3 v0 = vertices[t.v0];
+			const f3 e1 = vertices[t.v1] - v0; // Raycaster.cpp:235
+			const f3 e2 = vertices[t.v2] - v0; // Raycaster.cpp:236
+			float * o = out.tris.data() + 12 * i;
+			const float v[9] = {v0.x, v0.y, v0.z, e1.x, e1.y, e1.z, e2.x, e2.y, e2.z};
+			std::memcpy(o, v, sizeof(v));
+			o[9] = fromBits(t.meshId);
+			o[10] = fromBits(t.localId);
+		}
+	});
 	// Top level over the mesh roots (near-first queries only): Rendu tests every root box in turn
 	// (Raycaster.cpp:149-155); here the roots are grouped under a binary tree of exact box merges,
 	// split at the median root centroid on the largest axis. A group's box contains its members'
 	// boxes, so it culls nothing that a member's own test would keep.
-	std::vector<HNode> ext(nodes);
 	// Near-first layout only: a 2-triangle leaf becomes a node over two 1-triangle leaves with the
-	// triangles' own boxes (Bounds.cpp:12-15), so that the 4-wide node culls each triangle.
+	// triangles' own boxes (Bounds.cpp:12-15), so that the 4-wide node culls each triangle. Sized
+	// once for those leaves and the top level, so the copy never reallocates.
+	size_t pairs = 0;
+	for(const HNode & n : nodes)
+		pairs += (n.leaf && n.right == 2) ? 1 : 0;
+	std::vector<HNode> ext;
+	ext.reserve(nodes.size() + 2 * pairs + 2 * size_t(meshCount) + 1);
+	ext.assign(nodes.begin(), nodes.end());
 	const size_t baseCount = ext.size();
 	for(size_t i = 0; i < baseCount; ++i) {
 		if(!ext[i].leaf || ext[i].right != 2)
