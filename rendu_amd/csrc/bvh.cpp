@@ -86,6 +86,12 @@ void HostScene::addMesh(const float * xyz, size_t nv, const uint32_t * idx, size
 	++meshCount;
 }
 
+void HostScene::reserve(size_t nVertices, size_t nTriangles, size_t nMeshes) {
+	vertices.reserve(vertices.size() + nVertices);
+	tris.reserve(tris.size() + nTriangles);
+	nodes.reserve(nodes.size() + nMeshes);
+}
+
 namespace {
 
 // One iteration of Raycaster::updateHierarchy's loop (Raycaster.cpp:54-142) on a node covering

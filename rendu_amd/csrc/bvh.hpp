@@ -72,6 +72,9 @@ public:
 	// Raycaster.cpp:54-142.
 	void build();
 	void flatten(DeviceLayout & out) const;
+	// Sizes the stores for meshes still to come (vertices, triangles, meshes in total), so that
+	// adding many meshes does not regrow them mesh by mesh. Changes no content.
+	void reserve(size_t nVertices, size_t nTriangles, size_t nMeshes);
 };
 
 } // namespace rdu

@@ -402,7 +402,14 @@ int rdu_scene_commit(rdu_scene * s) {
 	int st = rdu_raycaster_create(&s->rc);
 	if(st)
 		return st;
-	// PathTracer::PathTracer: every object's mesh, with its model, in object order.
+	// PathTracer::PathTracer: every object's mesh, with its model, in object order (the raycaster's
+	// stores sized once for all of them).
+	size_t totalVerts = 0, totalTris = 0;
+	for(const rdu_object_desc & o : s->objects) {
+		totalVerts += s->meshes[o.mesh].nv;
+		totalTris += s->meshes[o.mesh].ni / 3;
+	}
+	s->rc->scene.reserve(totalVerts, totalTris, s->objects.size());
 	for(const rdu_object_desc & o : s->objects) {
 		const rdu_scene::MeshRec & m = s->meshes[o.mesh];
 		if((st = rdu_raycaster_add_mesh(s->rc, s->positions.data() + 3 * size_t(m.vtx), m.nv, s->indices.data() + m.idx, m.ni,
