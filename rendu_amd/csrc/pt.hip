@@ -519,6 +519,7 @@ int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, cons
 				TraceOptions so;
 				applyTuning(so, tn);
 				so.wide = true;
+				so.nearFar = true; // shadow rays leave scattered surface points: incoherent
 				so.perm = P.shadowQueue[0];
 				so.countDev = P.counts + kCountShadow;
 				e = launchAny(bvh, P.shadowRays, nPaths, P.occluded, so, scratch.gstack, nullptr, stream);
