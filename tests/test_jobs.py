@@ -144,3 +144,25 @@ def test_render_working_sets_move_between_scenes():
         np.testing.assert_array_equal(img, b1)
     assert a3.mean() > 0
     assert b.last_stats()["graph_passes"] > 0
+
+
+@pytest.mark.gpu
+def test_memory_is_reused_across_scenes():
+    """Scenes built, rendered and dropped one after another reuse the library's device memory
+    (render working sets and buffers come from pools): after the first cycles the device's free
+    memory stops falling."""
+    import torch
+    from conftest import log_case
+    from rendu_amd.pathtracer import PathTracer
+    log_case("12 create / render / destroy cycles, device free memory stable")
+    scn, cam, w, h, spp, depth = scenes.random_job(21)
+    free = []
+    for k in range(12):
+        pt = PathTracer(scn)
+        img = np.zeros((160, 160, 3), np.float32)
+        pt.render(cam(160, 160), 8, depth, img, seed=k)
+        del pt
+        torch.cuda.synchronize()
+        free.append(torch.cuda.mem_get_info()[0])
+    # Later cycles allocate nothing new from the device: free memory moves by less than 64 MiB.
+    assert max(free[3:]) - min(free[3:]) < 64 << 20, [f >> 20 for f in free]
