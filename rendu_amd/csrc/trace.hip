@@ -35,7 +35,7 @@ namespace rdu {
 
 namespace {
 
-constexpr int kBlock = 128;
+constexpr int kBlock = 256;
 constexpr int kWideLds = 16;
 
 __device__ __forceinline__ f3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
