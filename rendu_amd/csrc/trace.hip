@@ -35,7 +35,9 @@ namespace rdu {
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;       // persistent kernels (4 wavefronts per workgroup)
+constexpr int kSimpleBlock = 128; // one-ray-per-lane kernel
+static_assert(kBlock % kSimpleBlock == 0, "spill-stack slots are sized for the persistent grid");
 constexpr int kWideLds = 16;
 
 __device__ __forceinline__ f3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
@@ -89,8 +91,9 @@ __device__ __forceinline__ float4 ldGlobal(uintptr_t a) {
 	return make_float4(v.x, v.y, v.z, v.w);
 }
 
+template<int B>
 struct LaneStack {
-	int (*ls)[kBlock];
+	int (*ls)[B];
 	int * gs;
 	size_t gstride, slot;
 	int lds;
@@ -159,7 +162,7 @@ __device__ __forceinline__ bool testLeaf(const DevBVH & bvh, const DevRay & r, i
 // Binary tree, reference or near-first order, from one root code.
 template<bool ANY, bool REF, bool SPILL, bool EXACT, bool STATS>
 __device__ __forceinline__ void traverseBinary(const DevBVH & bvh, const DevRay & r, int c, float mini,
-	LaneStack & st, TraceState & s) {
+	LaneStack<kSimpleBlock> & st, TraceState & s) {
 	st.sp = 0;
 	while(true) {
 		if(c >= 0) {
@@ -222,7 +225,7 @@ __device__ __forceinline__ void cswap(float & a, int & ca, float & b, int & cb) 
 
 // 4-wide tree, near-first order, from one root code.
 template<bool ANY, bool SPILL, bool EXACT, bool STATS>
-__device__ __forceinline__ void traverseWide(const DevBVH & bvh, const DevRay & r, int c, float mini, LaneStack & st,
+__device__ __forceinline__ void traverseWide(const DevBVH & bvh, const DevRay & r, int c, float mini, LaneStack<kSimpleBlock> & st,
 	TraceState & s) {
 	st.sp = 0;
 	while(true) {
@@ -275,7 +278,7 @@ __device__ __forceinline__ void traverseWide(const DevBVH & bvh, const DevRay & 
 }
 
 template<bool ANY, bool REF, bool WIDE, bool SPILL, bool EXACT, bool STATS>
-__device__ __forceinline__ void traceRay(const DevBVH & bvh, const DevRay & r, LaneStack & st, TraceState & s) {
+__device__ __forceinline__ void traceRay(const DevBVH & bvh, const DevRay & r, LaneStack<kSimpleBlock> & st, TraceState & s) {
 	const float mini = r.mini;
 	const float maxi0 = r.maxi;
 	// Rendu pushes every root whose box passes (tested with the query's maxi), then pops the last
@@ -305,12 +308,12 @@ __device__ __forceinline__ bool finite3(f3 v) {
 }
 
 template<bool ANY, bool REF, bool WIDE, bool SPILL, bool STATS>
-__global__ __launch_bounds__(kBlock) void traceKernel(DevBVH bvh, const float4 * __restrict__ rays, size_t n,
+__global__ __launch_bounds__(kSimpleBlock) void traceKernel(DevBVH bvh, const float4 * __restrict__ rays, size_t n,
 	const uint32_t * __restrict__ perm, const uint32_t * __restrict__ countDev, rdu_hit * __restrict__ hits,
 	uint8_t * __restrict__ occ, int * __restrict__ gstack, int lds, uint64_t * stats) {
-	__shared__ int ls[WIDE ? kWideLds : kLdsStack][kBlock];
-	const size_t gid = size_t(blockIdx.x) * kBlock + threadIdx.x;
-	LaneStack st{ls, gstack, size_t(gridDim.x) * kBlock, gid, lds};
+	__shared__ int ls[WIDE ? kWideLds : kLdsStack][kSimpleBlock];
+	const size_t gid = size_t(blockIdx.x) * kSimpleBlock + threadIdx.x;
+	LaneStack<kSimpleBlock> st{ls, gstack, size_t(gridDim.x) * kSimpleBlock, gid, lds};
 	TraceState s;
 	if(countDev)
 		n = size_t(*countDev) < n ? size_t(*countDev) : n;
@@ -408,8 +411,8 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 	TraceOptions o = opt;
 	o.wide = wide;
 	const bool spill = stackNeed(bvh, o) > lds;
-	const dim3 grid(unsigned((n + kBlock - 1) / kBlock));
-	const dim3 block(kBlock);
+	const dim3 grid(unsigned((n + kSimpleBlock - 1) / kSimpleBlock));
+	const dim3 block(kSimpleBlock);
 	const uint32_t * perm = opt.perm;
 #define RDU_LAUNCH(R, W, S, ST)                                                                               \
 	hipLaunchKernelGGL((traceKernel<ANY, R, W, S, ST>), grid, block, 0, stream, bvh, rays, n, perm, opt.countDev, hits, \
@@ -528,7 +531,7 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 	const uint64_t sliceBegin = uint64_t(n) * die / xcds;
 	const uint64_t sliceEnd = uint64_t(n) * (die + 1) / xcds;
 	uint64_t cursor = 0; // position in this wavefront's sequence (wave-uniform)
-	LaneStack st{ls, gstack, size_t(gridDim.x) * kBlock, gtid, lds};
+	LaneStack<kBlock> st{ls, gstack, size_t(gridDim.x) * kBlock, gtid, lds};
 	uint64_t nodeVisits = 0, triTests = 0;
 
 	bool has = false, pend = false;
@@ -869,7 +872,8 @@ size_t globalStackInts(const DevBVH & bvh, const TraceOptions & opt, size_t n) {
 	if(need <= 0)
 		return 0;
 	// One slot per lane of the widest grid a launch of n rays uses: the persistent grid rounds
-	// its block count up to a whole number per XCD, so it can exceed ceil(n / kBlock).
+	// its block count up to a whole number per XCD, so it can exceed ceil(n / kBlock); the
+	// one-ray-per-lane grid, ceil(n / kSimpleBlock) blocks of kSimpleBlock, never exceeds that.
 	const size_t blocks = (n + kBlock - 1) / kBlock;
 	const size_t slots = ((blocks + kXcds - 1) / kXcds) * kXcds * kBlock;
 	return size_t(need) * slots;
