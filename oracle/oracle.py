@@ -180,6 +180,11 @@ _pt_sig = {
     "orc_sample_disk": (None, [c_float, c_float, POINTER(c_float)]),
     "orc_texture_probe": (None, [c_void_p, c_int, c_int, c_float, c_float, c_float, POINTER(c_float)]),
     "orc_philox": (None, [ctypes.c_uint32] * 6 + [POINTER(c_uint32)]),
+    "orc_light_sample": (None, [c_void_p, POINTER(c_float), POINTER(c_float), POINTER(c_float), POINTER(c_float)]),
+    "orc_sky_eval": (None, [c_void_p, c_int] + [POINTER(c_float)] * 4),
+    "orc_sky_phase": (None, [c_float, POINTER(c_float), POINTER(c_float)]),
+    "orc_local_frame": (None, [c_void_p, ctypes.c_uint32, ctypes.c_uint32, c_float, c_float, POINTER(c_float),
+                               POINTER(c_float)]),
 }
 for _name, (_res, _args) in _pt_sig.items():
     _fn = getattr(lib, _name)
@@ -216,6 +221,22 @@ def ggx_pdf(wo, base, rough, metal, wi):
     p = np.zeros(1, np.float32)
     lib.orc_ggx_pdf(_f(_v3(wo)), _f(_v3(base)), rough, metal, _f(_v3(wi)), _f(p))
     return float(p[0])
+
+
+def light_sample(light_desc, position):
+    """Light::sample (Point/Spot/DirectionalLight.cpp) at `position`: (direction, dist, attenuation)."""
+    d = np.zeros(3, np.float32)
+    dist = np.zeros(1, np.float32)
+    att = np.zeros(1, np.float32)
+    lib.orc_light_sample(ctypes.byref(light_desc), _f(_v3(position)), _f(d), _f(dist), _f(att))
+    return d, float(dist[0]), float(att[0])
+
+
+def sky_phase(cos_angle):
+    """MaterialSky::rayleighPhase, miePhase (MaterialSky.cpp:89-98)."""
+    r, m = np.zeros(1, np.float32), np.zeros(1, np.float32)
+    lib.orc_sky_phase(cos_angle, _f(r), _f(m))
+    return float(r[0]), float(m[0])
 
 
 def sample_disk(u0, u1):
@@ -259,6 +280,18 @@ class OraclePathTracer:
         if spp < 0:
             raise RuntimeError("oracle render refused its arguments")
         return spp, int(cnt[0]), int(cnt[1])
+
+    def sky(self, tex, origin, direction, sun):
+        """MaterialSky::eval with texture `tex` as the 512x512 scattering table."""
+        out = np.zeros(3, np.float32)
+        lib.orc_sky_eval(self.h, tex, _f(_v3(origin)), _f(_v3(direction)), _f(_v3(sun)), _f(out))
+        return out
+
+    def local_frame(self, obj, local_id, u, v, ray_dir):
+        """buildLocalFrame at a hit on object `obj`: 3x3 array whose rows are t, b, n."""
+        out = np.zeros(9, np.float32)
+        lib.orc_local_frame(self.h, obj, local_id, u, v, _f(_v3(ray_dir)), _f(out))
+        return out.reshape(3, 3)
 
     def probe(self, tex, mode, x, y, z=0.0):
         out = np.zeros(4, np.float32)

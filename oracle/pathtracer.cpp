@@ -272,12 +272,25 @@ bool sphereRoots(v3 o, v3 d, float radius, v2 & roots) {
 	return true;
 }
 
+// MaterialSky::rayleighPhase / miePhase (MaterialSky.cpp:89-98), gMie = 0.758 (Sky.hpp:53-65).
+float skyRayleighPhase(float cosAngle) {
+	const float k = 1.0f / (4.0f * kPi);
+	return k * 3.0f / 4.0f * (1.0f + cosAngle * cosAngle);
+}
+float skyMiePhase(float cosAngle) {
+	const float gMie = 0.758f;
+	const float k = 1.0f / (4.0f * kPi);
+	const float g2 = gMie * gMie;
+	return k * 3.0f * (1.0f - g2) / (2.0f * (2.0f + g2)) * (1.0f + cosAngle * cosAngle) /
+		   std::pow(1.0f + g2 - 2.0f * gMie * cosAngle, 3.0f / 2.0f);
+}
+
 // MaterialSky::eval (MaterialSky.cpp:6-98) with Sky::AtmosphereParameters (Sky.hpp:53-65).
 v3 skyEval(const Texture & table, v3 rayOrigin, v3 rayDir, v3 sunDir) {
 	const v3 sunColor(1.474f, 1.8504f, 1.91198f);
 	const v3 kRayleigh(5.5e-6f, 13.0e-6f, 22.4e-6f);
 	const float groundRadius = 6371e3f, topRadius = 6471e3f, sunIntensity = 20.0f, kMie = 21e-6f;
-	const float heightRayleigh = 8000.0f, heightMie = 1200.0f, gMie = 0.758f, sunRadius = 0.04675f;
+	const float heightRayleigh = 8000.0f, heightMie = 1200.0f, sunRadius = 0.04675f;
 	const float sunRadiusCos = 0.998f;
 	const unsigned samplesCount = 16;
 	const v3 planetPos = rayOrigin + v3(0.0f, groundRadius, 0.0f) + v3(0.0f, 1.0f, 0.0f);
@@ -313,13 +326,8 @@ v3 skyEval(const Texture & table, v3 rayOrigin, v3 rayDir, v3 sunDir) {
 		mieScatt = mieScatt + mieStep * attenuation;
 		transmittance = transmittance + directAttenuation;
 	}
-	const float k = 1.0f / (4.0f * kPi);
-	const float rayleighPhase = k * 3.0f / 4.0f * (1.0f + cosViewSun * cosViewSun);
-	const float g2 = gMie * gMie;
-	const float miePhase = k * 3.0f * (1.0f - g2) / (2.0f * (2.0f + g2)) * (1.0f + cosViewSun * cosViewSun) /
-						   std::pow(1.0f + g2 - 2.0f * gMie * cosViewSun, 3.0f / 2.0f);
-	const v3 rayleighPart = rayleighPhase * kRayleigh * rayleighScatt;
-	const v3 miePart = kMie * miePhase * mieScatt;
+	const v3 rayleighPart = skyRayleighPhase(cosViewSun) * kRayleigh * rayleighScatt;
+	const v3 miePart = kMie * skyMiePhase(cosViewSun) * mieScatt;
 	v3 sunRadiance(0.0f);
 	const bool didHitGroundForward = didHitGround && interGround.y > 0.0f;
 	if(!didHitGroundForward && dot(rayDir, sunDir) > sunRadiusCos)
@@ -742,6 +750,49 @@ void orc_texture_probe(void * h, int tex, int mode, float x, float y, float z, f
 	out[1] = c.y;
 	out[2] = c.z;
 	out[3] = 0.0f;
+}
+// Light::sample probe: returns the direction, dist and attenuation for a light at `position`.
+void orc_light_sample(const rdu_light_desc * L, const float * position, float * dir, float * dist, float * atten) {
+	float d = 0.0f, a = 0.0f;
+	const v3 r = lightSample(*L, v3(position[0], position[1], position[2]), d, a);
+	dir[0] = r.x;
+	dir[1] = r.y;
+	dir[2] = r.z;
+	*dist = d;
+	*atten = a;
+}
+// MaterialSky::eval probe with texture `tex` of scene h as the scattering table.
+void orc_sky_eval(void * h, int tex, const float * origin, const float * dir, const float * sun, float * out) {
+	const Scene * s = static_cast<const Scene *>(h);
+	const v3 r = skyEval(s->textures[size_t(tex)], v3(origin[0], origin[1], origin[2]), v3(dir[0], dir[1], dir[2]),
+		v3(sun[0], sun[1], sun[2]));
+	out[0] = r.x;
+	out[1] = r.y;
+	out[2] = r.z;
+}
+void orc_sky_phase(float cosAngle, float * rayleigh, float * mie) {
+	*rayleigh = skyRayleighPhase(cosAngle);
+	*mie = skyMiePhase(cosAngle);
+}
+// buildLocalFrame probe on object `object` of a committed scene, at hit (localId, u, v) of its
+// mesh; the texture coordinates are interpolated as render does. out: tbn columns t, b, n.
+void orc_local_frame(void * h, uint32_t object, uint32_t localId, float u, float v, const float * rayDir, float * out) {
+	const Scene * s = static_cast<const Scene *>(h);
+	const Object & obj = s->objects[object];
+	Hit hit;
+	hit.hit = true;
+	hit.dist = 1.0f;
+	hit.u = u;
+	hit.v = v;
+	hit.w = 1.0f - u - v;
+	hit.localId = localId;
+	hit.meshId = object;
+	const MeshData & mesh = s->meshes[obj.mesh];
+	const v2 uv = obj.useUV ? interp2(mesh.uv, mesh.idx, hit) : v2{0.5f, 0.5f};
+	const m3 tbn = buildLocalFrame(*s, obj, hit, v3(rayDir[0], rayDir[1], rayDir[2]), uv);
+	for(int c = 0; c < 3; ++c)
+		for(int r = 0; r < 3; ++r)
+			out[3 * c + r] = tbn.c[c][r];
 }
 void orc_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1, uint32_t * out) {
 	const u4 r = philox4x32_10({c0, c1, c2, c3}, k0, k1);

@@ -247,3 +247,70 @@ def furnace(subdivisions: int = 3, level: float = 0.5, roughness: float = 1.0, m
         return c
 
     return scn, cam
+
+
+def masked_layers(layers: int, light_radius: float = 100.0):
+    """Shadow-walk probe: a 400 x 400 floor at y = 0 seen from (0, 0.5, 0) straight down, a point
+    light at (0, 5, 0), `layers` alpha-0 cut-out quads (masked, alpha 0 everywhere) evenly spaced
+    between y = 1 and y = 4.5, and an opaque emissive-black quad at y = 6 above the light. Every
+    primary ray hits the floor; its light sample crosses every layer, so Rendu's shadow walk
+    (PathTracer.cpp:124-157) makes exactly layers + 1 visibility queries, and the light is seen
+    only if each step shortens the remaining distance (otherwise the last query reaches y = 6)."""
+    scn = S.Scene()
+    e = 200.0
+    m = scn.add_mesh(quad((-e, 0, e), (e, 0, e), (e, 0, -e), (-e, 0, -e)))
+    scn.objects.append(S.Object(m, _regular(scn, [0.6, 0.6, 0.6])))
+    t_cut = scn.add_texture(S.constant_texture([1.0, 1.0, 1.0, 0.0]))
+    t_flat = scn.add_texture(S.constant_texture([0.5, 0.5, 1.0, 1.0]))
+    t_rm = scn.add_texture(S.constant_texture([1.0, 0.0, 1.0, 1.0]))
+    cut = scn.add_material(S.Material(S.MATERIAL_REGULAR, (t_cut, t_flat, t_rm), True))
+    for k in range(layers):
+        y = 1.0 + 3.5 * (k + 0.5) / max(layers, 1)
+        m = scn.add_mesh(quad((-e, y, e), (e, y, e), (e, y, -e), (-e, y, -e)))
+        scn.objects.append(S.Object(m, cut, use_texcoords=True))
+    m = scn.add_mesh(quad((-2, 6, 2), (2, 6, 2), (2, 6, -2), (-2, 6, -2)))
+    scn.objects.append(S.Object(m, _emissive(scn, [0.0, 0.0, 0.0])))
+    scn.lights.append(S.Light(S.LIGHT_POINT, (4.0, 4.0, 4.0), (0.0, 5.0, 0.0), radius=light_radius))
+    scn.background = S.Background(S.BG_COLOR, (0.0, 0.0, 0.0))
+
+    def cam(width, height):
+        c = Camera()
+        c.pose([0.0, 0.5, 0.0], [0.0, 0.0, 0.0], [0.0, 0.0, 1.0])
+        c.projection(ratio=width / height, fov=1.0)
+        return c
+
+    return scn, cam
+
+
+def background_probe(mode: int, level: float = 1.0):
+    """A white rough floor (y = 0, 40 x 40) seen from (0, 1, 4) towards (0, 0.6, 0): the upper
+    rows see the background directly, the lower rows see only the floor, which is lit by nothing
+    but the background. `mode` is a Background mode; every mode shows the constant `level`
+    (IMAGE: a 64 x 64 image whose G channel is the column index / 64, R and B = level)."""
+    scn = S.Scene()
+    e = 20.0
+    m = scn.add_mesh(quad((-e, 0, e), (e, 0, e), (e, 0, -e), (-e, 0, -e)))
+    scn.objects.append(S.Object(m, _regular(scn, [1.0, 1.0, 1.0])))
+    c = [level, level, level, 1.0]
+    if mode == S.BG_COLOR:
+        scn.background = S.Background(S.BG_COLOR, (level, level, level))
+    elif mode == S.BG_IMAGE:
+        img = np.empty((64, 64, 4), F32)
+        img[...] = c
+        img[..., 1] = (np.arange(64) / 64.0).astype(F32)[None, :]
+        scn.background = S.Background(S.BG_IMAGE, texture=scn.add_texture(img))
+    elif mode == S.BG_SKYBOX:
+        faces = [scn.add_texture(S.constant_texture(c)) for _ in range(6)]
+        scn.background = S.Background(S.BG_SKYBOX, cube=tuple(faces))
+    elif mode == S.BG_LATLONG:
+        scn.background = S.Background(S.BG_LATLONG, texture=scn.add_texture(S.constant_texture(c)))
+    else:
+        raise ValueError("background_probe: COLOR, IMAGE, SKYBOX or LATLONG")
+
+    def cam(width, height):
+        cm = Camera()
+        cm.pose([0.0, 1.0, 4.0], [0.0, 0.6, 0.0], [0.0, 1.0, 0.0])
+        cm.projection(ratio=width / height, fov=0.9)
+        return cm
+
+    return scn, cam
