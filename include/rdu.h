@@ -269,7 +269,8 @@ int rdu_scene_last_render_stats(const rdu_scene* scene, rdu_render_stats* out);
  * and replayed for every later pass with the same configuration, within a render and across
  * renders of the scene (also with another seed: progressive renders replay). Each pass reads its
  * first sample index and the seed from device memory, so a replay renders exactly what the eager
- * pass would. Cached renders always run eagerly. Off by default. */
+ * pass would. Cached renders, and renders of scenes with masked (alpha cut-out) materials, whose
+ * shadow walk checks on the host whether rays still walk, always run eagerly. Off by default. */
 int rdu_scene_set_graph(rdu_scene* scene, int enable);
 
 

@@ -16,27 +16,36 @@ bool allocFaultInjected() {
 	return v && std::strcmp(v, "1") == 0;
 }
 
-// Library buffers come from the current device's default stream-ordered pool, whose release
-// threshold is raised once per device so that freed blocks stay in the pool for the next
-// allocation instead of going back to the driver. Every driver allocation is a kernel call that
-// serialises with the other threads' driver calls (and has been seen to take over a second), so
-// scenes built and dropped one after another, or from several threads, reuse memory without them.
-int poolReady(int device) {
+// The library's pool per device: hipMallocFromPoolAsync / hipFreeAsync with a release threshold at
+// its maximum, so freed blocks are reused without driver calls. Every driver allocation is a kernel
+// call that serialises with the other threads' driver calls (and has been seen to take over a
+// second), so scenes built and dropped one after another, or from several threads, reuse memory
+// without them. Pools live until the process ends.
+hipMemPool_t pools[64] = {};
+
+int libraryPool(int device, hipMemPool_t * out) {
 	static std::mutex m;
-	static uint64_t ready = 0;
 	if(device < 0 || device >= 64)
-		return RDU_OK; // no pool set-up: plain pool behaviour
+		return fail(RDU_ERR_UNSUPPORTED, "devices 0..63 only");
 	std::lock_guard<std::mutex> g(m);
-	if((ready >> device) & 1u)
-		return RDU_OK;
-	hipMemPool_t pool = nullptr;
-	int st = hipCheck(hipDeviceGetDefaultMemPool(&pool, device), "hipDeviceGetDefaultMemPool");
-	if(st)
-		return st;
-	uint64_t keep = UINT64_MAX;
-	if((st = hipCheck(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep), "hipMemPoolSetAttribute")))
-		return st;
-	ready |= uint64_t(1) << device;
+	if(!pools[device]) {
+		hipMemPoolProps props{};
+		props.allocType = hipMemAllocationTypePinned;
+		props.handleTypes = hipMemHandleTypeNone;
+		props.location.type = hipMemLocationTypeDevice;
+		props.location.id = device;
+		hipMemPool_t p = nullptr;
+		int st = hipCheck(hipMemPoolCreate(&p, &props), "hipMemPoolCreate");
+		if(st)
+			return st;
+		uint64_t keep = UINT64_MAX;
+		if((st = hipCheck(hipMemPoolSetAttribute(p, hipMemPoolAttrReleaseThreshold, &keep), "hipMemPoolSetAttribute"))) {
+			(void)hipMemPoolDestroy(p);
+			return st;
+		}
+		pools[device] = p;
+	}
+	*out = pools[device];
 	return RDU_OK;
 }
 } // namespace
@@ -68,12 +77,28 @@ int currentDevice(int * device) {
 	return hipCheck(hipGetDevice(device), "hipGetDevice");
 }
 
+int poolAlloc(void ** ptr, size_t bytes, hipStream_t stream) {
+	*ptr = nullptr;
+	if(allocFaultInjected())
+		return fail(RDU_ERR_ALLOC, "device allocation failed (RDU_FAIL_ALLOC=1)");
+	int dev = 0, st;
+	hipMemPool_t pool = nullptr;
+	if((st = hipCheck(hipGetDevice(&dev), "hipGetDevice")) || (st = libraryPool(dev, &pool)))
+		return st;
+	const hipError_t e = hipMallocFromPoolAsync(ptr, bytes ? bytes : 256, pool, stream);
+	if(e != hipSuccess) {
+		*ptr = nullptr;
+		return hipCheck(e, "hipMallocFromPoolAsync");
+	}
+	return RDU_OK;
+}
+
 DevBuffer::~DevBuffer() { release(); }
 
 void DevBuffer::release() {
 	if(ptr) {
 		// As hipFree would: wait until no stream of the block's device can still use it, then
-		// return it (to that device's pool, where the next allocation may take it at once).
+		// return it to the pool.
 		int current = -1;
 		(void)hipGetDevice(&current);
 		if(device >= 0 && device != current)
@@ -89,27 +114,49 @@ void DevBuffer::release() {
 	device = -1;
 }
 
+void DevBuffer::releaseOn(hipStream_t s) {
+	if(ptr)
+		(void)hipFreeAsync(ptr, s);
+	ptr = nullptr;
+	bytes = 0;
+	device = -1;
+}
+
 int DevBuffer::reserve(size_t n) {
 	if(n <= bytes && ptr)
 		return RDU_OK;
 	release();
-	if(allocFaultInjected())
-		return fail(RDU_ERR_ALLOC, "device allocation failed (RDU_FAIL_ALLOC=1)");
-	int dev = 0, st;
-	if((st = hipCheck(hipGetDevice(&dev), "hipGetDevice")) || (st = poolReady(dev)))
+	int dev = 0;
+	int st = hipCheck(hipGetDevice(&dev), "hipGetDevice");
+	if(st)
 		return st;
 	const size_t want = n < 256 ? 256 : n;
 	// Allocated in the null stream's order, which is complete once that stream has drained: the
 	// block is then usable from every stream, like a hipMalloc result.
-	hipError_t e = hipMallocAsync(&ptr, want, nullptr);
-	if(e == hipSuccess)
-		e = hipStreamSynchronize(nullptr);
+	if((st = poolAlloc(&ptr, want, nullptr)))
+		return st;
+	const hipError_t e = hipStreamSynchronize(nullptr);
 	if(e != hipSuccess) {
-		if(ptr)
-			(void)hipFreeAsync(ptr, nullptr);
+		(void)hipFreeAsync(ptr, nullptr);
 		ptr = nullptr;
-		return hipCheck(e, "hipMallocAsync");
+		return hipCheck(e, "hipStreamSynchronize");
 	}
+	bytes = want;
+	device = dev;
+	return RDU_OK;
+}
+
+int DevBuffer::reserveOn(size_t n, hipStream_t s) {
+	if(n <= bytes && ptr)
+		return RDU_OK;
+	releaseOn(s);
+	int dev = 0;
+	int st = hipCheck(hipGetDevice(&dev), "hipGetDevice");
+	if(st)
+		return st;
+	const size_t want = n < 256 ? 256 : n;
+	if((st = poolAlloc(&ptr, want, s)))
+		return st;
 	bytes = want;
 	device = dev;
 	return RDU_OK;

@@ -26,11 +26,11 @@ int checkBuilt(const rdu_raycaster * rc) {
 	return RDU_OK;
 }
 
-int ensureGlobalStack(const rdu_raycaster * rc, const TraceOptions & opt, size_t n) {
+int ensureGlobalStack(const rdu_raycaster * rc, const TraceOptions & opt, size_t n, hipStream_t s) {
 	const size_t ints = globalStackInts(rc->dev, opt, n);
 	if(ints == 0)
 		return RDU_OK;
-	return rc->gstack.reserve(ints * sizeof(int));
+	return rc->gstack.reserveOn(ints * sizeof(int), s);
 }
 
 // Traversal choice for a batch. RDU_TRACE_LAYOUT=binary selects the binary near-first kernels
@@ -95,11 +95,23 @@ int query(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, void * out, 
 	if((st = setDevice(rc->device)))
 		return st;
 	hipStream_t s = static_cast<hipStream_t>(stream);
-	if((st = ensureGlobalStack(rc, opt, n)))
+	// Order this batch after the previous batch's use of the scratch buffers (on whatever stream
+	// it ran), and record the end of this batch's use on `s` before returning.
+	if(!rc->scratchDone &&
+		(st = hipCheck(hipEventCreateWithFlags(&rc->scratchDone, hipEventDisableTiming), "hipEventCreate")))
+		return st;
+	if((st = hipCheck(hipStreamWaitEvent(s, rc->scratchDone, 0), "hipStreamWaitEvent")))
+		return st;
+	struct MarkDone {
+		const rdu_raycaster * rc;
+		hipStream_t s;
+		~MarkDone() { (void)hipEventRecord(rc->scratchDone, s); }
+	} markDone{rc, s};
+	if((st = ensureGlobalStack(rc, opt, n, s)))
 		return st;
 	uint64_t * dstats = nullptr;
 	if(statsOut) {
-		if((st = rc->stats.reserve(2 * sizeof(uint64_t))))
+		if((st = rc->stats.reserveOn(2 * sizeof(uint64_t), s)))
 			return st;
 		dstats = static_cast<uint64_t *>(rc->stats.ptr);
 		if((st = hipCheck(hipMemsetAsync(dstats, 0, 2 * sizeof(uint64_t), s), "hipMemsetAsync")))
@@ -112,7 +124,7 @@ int query(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, void * out, 
 		dRays = reinterpret_cast<const float4 *>(rays);
 		dOut = out;
 	} else {
-		if((st = rc->stageIn.reserve(n * sizeof(rdu_ray))) || (st = rc->stageOut.reserve(outBytes)))
+		if((st = rc->stageIn.reserveOn(n * sizeof(rdu_ray), s)) || (st = rc->stageOut.reserveOn(outBytes, s)))
 			return st;
 		if((st = hipCheck(hipMemcpyAsync(rc->stageIn.ptr, rays, n * sizeof(rdu_ray), hipMemcpyHostToDevice, s),
 				"hipMemcpyAsync H2D")))
@@ -122,8 +134,8 @@ int query(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, void * out, 
 	}
 	if((flags & RDU_INCOHERENT) && !opt.referenceOrder && n >= kReorderMin) {
 		const size_t tb = sortTempBytes(n);
-		if((st = rc->keys.reserve(n * 4)) || (st = rc->keysAlt.reserve(n * 4)) || (st = rc->perm.reserve(n * 4)) ||
-			(st = rc->permAlt.reserve(n * 4)) || (st = rc->sortTemp.reserve(tb)))
+		if((st = rc->keys.reserveOn(n * 4, s)) || (st = rc->keysAlt.reserveOn(n * 4, s)) || (st = rc->perm.reserveOn(n * 4, s)) ||
+			(st = rc->permAlt.reserveOn(n * 4, s)) || (st = rc->sortTemp.reserveOn(tb, s)))
 			return st;
 		if((st = hipCheck(hipError_t(launchRayOrder(rc->dev, dRays, n, static_cast<uint32_t *>(rc->keys.ptr),
 								  static_cast<uint32_t *>(rc->keysAlt.ptr), static_cast<uint32_t *>(rc->perm.ptr),
@@ -176,44 +188,62 @@ int query(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, void * out, 
 	return RDU_OK;
 }
 
-// Per-thread resources of the per-ray entry points, one set per device: a stream, a pinned
+// Per-thread resources of the GPU per-ray entry points, one set per device: a stream, a pinned
 // host-mapped record (the ray, then the result) that the kernel reads and writes in place, and a
-// spill stack. Created on a thread's first per-ray call on a device and kept for the life of the
-// process (a stream and a few KB per thread and device, deliberately not freed at thread exit,
-// when the HIP runtime may already be gone), so that per-ray calls from many host threads need
-// no lock and no copies and run concurrently on the GPU.
+// spill stack. Created on a thread's first GPU per-ray call on a device and released when the
+// thread exits, so that per-ray calls from many host threads need no lock and no copies and run
+// concurrently on the GPU.
 struct PerRayCtx {
+	int device = 0;
 	hipStream_t stream = nullptr;
 	unsigned char * host = nullptr; // rdu_ray at 0, rdu_hit at 32, occlusion byte at 56
 	unsigned char * dev = nullptr;  // device address of `host`
-	int * gstack = nullptr;
-	size_t gstackInts = 0;
+	DevBuffer gstack;               // in `stream`'s order
+	~PerRayCtx() {
+		if(!stream)
+			return;
+		int current = -1;
+		(void)hipGetDevice(&current);
+		(void)hipSetDevice(device);
+		(void)hipStreamSynchronize(stream);
+		gstack.releaseOn(stream);
+		(void)hipStreamSynchronize(stream);
+		(void)hipStreamDestroy(stream);
+		if(host)
+			(void)hipHostFree(host);
+		if(current >= 0)
+			(void)hipSetDevice(current);
+	}
 };
 constexpr int kMaxPerRayDevices = 16;
-thread_local PerRayCtx * tlsPerRay[kMaxPerRayDevices] = {};
+struct PerRayThread {
+	PerRayCtx * ctx[kMaxPerRayDevices] = {};
+	~PerRayThread() {
+		for(PerRayCtx * c : ctx)
+			delete c;
+	}
+};
+thread_local PerRayThread tlsPerRay;
 
 int perRayCtx(int device, PerRayCtx ** out) {
 	if(device < 0 || device >= kMaxPerRayDevices)
 		return fail(RDU_ERR_UNSUPPORTED, "per-ray queries support devices 0..15");
-	if(!tlsPerRay[device]) {
-		PerRayCtx c;
-		int st;
-		if((st = hipCheck(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate")))
-			return st;
-		if((st = hipCheck(hipHostMalloc(reinterpret_cast<void **>(&c.host), 64, hipHostMallocMapped), "hipHostMalloc"))) {
-			(void)hipStreamDestroy(c.stream);
-			return st;
-		}
-		if((st = hipCheck(hipHostGetDevicePointer(reinterpret_cast<void **>(&c.dev), c.host, 0), "hipHostGetDevicePointer"))) {
-			(void)hipHostFree(c.host);
-			(void)hipStreamDestroy(c.stream);
-			return st;
-		}
-		tlsPerRay[device] = new(std::nothrow) PerRayCtx(c);
-		if(!tlsPerRay[device])
+	PerRayCtx *& slot = tlsPerRay.ctx[device];
+	if(!slot) {
+		PerRayCtx * c = new(std::nothrow) PerRayCtx();
+		if(!c)
 			return fail(RDU_ERR_ALLOC, "host allocation failed");
+		c->device = device;
+		int st;
+		if((st = hipCheck(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate")) ||
+			(st = hipCheck(hipHostMalloc(reinterpret_cast<void **>(&c->host), 64, hipHostMallocMapped), "hipHostMalloc")) ||
+			(st = hipCheck(hipHostGetDevicePointer(reinterpret_cast<void **>(&c->dev), c->host, 0), "hipHostGetDevicePointer"))) {
+			delete c;
+			return st;
+		}
+		slot = c;
 	}
-	*out = tlsPerRay[device];
+	*out = slot;
 	return RDU_OK;
 }
 
@@ -230,22 +260,13 @@ int queryOne(const rdu_raycaster * rc, const rdu_ray & ray, rdu_hit * hit, uint8
 	if((st = perRayCtx(rc->device, &c)))
 		return st;
 	const size_t ints = globalStackInts(rc->dev, opt, 1);
-	if(ints > c->gstackInts) {
-		if(c->gstack)
-			(void)hipFree(c->gstack);
-		c->gstack = nullptr;
-		c->gstackInts = 0;
-		if(std::getenv("RDU_FAIL_ALLOC") || hipMalloc(reinterpret_cast<void **>(&c->gstack), ints * sizeof(int)) != hipSuccess) {
-			(void)hipGetLastError();
-			c->gstack = nullptr;
-			return fail(RDU_ERR_ALLOC, "device allocation failed (per-ray spill stack)");
-		}
-		c->gstackInts = ints;
-	}
+	if(ints && (st = c->gstack.reserveOn(ints * sizeof(int), c->stream)))
+		return st;
 	std::memcpy(c->host, &ray, sizeof(rdu_ray));
 	const float4 * dRay = reinterpret_cast<const float4 *>(c->dev);
-	const int e = ANY ? launchAny(rc->dev, dRay, 1, c->dev + 56, opt, c->gstack, nullptr, c->stream)
-					  : launchClosest(rc->dev, dRay, 1, c->dev + 32, opt, c->gstack, nullptr, c->stream);
+	int * gs = static_cast<int *>(c->gstack.ptr);
+	const int e = ANY ? launchAny(rc->dev, dRay, 1, c->dev + 56, opt, gs, nullptr, c->stream)
+					  : launchClosest(rc->dev, dRay, 1, c->dev + 32, opt, gs, nullptr, c->stream);
 	if((st = hipCheck(hipError_t(e), "trace kernel launch")) ||
 		(st = hipCheck(hipStreamSynchronize(c->stream), "hipStreamSynchronize")))
 		return st;

@@ -31,6 +31,9 @@ int ensureTrain(rdu_rcache * c, uint32_t n) {
 	int st;
 	const size_t recs = size_t(cap) * kRcRecordsPerSample;
 	const size_t tempBytes = rcReduceTempBytes(uint32_t(recs));
+	// capacity and the pointers describe the buffers only after every one of them has grown: a
+	// growth that fails half-way leaves capacity 0, so the next call grows them all again.
+	c->tb.capacity = 0;
 	if((st = c->tx.reserve(size_t(kRcIn + 5 * kRcWidth) * cap * 2)) ||
 		(st = c->tg.reserve(size_t(5 * kRcWidth + kRcOutPad) * cap * 2)) ||
 		(st = c->tloss.reserve(size_t(cap) * 4)) || (st = c->rKey.reserve(recs * 4)) ||

@@ -23,7 +23,8 @@ struct RenderPaths {
 	int device = 0;
 	DevBuffer pRays, pHits, pAtten, pColor, pQ0, pQ1, pCounts, pTotals, pSRays, pSContrib, pSQ0, pSQ1, pOcc, pSHits,
 		pGstack, pImage;
-	size_t pathCap = 0;
+	size_t pathCap = 0;             // paths every path buffer above holds (0 while a growth is unfinished)
+	uint32_t * walkLeft = nullptr;  // pinned host word for the shadow walk's count (scenes with cut-outs)
 };
 
 namespace {
@@ -134,10 +135,17 @@ struct rdu_scene {
 	}
 	~rdu_scene() {
 		dropGraph();
+		// Renders run on `stream` (uploads are synchronous): once it drains, every block of the
+		// scene is idle and is freed without waiting for the rest of the device.
 		if(stream)
-			(void)hipStreamDestroy(stream);
+			(void)hipStreamSynchronize(stream);
+		for(rdu::DevBuffer * b : {&dTexels, &dTextures, &dNrm, &dTan, &dBit, &dUv, &dIdx, &dMeshes, &dObjects, &dMaterials,
+				 &dLights, &cQuery, &cQAtten, &cQPath, &cPred, &cRec, &cRecPath, &cTrainQ, &cTrainT, &cHoldQ, &cHoldT, &cCounts})
+			b->releaseOn(nullptr);
 		if(rc)
 			rdu_raycaster_destroy(rc);
+		if(stream)
+			(void)hipStreamDestroy(stream);
 	}
 };
 
@@ -176,37 +184,51 @@ int uploadVec(DevBuffer & b, const std::vector<T> & v) {
 }
 
 // Sizes a leased set for passes of P paths over this scene's tree (the spill stack depends on the
-// tree's depth, so it is checked on every render).
-int ensurePaths(RenderPaths & R, const rdu_scene * s, size_t P) {
+// tree's depth, so it is checked on every render). Growth is ordered on the render's stream: the
+// set's previous render drained its own stream before the lease, so the old blocks are idle. While
+// a growth is unfinished pathCap is 0, so a set whose growth failed half-way (some buffers freed,
+// some small) is regrown in full by the next render that leases it instead of being used as is.
+int ensurePaths(RenderPaths & R, const rdu_scene * s, size_t P, hipStream_t stream) {
 	int st;
 	if(P > R.pathCap) {
-		if((st = R.pRays.reserve(P * 32)) || (st = R.pHits.reserve(P * sizeof(rdu_hit))) || (st = R.pAtten.reserve(P * 16)) ||
-			(st = R.pColor.reserve(P * 16)) || (st = R.pQ0.reserve(P * 4)) || (st = R.pQ1.reserve(P * 4)) ||
-			(st = R.pCounts.reserve(kCountSlots * 4)) || (st = R.pTotals.reserve(2 * 8)) || (st = R.pSRays.reserve(P * 32)) ||
-			(st = R.pSContrib.reserve(P * 16)) || (st = R.pSQ0.reserve(P * 4)) || (st = R.pSQ1.reserve(P * 4)) ||
-			(st = R.pOcc.reserve(P)) || (st = R.pSHits.reserve(P * sizeof(rdu_hit))))
+		R.pathCap = 0;
+		if((st = R.pRays.reserveOn(P * 32, stream)) || (st = R.pHits.reserveOn(P * sizeof(rdu_hit), stream)) ||
+			(st = R.pAtten.reserveOn(P * 16, stream)) || (st = R.pColor.reserveOn(P * 16, stream)) ||
+			(st = R.pQ0.reserveOn(P * 4, stream)) || (st = R.pQ1.reserveOn(P * 4, stream)) ||
+			(st = R.pCounts.reserveOn(kCountSlots * 4, stream)) || (st = R.pTotals.reserveOn(2 * 8, stream)) ||
+			(st = R.pSRays.reserveOn(P * 32, stream)) || (st = R.pSContrib.reserveOn(P * 16, stream)) ||
+			(st = R.pSQ0.reserveOn(P * 4, stream)) || (st = R.pSQ1.reserveOn(P * 4, stream)) ||
+			(st = R.pOcc.reserveOn(P, stream)) || (st = R.pSHits.reserveOn(P * sizeof(rdu_hit), stream)))
 			return st;
 		R.pathCap = P;
 	}
+	if(!R.walkLeft && s->dev.anyMasked &&
+		(st = hipCheck(hipHostMalloc(reinterpret_cast<void **>(&R.walkLeft), sizeof(uint32_t), hipHostMallocDefault),
+			 "hipHostMalloc")))
+		return st;
 	TraceOptions o;
 	o.wide = true;
 	const size_t g = globalStackInts(s->rc->dev, o, P);
-	if(g && (st = R.pGstack.reserve(g * sizeof(int))))
+	if(g && (st = R.pGstack.reserveOn(g * sizeof(int), stream)))
 		return st;
 	return RDU_OK;
 }
 
-// Cache-mode buffers for passes of up to P paths with `records` training records.
+// Cache-mode buffers for passes of up to P paths with `records` training records, on the scene's
+// stream (the only one their users run on). cacheCap is 0 while a growth is unfinished.
 int ensureCacheBuffers(rdu_scene * s, size_t P, size_t records) {
 	if(P <= s->cacheCap && records <= s->cRecPath.bytes / 4)
 		return RDU_OK;
 	const size_t hold = records / 8 + 1;
 	const size_t pred = P > hold ? P : hold;
+	hipStream_t q = s->stream;
+	s->cacheCap = 0;
 	int st;
-	if((st = s->cQuery.reserve(P * 32)) || (st = s->cQAtten.reserve(P * 16)) || (st = s->cQPath.reserve(P * 4)) ||
-		(st = s->cPred.reserve(pred * 16)) || (st = s->cRec.reserve(records * 64)) || (st = s->cRecPath.reserve(records * 4)) ||
-		(st = s->cTrainQ.reserve(records * 32)) || (st = s->cTrainT.reserve(records * 16)) ||
-		(st = s->cHoldQ.reserve(hold * 32)) || (st = s->cHoldT.reserve(hold * 16)) || (st = s->cCounts.reserve(kCacheSlots * 4)))
+	if((st = s->cQuery.reserveOn(P * 32, q)) || (st = s->cQAtten.reserveOn(P * 16, q)) || (st = s->cQPath.reserveOn(P * 4, q)) ||
+		(st = s->cPred.reserveOn(pred * 16, q)) || (st = s->cRec.reserveOn(records * 64, q)) ||
+		(st = s->cRecPath.reserveOn(records * 4, q)) || (st = s->cTrainQ.reserveOn(records * 32, q)) ||
+		(st = s->cTrainT.reserveOn(records * 16, q)) || (st = s->cHoldQ.reserveOn(hold * 32, q)) ||
+		(st = s->cHoldT.reserveOn(hold * 16, q)) || (st = s->cCounts.reserveOn(kCacheSlots * 4, q)))
 		return st;
 	s->cacheCap = P;
 	return RDU_OK;
@@ -402,6 +424,16 @@ int rdu_scene_commit(rdu_scene * s) {
 	int st = rdu_raycaster_create(&s->rc);
 	if(st)
 		return st;
+	// A failed commit leaves the scene uncommitted and without a raycaster, so it can be retried.
+	struct DropOnFailure {
+		rdu_scene * s;
+		~DropOnFailure() {
+			if(!s->committed && s->rc) {
+				rdu_raycaster_destroy(s->rc);
+				s->rc = nullptr;
+			}
+		}
+	} dropOnFailure{s};
 	// PathTracer::PathTracer: every object's mesh, with its model, in object order (the raycaster's
 	// stores sized once for all of them).
 	size_t totalVerts = 0, totalTris = 0;
@@ -541,10 +573,10 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 		return st;
 	hipStream_t stream = s->stream;
 	PathsLease R(s->rc->device, pixels * perPass, stream);
-	if((st = ensurePaths(*R, s, pixels * perPass)))
+	if((st = ensurePaths(*R, s, pixels * perPass, stream)))
 		return st;
 	const size_t imgBytes = pixels * components * sizeof(float);
-	if((st = R->pImage.reserve(imgBytes)))
+	if((st = R->pImage.reserveOn(imgBytes, stream)))
 		return st;
 	if(!tuneOut && (st = hipCheck(hipMemcpyAsync(R->pImage.ptr, image, imgBytes, hipMemcpyHostToDevice, stream), "image upload")))
 		return st;
@@ -567,6 +599,7 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 	P.shadowHits = static_cast<rdu_hit *>(R->pSHits.ptr);
 	PtScratch scratch;
 	scratch.gstack = static_cast<int *>(R->pGstack.ptr);
+	scratch.walkLeft = R->walkLeft;
 	// Radiance-cache mode set-up.
 	PtCache C;
 	rdu_cache_opts co{};
@@ -676,7 +709,8 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 			return ptRenderPass(s->dev, s->rc->dev, f, P, uint32_t(n), po, dImage, components, scratch, cache ? &C : nullptr,
 				tune, stream);
 		};
-		if(s->useGraph && !cache && n == perPass) {
+		// Scenes with cut-outs run eagerly: their shadow walk reads a count on the host.
+		if(s->useGraph && !cache && n == perPass && !s->dev.anyMasked) {
 			// A configuration seen on the previous full pass is captured once, then replayed.
 			std::vector<unsigned char> key = passKey(f, po, P, scratch, dImage, uint32_t(n), components, tune);
 			if(!(s->graphExec && key == s->graphKey) && key == s->lastPassKey) {

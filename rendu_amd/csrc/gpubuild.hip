@@ -340,9 +340,9 @@ __global__ void gbSeedQueue(uint2 * q, uint32_t * count, uint32_t * wideCount) {
 	}
 }
 
-// One stream-ordered block for every temporary of a build: hipMallocAsync / hipFreeAsync order
-// with the build's stream, where a plain hipFree would wait for the whole device, so builds overlap
-// other streams' renders. Sub-allocations are 256-B aligned.
+// One stream-ordered block (from the library's pool) for every temporary of a build: allocation
+// and free order with the build's stream, where a plain hipFree would wait for the whole device,
+// so builds overlap other streams' renders. Sub-allocations are 256-B aligned.
 class Arena {
 public:
 	explicit Arena(hipStream_t s) : s_(s) {}
@@ -358,11 +358,7 @@ public:
 		return o;
 	}
 	int allocate() {
-		if(hipMallocAsync(&base_, size_ ? size_ : 256, s_) != hipSuccess) {
-			base_ = nullptr;
-			return fail(RDU_ERR_ALLOC, "GPU build: stream-ordered allocation failed");
-		}
-		return RDU_OK;
+		return poolAlloc(&base_, size_ ? size_ : 256, s_);
 	}
 	template<typename T>
 	T * at(size_t off) const {

@@ -527,20 +527,35 @@ int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, cons
 					return e;
 				hipLaunchKernelGGL(ptShadowApply, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, P, P.counts);
 			} else {
-				// Cut-out walk: bounded by the number of masked layers a ray can cross; each step
-				// is a closest-hit query on the remaining shadow rays.
-				for(int layer = 0; layer < opt.maxShadowLayers; ++layer) {
-					TraceOptions so;
-					applyTuning(so, tn);
-					so.wide = true;
-					so.perm = P.shadowQueue[layer & 1];
-					so.countDev = P.counts + kCountShadow;
-					e = launchClosest(bvh, P.shadowRays, nPaths, P.shadowHits, so, scratch.gstack, nullptr, stream);
-					if(e)
-						return e;
-					hipLaunchKernelGGL(ptShadowWalk, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, S, P,
-						P.shadowQueue[layer & 1], P.shadowQueue[1 - (layer & 1)], P.counts);
-					hipLaunchKernelGGL(ptShadowSwap, dim3(1), dim3(64), 0, stream, P.counts, P.totals);
+				// Cut-out walk: each step is a closest-hit query on the shadow rays still walking
+				// (their count stays on the device). Steps run in rounds of maxShadowLayers; after
+				// each round the host reads how many rays walk on and stops at none, so a ray
+				// crosses any number of transparent layers, as Rendu's loop does.
+				int layer = 0;
+				for(;;) {
+					for(int k = 0; k < opt.maxShadowLayers; ++k, ++layer) {
+						TraceOptions so;
+						applyTuning(so, tn);
+						so.wide = true;
+						so.perm = P.shadowQueue[layer & 1];
+						so.countDev = P.counts + kCountShadow;
+						e = launchClosest(bvh, P.shadowRays, nPaths, P.shadowHits, so, scratch.gstack, nullptr, stream);
+						if(e)
+							return e;
+						hipLaunchKernelGGL(ptShadowWalk, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, S, P,
+							P.shadowQueue[layer & 1], P.shadowQueue[1 - (layer & 1)], P.counts);
+						hipLaunchKernelGGL(ptShadowSwap, dim3(1), dim3(64), 0, stream, P.counts, P.totals);
+					}
+					if(!scratch.walkLeft)
+						return int(hipErrorInvalidValue); // masked scenes need the host count
+					hipError_t he = hipMemcpyAsync(scratch.walkLeft, P.counts + kCountShadow, sizeof(uint32_t),
+						hipMemcpyDeviceToHost, stream);
+					if(he == hipSuccess)
+						he = hipStreamSynchronize(stream);
+					if(he != hipSuccess)
+						return int(he);
+					if(*scratch.walkLeft == 0)
+						break;
 				}
 			}
 		}

@@ -55,6 +55,7 @@ struct PtOptions {
 
 struct PtScratch {
 	int * gstack = nullptr; // traversal stack spill, globalStackInts() for the pass size
+	uint32_t * walkLeft = nullptr; // pinned host word: shadow rays still walking (scenes with cut-outs)
 };
 
 // Radiance-cache mode of a pass (an extension; plain tracing when absent). Paths of the pass's
@@ -98,7 +99,9 @@ struct PtCache {
 // One pass of samplesInPass samples per pixel, starting at the sample index the device holds in
 // kCountBaseNext, with the random key in kCountSeed (both set with ptSetSampleBase before the
 // first pass). Traversal launches use `tune`.
-// Enqueues only kernels (no copies, fills or host synchronisation), so it can be captured in a graph.
+// Enqueues only kernels (no copies, fills or host synchronisation), so it can be captured in a graph,
+// except in scenes with alpha cut-outs: their shadow walk reads the number of rays still walking
+// into scratch.walkLeft between rounds of steps (a host synchronisation), so it runs eagerly.
 int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, const PtPaths & P, uint32_t samplesInPass,
 	const PtOptions & opt, float * dImage, uint32_t components, PtScratch & scratch, const PtCache * cache,
 	const TraceTuning & tune, hipStream_t stream);

@@ -20,10 +20,14 @@ struct rdu_raycaster {
 	// GPU-built tree (rdu_raycaster_update_hierarchy_gpu): near-first queries only.
 	rdu::GpuBvh gpu;
 	bool gpuBuilt = false;
-	// Staging for host-pointer queries and global stack spill; guarded by `lock`.
+	// Staging for host-pointer queries and global stack spill; guarded by `lock`. Batches use
+	// them on their own streams: each batch's stream first waits for `scratchDone`, recorded on
+	// the previous batch's stream after its last use of them, so batches on different streams
+	// never overlap on this memory, and the buffers grow and shrink in that stream order.
 	mutable std::mutex lock;
 	mutable rdu::DevBuffer stageIn, stageOut, gstack, stats;
 	mutable rdu::DevBuffer keys, keysAlt, perm, permAlt, sortTemp; // incoherent-batch ordering
+	mutable hipEvent_t scratchDone = nullptr;
 	// Autotuned launch parameters per [incoherent][any-hit] class; classes this raycaster has not
 	// tuned take the process-wide tuning cache's entry for the scene's statistics, if any.
 	rdu::TraceTuning tuned[2][2];
@@ -35,6 +39,16 @@ struct rdu_raycaster {
 	mutable std::vector<hipEvent_t> timingEvents; // 2 per launch
 	mutable size_t timedLaunches = 0;
 	~rdu_raycaster() {
+		// Batches still in flight use the scratch buffers and the tree until `scratchDone`;
+		// renders and per-ray queries have finished (their callers wait for them). After that
+		// every block is idle and is freed without waiting for the rest of the device.
+		if(scratchDone) {
+			(void)hipEventSynchronize(scratchDone);
+			(void)hipEventDestroy(scratchDone);
+		}
+		for(rdu::DevBuffer * b : {&stageIn, &stageOut, &gstack, &stats, &keys, &keysAlt, &perm, &permAlt, &sortTemp,
+				 &nodes, &tris, &roots, &wide, &gpu.tris, &gpu.wide})
+			b->releaseOn(nullptr);
 		for(hipEvent_t e : timingEvents)
 			(void)hipEventDestroy(e);
 	}

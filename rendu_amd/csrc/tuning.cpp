@@ -27,7 +27,9 @@ uint32_t floorLog2(size_t v) {
 	return k;
 }
 
-constexpr const char * kHeader = "rdu-tune 1";
+// Format 2: blocks_per_cu counts 256-thread workgroups of the persistent traversal (format 1
+// files counted 128-thread ones and are rejected, so a stale grid size is never loaded).
+constexpr const char * kHeader = "rdu-tune 2";
 
 } // namespace
 

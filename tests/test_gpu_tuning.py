@@ -60,7 +60,7 @@ def test_loaded_parameters_keep_results(soup, tmp_path, params, n, incoherent):
     assert soup.tuning(n, incoherent=incoherent)["source"] == 0
     cls = INCOHERENT if incoherent else 0
     f = tmp_path / "tune.txt"
-    f.write_text("rdu-tune 1\n" + key_line(soup, n, cls, params) + "\n" + key_line(soup, n, cls | ANY, params) + "\n")
+    f.write_text("rdu-tune 2\n" + key_line(soup, n, cls, params) + "\n" + key_line(soup, n, cls | ANY, params) + "\n")
     tuning.load(f)
     got = soup.tuning(n, incoherent=incoherent)
     assert got["source"] == 2
@@ -180,12 +180,17 @@ def test_graph_replays_across_seeds_and_recaptures_on_a_new_frame():
     assert np.array_equal(img, refs[(32, 32, 1)]) and st["graph_passes"] == 0
 
 
-def test_graph_replay_masked_walk_and_lights():
-    """A scene with alpha cut-outs (the closest-hit shadow walk, several kernels per layer), three
-    analytic lights and a lat-long background replays bit for bit."""
+def test_graph_replay_lights_latlong_and_eager_cut_outs():
+    """A scene with analytic lights (any-hit shadow rays) and a lat-long background replays bit for
+    bit. A scene with alpha cut-outs renders eagerly even with replay on (its shadow walk reads a
+    count on the host between rounds of steps, so it has no fixed launch sequence): same image,
+    nothing replayed."""
     from test_gpu_pathtracer import feature_scene
     from rendu_amd import scene as S
-    scn, cam_of = feature_scene(S.BG_LATLONG)
+    scn, cam_of = scenes.background_probe(S.BG_LATLONG, level=0.6)
+    scn.lights.append(S.Light(S.LIGHT_POINT, (3.0, 3.0, 3.0), (0.5, 1.5, 0.5), radius=8.0))
+    scn.lights.append(S.Light(S.LIGHT_SPOT, (2.0, 2.0, 2.0), (0.0, 2.0, 1.0), (0.0, -1.0, 0.0), radius=6.0,
+                              angles=(0.3, 0.6)))
     w, h = 40, 30
     cam = cam_of(w, h)
     ref = np.zeros((h, w, 3), F32)
@@ -193,8 +198,20 @@ def test_graph_replay_masked_walk_and_lights():
     pt = PathTracer(scn)
     pt.set_graph(True)
     for k in range(3):
-        log_case(f"graph replay feature scene render {k}")
+        log_case(f"graph replay lights + lat-long render {k}")
         img = np.zeros((h, w, 3), F32)
         st = pt.render(cam, 8, 5, img, seed=3, max_paths=w * h * 2)
         assert np.array_equal(img, ref)
         assert st["graph_passes"] == (3 if k == 0 else 4) and st["shadow_rays"] > 0
+    scn, cam_of = feature_scene(S.BG_LATLONG)
+    cam = cam_of(w, h)
+    ref = np.zeros((h, w, 3), F32)
+    PathTracer(scn).render(cam, 8, 5, ref, seed=3, max_paths=w * h * 2)
+    pt = PathTracer(scn)
+    pt.set_graph(True)
+    for k in range(2):
+        log_case(f"cut-out scene with replay enabled, render {k}")
+        img = np.zeros((h, w, 3), F32)
+        st = pt.render(cam, 8, 5, img, seed=3, max_paths=w * h * 2)
+        assert np.array_equal(img, ref)
+        assert st["graph_passes"] == 0 and st["graph_captures"] == 0 and st["shadow_rays"] > 0

@@ -7,7 +7,7 @@ import pytest
 from rendu_amd import tuning
 from rendu_amd._lib import RduError
 
-HEADER = "rdu-tune 1\n"
+HEADER = "rdu-tune 2\n"
 
 
 @pytest.fixture(autouse=True)
@@ -49,7 +49,7 @@ def test_merge_file_entries_win(tmp_path):
 
 @pytest.mark.parametrize("body", [
     "",                                 # no header
-    "rdu-tune 2\n",                     # another format
+    "rdu-tune 1\n",                     # the format of 128-thread workgroups
     HEADER + "20 11 0 23 1 24 128\n",   # a field missing
     HEADER + "20 11 0 23 1 24 128 6 7\n",  # a field too many
     HEADER + "20 11 0 23 1 65 128 6\n",  # refill above 64
