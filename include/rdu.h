@@ -86,6 +86,19 @@ int rdu_raycaster_update_hierarchy_gpu(rdu_raycaster* rc, void* stream);
 int rdu_raycaster_intersect(const rdu_raycaster* rc, const rdu_ray* rays, size_t n, rdu_hit* hits, uint32_t flags,
 	void* stream);
 
+/* Allocation-free variant of rdu_raycaster_intersect. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same results as rdu_raycaster_intersect with the same flags;
+ * rays and hits must be device memory (flags must hold RDU_DEVICE_POINTERS) and the call is
+ * asynchronous on `stream`. It allocates nothing: the spill stack and the ray-ordering buffers of a
+ * batch come from a fixed 64 MiB device array that each device the library runs on holds until the
+ * process ends. A batch needing more (about 2^21 RDU_INCOHERENT rays with ordering, fewer on deep
+ * trees) is rejected with RDU_ERR_UNSUPPORTED before anything is launched. Every call on a device
+ * shares that array: calls that overlap on one device race, so callers serialise them (one
+ * stream, or their own ordering). */
+int rdu_raycaster_intersect_noalloc(const rdu_raycaster* rc, const rdu_ray* rays, size_t n, rdu_hit* hits,
+	uint32_t flags, void* stream);
+
 /* Batch form of Raycaster::intersectsAny (Raycaster.hpp:74, Raycaster.cpp:187-224):
  * occluded[i] = 1 if any triangle is hit in (tmin, tmax), else 0. */
 int rdu_raycaster_intersect_any(const rdu_raycaster* rc, const rdu_ray* rays, size_t n, uint8_t* occluded,

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <string>
 #include <vector>
 
 using namespace rdu;
@@ -277,9 +278,96 @@ int queryOne(const rdu_raycaster * rc, const rdu_ray & ray, rdu_hit * hit, uint8
 	return RDU_OK;
 }
 
+// Scratch of the allocation-free batch entry (rdu_raycaster_intersect_noalloc): a file-scope
+// device array, one copy per device, held by every device the library runs on until the process
+// ends. Host code never passes `scratch` itself to a kernel (on the host the name stands for a
+// host-side placeholder, not a device address): each device's address comes from
+// hipGetSymbolAddress, looked up once per device with that device current, and is what the
+// kernels receive. Every call on a device shares its array, so calls that overlap on one device
+// race; nothing in it needs to start at zero (every kernel writes its part before reading it).
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kNoAllocDevices = 64;
+std::mutex noAllocMutex;
+unsigned char * noAllocScratch[kNoAllocDevices] = {};
+
+// The calling thread's current device must be `device`.
+int noAllocScratchFor(int device, unsigned char ** out) {
+	if(device < 0 || device >= kNoAllocDevices)
+		return fail(RDU_ERR_UNSUPPORTED, "the allocation-free entry supports devices 0..63");
+	std::lock_guard<std::mutex> g(noAllocMutex);
+	if(!noAllocScratch[device]) {
+		void * p = nullptr;
+		const int st = hipCheck(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+		if(st)
+			return st;
+		noAllocScratch[device] = static_cast<unsigned char *>(p);
+	}
+	*out = noAllocScratch[device];
+	return RDU_OK;
+}
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
 } // namespace
 
 extern "C" {
+
+int rdu_raycaster_intersect_noalloc(const rdu_raycaster * rc, const rdu_ray * rays, size_t n, rdu_hit * hits,
+	uint32_t flags, void * stream) {
+	clearError();
+	int st = checkBuilt(rc);
+	if(st)
+		return st;
+	if(!(flags & RDU_DEVICE_POINTERS))
+		return fail(RDU_ERR_INVALID, "the allocation-free entry takes device pointers only (RDU_DEVICE_POINTERS)");
+	if(n == 0)
+		return RDU_OK;
+	if(!rays || !hits)
+		return fail(RDU_ERR_INVALID, "null rays or output");
+	if(n > (size_t(1) << 31))
+		return fail(RDU_ERR_UNSUPPORTED, "batch larger than 2^31 rays");
+	TraceOptions opt = traceOptions(flags);
+	if(rc->gpuBuilt && (opt.referenceOrder || !opt.wide))
+		return fail(RDU_ERR_STATE, "a GPU-built tree has no reference visiting order or binary layout");
+	applyTuning(opt, tuningFor(rc, n, flags, false, nullptr));
+	opt.nearFar = (flags & RDU_INCOHERENT) != 0;
+	opt.unorderedAny = (flags & RDU_INCOHERENT) == 0;
+	// The same work as rdu_raycaster_intersect, with every scratch buffer carved out of `scratch`:
+	// the spill stack, then (sorted incoherent batches) keys, keys', order, order' and the sort's
+	// temporary storage, each 256-byte aligned.
+	const bool order = (flags & RDU_INCOHERENT) && !opt.referenceOrder && n >= kReorderMin;
+	const size_t stackBytes = align256(globalStackInts(rc->dev, opt, n) * sizeof(int));
+	const size_t keyBytes = align256(n * 4);
+	const size_t tempBytes = order ? sortTempBytes(n) : 0;
+	const size_t need = stackBytes + (order ? 4 * keyBytes + align256(tempBytes) : 0);
+	if(need > kNoAllocScratchBytes)
+		return fail(RDU_ERR_UNSUPPORTED, "batch needs " + std::to_string(need) + " bytes of scratch; the allocation-free entry holds " +
+											  std::to_string(kNoAllocScratchBytes) + " (use rdu_raycaster_intersect, or smaller batches)");
+	if((st = setDevice(rc->device)))
+		return st;
+	unsigned char * base = nullptr;
+	if((st = noAllocScratchFor(rc->device, &base)))
+		return st;
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	const float4 * dRays = reinterpret_cast<const float4 *>(rays);
+	int * gstack = stackBytes ? reinterpret_cast<int *>(base) : nullptr;
+	if(order) {
+		unsigned char * k = base + stackBytes;
+		uint32_t * keys = reinterpret_cast<uint32_t *>(k);
+		uint32_t * keysAlt = reinterpret_cast<uint32_t *>(k + keyBytes);
+		uint32_t * perm = reinterpret_cast<uint32_t *>(k + 2 * keyBytes);
+		uint32_t * permAlt = reinterpret_cast<uint32_t *>(k + 3 * keyBytes);
+		void * temp = k + 4 * keyBytes;
+		if((st = hipCheck(hipError_t(launchRayOrder(rc->dev, dRays, n, keys, keysAlt, perm, permAlt, temp, tempBytes, s)),
+				"ray ordering")))
+			return st;
+		opt.perm = perm;
+		opt.spatialOrder = true;
+	}
+	return hipCheck(hipError_t(launchClosest(rc->dev, dRays, n, hits, opt, gstack, nullptr, s)), "trace kernel launch");
+}
 
 int rdu_raycaster_create(rdu_raycaster ** out) {
 	clearError();
