@@ -104,10 +104,19 @@ int rdu_raycaster_intersect_noalloc(const rdu_raycaster* rc, const rdu_ray* rays
 int rdu_raycaster_intersect_any(const rdu_raycaster* rc, const rdu_ray* rays, size_t n, uint8_t* occluded,
 	uint32_t flags, void* stream);
 
-/* Per-ray Raycaster::intersects with Rendu's signature and defaults. Runs as a batch of one on the
- * GPU in reference order (synchronous); prefer the batch entry for throughput. */
+/* Per-ray Raycaster::intersects with Rendu's signature and defaults (Raycaster.hpp:65,
+ * Raycaster.cpp:144-185), synchronous and thread-safe, in Rendu's visiting order: every field
+ * bit-identical to Rendu's and to the RDU_REFERENCE_ORDER batch. One ray is a chain of dependent
+ * reads that a host core finishes sooner than a GPU round trip, so on a tree from
+ * rdu_raycaster_update_hierarchy the call walks the host copy of the uploaded tree (no lock, no
+ * allocation); on a GPU-built tree, or after rdu_raycaster_set_per_ray_device(rc, 1), it is a
+ * one-ray GPU launch. Prefer the batch entry for throughput. */
 int rdu_raycaster_intersect1(const rdu_raycaster* rc, const float origin[3], const float direction[3], float mini,
 	float maxi, rdu_hit* out);
+
+/* Where per-ray calls (intersect1, intersect_any1, visible) run on a host-built tree: 0 (default)
+ * the host walk of the uploaded tree, 1 a one-ray GPU launch each (no reference counterpart). */
+int rdu_raycaster_set_per_ray_device(rdu_raycaster* rc, int on_device);
 
 /* Per-ray Raycaster::intersectsAny. *occluded = 1 on a hit. */
 int rdu_raycaster_intersect_any1(const rdu_raycaster* rc, const float origin[3], const float direction[3],
@@ -260,6 +269,18 @@ const rdu_raycaster* rdu_scene_raycaster(const rdu_scene* scene);
  * like Rendu's Image is accumulated into (+=) and then divided by the sample count; the caller
  * zeroes it first. samples is rounded to a power of two as Rendu does. opts may be NULL. */
 int rdu_pathtracer_render(rdu_scene* scene, const rdu_camera_desc* camera, size_t samples, size_t depth,
+	float* image, uint32_t width, uint32_t height, uint32_t components, const rdu_render_opts* opts);
+
+/* The camera as PathTracer::render reads it (PathTracer.cpp:176-196): Camera::position() and the
+ * outputs of Camera::pixelShifts(corner, dx, dy) (Camera.cpp:42-49). */
+typedef struct {
+	float position[3], corner[3], dx[3], dy[3];
+} rdu_camera_frame;
+
+/* rdu_pathtracer_render with the camera given by its frame: a caller holding Rendu's Camera passes
+ * exactly the frame Rendu's render would use (the frame of a rdu_camera_desc is recomputed from
+ * its pose inputs and equals it only when they are Camera::pose's own inputs). */
+int rdu_pathtracer_render_frame(rdu_scene* scene, const rdu_camera_frame* frame, size_t samples, size_t depth,
 	float* image, uint32_t width, uint32_t height, uint32_t components, const rdu_render_opts* opts);
 
 /* Timing and counters of the last render on this scene (no reference counterpart). */

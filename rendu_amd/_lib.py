@@ -71,6 +71,7 @@ def _load() -> ctypes.CDLL:
         "rdu_raycaster_intersect": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_uint32, c_void_p]),
         "rdu_raycaster_intersect_any": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_uint32, c_void_p]),
         "rdu_raycaster_intersect_noalloc": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_uint32, c_void_p]),
+        "rdu_raycaster_set_per_ray_device": (c_int, [c_void_p, c_int]),
         "rdu_raycaster_intersect1": (c_int, [c_void_p, fp, fp, c_float, c_float, c_void_p]),
         "rdu_raycaster_intersect_any1": (c_int, [c_void_p, fp, fp, c_float, c_float, POINTER(c_int)]),
         "rdu_raycaster_visible": (c_int, [c_void_p, fp, fp, POINTER(c_int)]),

@@ -248,12 +248,21 @@ int perRayCtx(int device, PerRayCtx ** out) {
 	return RDU_OK;
 }
 
-// One ray, synchronous, on the calling thread's own stream and pinned record (no raycaster lock).
+// One ray, synchronous, with no raycaster lock: on a host-built tree the host walk of the same
+// flattened tree (host_query.cpp), else (GPU-built tree, or per-ray device mode) a one-ray launch on
+// the calling thread's own stream and pinned record.
 template<bool ANY>
 int queryOne(const rdu_raycaster * rc, const rdu_ray & ray, rdu_hit * hit, uint8_t * occ) {
 	int st = checkBuilt(rc);
 	if(st)
 		return st;
+	if(!rc->gpuBuilt && !rc->perRayDevice) {
+		if(ANY)
+			*occ = hostIntersectAny(rc, ray) ? 1 : 0;
+		else
+			hostIntersect(rc, ray, hit);
+		return RDU_OK;
+	}
 	TraceOptions opt = traceOptions(perRayFlags(rc));
 	if((st = setDevice(rc->device)))
 		return st;
@@ -514,6 +523,14 @@ int rdu_raycaster_intersect_any1(const rdu_raycaster * rc, const float o[3], con
 	const int st = queryOne<true>(rc, r, nullptr, &occ);
 	*occluded = occ;
 	return st;
+}
+
+int rdu_raycaster_set_per_ray_device(rdu_raycaster * rc, int on_device) {
+	clearError();
+	if(!rc)
+		return fail(RDU_ERR_INVALID, "null raycaster");
+	rc->perRayDevice = on_device != 0;
+	return RDU_OK;
 }
 
 int rdu_raycaster_visible(const rdu_raycaster * rc, const float p0[3], const float p1[3], int * visible) {

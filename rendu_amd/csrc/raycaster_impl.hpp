@@ -10,6 +10,14 @@
 #include <mutex>
 #include <vector>
 
+struct rdu_raycaster;
+
+namespace rdu {
+// host_query.cpp: one ray in Rendu's visiting order over the host copy of the flattened tree.
+void hostIntersect(const rdu_raycaster * rc, const rdu_ray & ray, rdu_hit * out);
+bool hostIntersectAny(const rdu_raycaster * rc, const rdu_ray & ray);
+} // namespace rdu
+
 struct rdu_raycaster {
 	rdu::HostScene scene;
 	rdu::DeviceLayout layout;
@@ -20,6 +28,10 @@ struct rdu_raycaster {
 	// GPU-built tree (rdu_raycaster_update_hierarchy_gpu): near-first queries only.
 	rdu::GpuBvh gpu;
 	bool gpuBuilt = false;
+	// Per-ray calls (intersect1, intersect_any1, visible) on a host-built tree walk the host copy
+	// of the flattened tree (host_query.cpp) unless this is set (rdu_raycaster_set_per_ray_device):
+	// then each call is a one-ray GPU launch. GPU-built trees always take the GPU.
+	bool perRayDevice = false;
 	// Staging for host-pointer queries and global stack spill; guarded by `lock`. Batches use
 	// them on their own streams: each batch's stream first waits for `scratchDone`, recorded on
 	// the previous batch's stream after its last use of them, so batches on different streams

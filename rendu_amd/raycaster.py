@@ -98,6 +98,11 @@ class Raycaster:
         check(lib.rdu_raycaster_update_hierarchy_gpu(self._h, ctypes.c_void_p(stream) if stream else None))
 
     # -- per-ray API (reference signatures) ---------------------------------------------------------
+    def set_per_ray_device(self, on_device: bool) -> None:
+        """Per-ray calls as one-ray GPU launches (True) instead of the host walk of the uploaded
+        tree (False, the default on a host-built tree): rdu_raycaster_set_per_ray_device."""
+        check(lib.rdu_raycaster_set_per_ray_device(self._h, int(on_device)))
+
     def intersects(self, origin, direction, mini: float = 0.0001, maxi: float = 1e8) -> Hit:
         """Raycaster::intersects (Raycaster.hpp:65)."""
         o = np.asarray(origin, np.float32).reshape(3).copy()

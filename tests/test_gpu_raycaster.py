@@ -135,9 +135,18 @@ def test_any_hit_exact(mixed, mixed_rays, reference_order):
     assert np.array_equal(got, exp)
 
 
-def test_per_ray_api(mixed):
+@pytest.fixture(params=[False, True], ids=["host-walk", "gpu-launch"])
+def per_ray_mode(request, mixed):
+    """Per-ray calls on the host walk of the uploaded tree (default) or as one-ray GPU launches."""
+    ours, _ = mixed
+    ours.set_per_ray_device(request.param)
+    yield request.param
+    ours.set_per_ray_device(False)
+
+
+def test_per_ray_api(mixed, per_ray_mode):
     ours, ref = mixed
-    log_case("per-ray intersects/intersectsAny/visible")
+    log_case(f"per-ray intersects/intersectsAny/visible (GPU launch: {per_ray_mode})")
     rays = workloads.soup_rays(0, 64, seed=9)
     exp = ref.intersect(rays, "closest")
     for i in range(64):
@@ -157,12 +166,12 @@ def test_per_ray_api(mixed):
         assert vis == (not bool(ref.intersect(seg, "any")["hit"][0]))
 
 
-def test_per_ray_api_from_many_threads(mixed):
-    """Per-ray calls from 8 host threads at once (each thread has its own stream and pinned record):
-    every result equals the oracle's, as from one thread."""
+def test_per_ray_api_from_many_threads(mixed, per_ray_mode):
+    """Per-ray calls from 8 host threads at once (host walk: no lock; GPU launches: each thread has
+    its own stream and pinned record): every result equals the oracle's, as from one thread."""
     import threading
     ours, ref = mixed
-    log_case("per-ray intersects from 8 threads")
+    log_case(f"per-ray intersects from 8 threads (GPU launch: {per_ray_mode})")
     rays = workloads.soup_rays(0, 8 * 96, seed=19)
     exp = ref.intersect(rays, "closest")
     got = [None] * rays.shape[0]
