@@ -507,16 +507,17 @@ const rdu_raycaster * rdu_scene_raycaster(const rdu_scene * s) { return s ? s->r
 
 // tuneOut: instead of rendering, time one pass per candidate of the traversal launch parameters
 // (image unused) and keep the fastest for the scene.
+// camFrame (when not null) gives the camera frame directly instead of `cam`'s pose inputs.
 static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples, size_t depth, float * image,
 	uint32_t width, uint32_t height, uint32_t components, const rdu_render_opts * opts, rdu_rcache * cache,
-	const rdu_cache_opts * copts, TuneResult * tuneOut = nullptr) {
+	const rdu_cache_opts * copts, TuneResult * tuneOut = nullptr, const rdu_camera_frame * camFrame = nullptr) {
 	clearError();
 	const auto t0 = std::chrono::steady_clock::now();
 	if(!s)
 		return fail(RDU_ERR_INVALID, "[PathTracer] No scene available.");
 	if(!s->committed)
 		return fail(RDU_ERR_STATE, "scene not committed");
-	if(!cam || (!image && !tuneOut))
+	if((!cam && !camFrame) || (!image && !tuneOut))
 		return fail(RDU_ERR_INVALID, "null camera or image");
 	if(components < 3)
 		return fail(RDU_ERR_INVALID, "[PathTracer] Expected a RGB image.");
@@ -531,23 +532,30 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 	po.depth = uint32_t(depth);
 	po.estimator = opt.estimator;
 	po.rrStart = opt.rr_start;
-	// Camera::pose + Camera::pixelShifts (Camera.cpp:10-18, 42-49).
-	const f3 eye = mk(cam->position[0], cam->position[1], cam->position[2]);
-	const f3 center = mk(cam->center[0], cam->center[1], cam->center[2]);
-	f3 up = normalize(mk(cam->up[0], cam->up[1], cam->up[2]));
-	const f3 viewDir = normalize(center - eye);
-	const f3 right = cross(viewDir, up);
-	up = cross(right, viewDir);
-	const float heightScale = std::tan(0.5f * cam->fov);
-	const float widthScale = cam->ratio * heightScale;
-	const float imageDist = length(center - eye);
 	PtFrame f;
 	f.width = width;
 	f.height = height;
-	f.eye = eye;
-	f.corner = center + imageDist * ((-widthScale) * right + heightScale * up);
-	f.dx = ((2.0f * widthScale) * imageDist) * right;
-	f.dy = ((-2.0f * heightScale) * imageDist) * up;
+	if(camFrame) {
+		f.eye = mk(camFrame->position[0], camFrame->position[1], camFrame->position[2]);
+		f.corner = mk(camFrame->corner[0], camFrame->corner[1], camFrame->corner[2]);
+		f.dx = mk(camFrame->dx[0], camFrame->dx[1], camFrame->dx[2]);
+		f.dy = mk(camFrame->dy[0], camFrame->dy[1], camFrame->dy[2]);
+	} else {
+		// Camera::pose + Camera::pixelShifts (Camera.cpp:10-18, 42-49).
+		const f3 eye = mk(cam->position[0], cam->position[1], cam->position[2]);
+		const f3 center = mk(cam->center[0], cam->center[1], cam->center[2]);
+		f3 up = normalize(mk(cam->up[0], cam->up[1], cam->up[2]));
+		const f3 viewDir = normalize(center - eye);
+		const f3 right = cross(viewDir, up);
+		up = cross(right, viewDir);
+		const float heightScale = std::tan(0.5f * cam->fov);
+		const float widthScale = cam->ratio * heightScale;
+		const float imageDist = length(center - eye);
+		f.eye = eye;
+		f.corner = center + imageDist * ((-widthScale) * right + heightScale * up);
+		f.dx = ((2.0f * widthScale) * imageDist) * right;
+		f.dy = ((-2.0f * heightScale) * imageDist) * up;
+	}
 	// getSampleGrid (PathTracer.cpp:57-69).
 	const int k = int(std::floor(std::log2(double(spp))));
 	int cx = int(std::pow(2, k / 2)), cy = cx;
@@ -794,6 +802,15 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 int rdu_pathtracer_render(rdu_scene * s, const rdu_camera_desc * cam, size_t samples, size_t depth, float * image,
 	uint32_t width, uint32_t height, uint32_t components, const rdu_render_opts * opts) {
 	return renderImpl(s, cam, samples, depth, image, width, height, components, opts, nullptr, nullptr);
+}
+
+int rdu_pathtracer_render_frame(rdu_scene * s, const rdu_camera_frame * frame, size_t samples, size_t depth,
+	float * image, uint32_t width, uint32_t height, uint32_t components, const rdu_render_opts * opts) {
+	if(!frame) {
+		clearError();
+		return fail(RDU_ERR_INVALID, "null camera frame");
+	}
+	return renderImpl(s, nullptr, samples, depth, image, width, height, components, opts, nullptr, nullptr, nullptr, frame);
 }
 
 int rdu_scene_set_graph(rdu_scene * s, int enable) {
