@@ -224,6 +224,53 @@ def cache_experiment_cornell():
     return out
 
 
+def host_cpu():
+    """Logical CPUs and the model name of the host (from /proc/cpuinfo)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count() or 1, "model": model}
+
+
+def cpu_jobs(orc, threads):
+    """The 64 batched jobs on the host cores, one after another as Rendu would run them (its
+    PathTracer renders one scene per call, over forParallel's threads): each job's scene is built
+    in full (the oracle's Raycaster::updateHierarchy, single-threaded as in Rendu) and rendered at a
+    quarter of its width and height with 4 samples per pixel; the render time is scaled by the
+    job's pixels x spp over the rendered ones (per-sample cost is the same scene, camera and depth).
+    Reports the serial batch's jobs/s and the per-job time distribution."""
+    from rendu_amd import scene as S
+    from rendu_amd import scenes
+    per_job, builds, renders = [], 0.0, 0.0
+    for s in range(64):
+        scn, cam, w, h, spp, depth = scenes.random_job(s)
+        t0 = time.perf_counter()
+        opt = orc.OraclePathTracer(scn)
+        b = time.perf_counter() - t0
+        ws, hs = max(1, w // 4), max(1, h // 4)
+        img = np.zeros((hs, ws, 3), np.float32)
+        t0 = time.perf_counter()
+        opt.render(S.camera_desc(cam(ws, hs)), 4, depth, img, S.RenderOpts(s, 0, 3, 0), threads=threads)
+        r = (time.perf_counter() - t0) * (w * h * 2 ** round(np.log2(spp))) / (ws * hs * 4)
+        per_job.append(b + r)
+        builds += b
+        renders += r
+        del opt
+    tot = sum(per_job)
+    return {"jobs": 64, "jobs_per_s": round(64 / tot, 4), "serial_batch_s": round(tot, 2),
+            "job_s_p50": round(float(np.percentile(per_job, 50)), 3),
+            "job_s_p99": round(float(np.percentile(per_job, 99)), 3),
+            "build_s_total": round(builds, 2), "render_s_total_scaled": round(renders, 2), "threads": threads,
+            "sample": "every job built in full; rendered at 1/4 x 1/4 resolution, 4 spp, scaled to its "
+                      "pixels x spp (power of two, as Rendu rounds)"}
+
+
 def load_pmc():
     """The committed rocprofv3 --pmc summary of the dominant kernel (None when absent)."""
     path = os.path.join(ROOT, "profiles", "trace_closest_soup1m_pmc.json")
@@ -506,10 +553,19 @@ def main():
     extra["jobs64_gpu_build"] = dict(J.summary(res_j, wall_j), workers=6, build_workers=2, order="longest first")
     del res_j
 
-    # Algorithmic bytes per traversal launch: each ray's order index (4 B) and ray (32 B) read once,
-    # each hit written once (24 B), and the traversed scene arrays (4-wide nodes, triangles) once.
-    alg_bytes = N_HALF * (4 + 32 + 24) + info["wide_bytes"]
+    # Algorithmic (compulsory) bytes per traversal launch, the per-ray figure of the workload's
+    # definition: each ray read once (32 B), each hit written once (16 B: t, u, v, triangle id), and
+    # the scene once per launch (Rendu's nodes at 32 B and triangles at 48 B: the box-and-children /
+    # vertex records a traversal must touch), here 48 + (nodes * 32 + triangles * 48) / 2^23 B per ray.
+    # The bytes this layout actually moves per ray (order index 4 B + ray 32 B + 24-B hit + the
+    # 4-wide node and triangle arrays once) are reported beside it.
+    alg_per_ray = 32 + 16 + (info["nodes"] * 32 + info["triangles"] * 48) / N_HALF
+    alg_bytes = N_HALF * alg_per_ray
     achieved = alg_bytes / (trav_ms * 1e-3) / 1e9
+    layout_per_ray = 4 + 32 + 24 + info["wide_bytes"] / N_HALF
+    extra["algorithmic_bytes_per_ray"] = round(alg_per_ray, 2)
+    extra["layout_bytes_per_ray"] = round(layout_per_ray, 2)
+    extra["layout_gbs"] = round(N_HALF * layout_per_ray / (trav_ms * 1e-3) / 1e9, 2)
     out = {
         "metric": "incoherent MRays/s at 1 GPU (closest hit, 1M-triangle soup)",
         "value": round(value, 2),
@@ -555,6 +611,7 @@ def main():
         from oracle import oracle as orc
         ncpu = os.cpu_count() or 1
         threads = max(1, min(ncpu - 1, 15))
+        host = host_cpu()
         ora = orc.OracleRaycaster()
         ora.add_mesh(mesh.positions, mesh.indices)
         ora.update()
@@ -598,13 +655,17 @@ def main():
             "frame_msamples_per_s": round(rate3 / 1e6, 4),
             "frame_1080p_64spp_s_scaled": round(1920 * 1080 * 64 / rate3, 1),
         }
+        out["extra"]["cpu_jobs64"] = cpu_jobs(orc, threads)
         out["cpu_baseline"] = {
             "value": round(args.cpu_passes * sample.shape[0] / cpu_s / 1e6, 4),
             "unit": "MRays/s",
             "cores": threads,
             "kind": "port",
             "sample": f"{args.cpu_passes} passes over the first {sample.shape[0]} of the same incoherent rays, "
-                      f"closest hit, {cpu_s:.1f} s",
+                      f"closest hit, {cpu_s:.1f} s, {threads} threads on a host with {host['nproc']} logical CPUs "
+                      f"({host['model']})",
+            "host_nproc": host["nproc"],
+            "host_cpu_model": host["model"],
         }
     print(json.dumps(out), flush=True)
 
