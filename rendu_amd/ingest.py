@@ -10,15 +10,19 @@ Host-side restatement of what the path-tracer app does before PathTracer::render
 * Resources::getMesh (ResourcesManager.cpp:306-340) with Mesh(stream, Indexed) (Mesh.cpp:100-260:
   only the first three vertices of a face, 'vt' v flipped to 1 - v, vertices shared per distinct
   'v/vt/vn' string) and Mesh::computeNormals (Mesh.cpp:327-350) when normals are missing;
-* Resources::getTexture (ResourcesManager.cpp:345-400): 'r,g,b,a' strings give 8 x 8 constant
+* Resources::getTexture (ResourcesManager.cpp:345-530): 'r,g,b,a' strings give 8 x 8 constant
   textures, file names resolve to .png (then .jpg, .jpeg, .bmp, .tga, .exr) anywhere under the
-  resource roots, cubemaps to name_px, _nx, _ny, _py, _pz, _nz, 8-bit images become value / 255
-  with 4 channels (Image::loadLDR, Image.cpp:291-330), cube faces flipped vertically.
+  resource roots, then to the first level of a mipmapped file (name_0), cubemaps to name_px, _nx,
+  _ny, _py, _pz, _nz or their first level (name_0_px, ...); 8-bit images become value / 255 with
+  4 channels (Image::loadLDR, Image.cpp:291-330), half-float EXR channels are read by name
+  (Image::loadHDR); cube faces are flipped vertically;
+* the atmosphere background ('sun'): Rendu's precomputed scattering table as its texture.
 
 Differences: tangent frames come from per-triangle UV derivatives accumulated per vertex (Rendu
-runs MikkTSpace); they matter only for normal-mapped materials. Only 8-bit non-interlaced PNG
-textures are decoded (JPEG, BMP, TGA and EXR raise). The atmosphere background ('sun') and
-animations are not supported.
+runs MikkTSpace); they matter only for normal-mapped materials. PNG (8-bit, non-interlaced) and
+uncompressed EXR are decoded here; JPEG goes through Pillow's decoder (libjpeg), whose inverse DCT
+and chroma upsampling can differ from stb_image's (Rendu's) by a level or two in some texels.
+Animations are not supported (scenes load in their initial state).
 """
 from __future__ import annotations
 
@@ -280,10 +284,11 @@ def _read_image(path: str, flip: bool) -> np.ndarray:
             if name in chans:
                 out[..., k] = chans[name]
         return out[::-1].copy() if flip else out
-    if not path.lower().endswith(".png"):
-        raise NotImplementedError(f"only PNG and uncompressed EXR textures are decoded ({os.path.basename(path)})")
-    with open(path, "rb") as f:
-        px = read_png(f.read())
+    low = path.lower()
+    if low.endswith((".jpg", ".jpeg", ".png")):
+        px = _read_ldr(path)
+    else:
+        raise NotImplementedError(f"only PNG, JPEG and uncompressed EXR textures are decoded ({os.path.basename(path)})")
     if px.ndim == 2:
         px = px[:, :, None]
     c = px.shape[2]
@@ -295,6 +300,25 @@ def _read_image(path: str, flip: bool) -> np.ndarray:
     else:
         out[..., :c] = px[..., :c].astype(F32) / F32(255.0)
     return out[::-1].copy() if flip else out
+
+
+def _read_ldr(path: str) -> np.ndarray:
+    """8-bit pixels (h, w, c) of a PNG or JPEG. Pillow decodes when present (palette PNGs expand
+    to RGB, or RGBA with their transparency chunk, as stb_image expands them); without it, the
+    library's own reader handles 8-bit non-paletted PNGs."""
+    try:
+        from PIL import Image as PilImage
+    except ImportError:  # pragma: no cover - Pillow is part of the image
+        if path.lower().endswith(".png"):
+            with open(path, "rb") as f:
+                return read_png(f.read())
+        raise NotImplementedError(f"JPEG textures need Pillow ({os.path.basename(path)})")
+    with PilImage.open(path) as im:
+        if im.mode == "P":
+            im = im.convert("RGBA" if "transparency" in im.info else "RGB")
+        elif im.mode not in ("L", "LA", "RGB", "RGBA"):
+            im = im.convert("RGB")
+        return np.asarray(im, dtype=np.uint8)
 
 
 def texture(res: Resources, name: str, cube: bool = False):
@@ -310,12 +334,17 @@ def texture(res: Resources, name: str, cube: bool = False):
         img = np.ascontiguousarray(np.broadcast_to(np.array(col, F32), (8, 8, 4)))
         out = [img] * 6 if cube else img
     elif cube:
-        faces = [res.image_path(name + s) for s in ("_px", "_nx", "_ny", "_py", "_pz", "_nz")]
-        if any(f is None for f in faces):
+        faces = None
+        for base in (name, name + "_0"):  # the cubemap, else the first level of a mipmapped one
+            paths = [res.image_path(base + s) for s in ("_px", "_nx", "_ny", "_py", "_pz", "_nz")]
+            if all(f is not None for f in paths):
+                faces = paths
+                break
+        if faces is None:
             raise FileNotFoundError(f"Unable to find cubemap {name}.")
         out = [_read_image(f, True) for f in faces]
     else:
-        p = res.image_path(name)
+        p = res.image_path(name) or res.image_path(name + "_0")
         if p is None:
             raise FileNotFoundError(f"Unable to find texture {name}.")
         out = _read_image(p, False)

@@ -176,3 +176,57 @@ def test_cornellbox_scene_gpu_vs_oracle():
     d = np.abs(got.astype(np.float64) - exp)
     assert (d <= 1e-4 * np.abs(exp) + 1e-6).mean() >= 0.98
     assert d.mean() <= 1e-3 * np.abs(exp).mean()
+
+
+PBR = os.path.join(os.path.dirname(__file__), "golden", "pbrdemo")
+ROOTS = [GOLD, COMMON, PBR]
+
+
+def test_furnace_scene_loads():
+    """resources/pbrdemo/furnace-test.scene: 22 spheres (radius 1) in rows of 5 and 6, dielectrics
+    then metals, roughness 0.0 ... 1.0 from left to right, white albedo, in Rendu's grey cubemap (the
+    first level of grey_cube_0_*.exr, 0.5 everywhere)."""
+    scn, cam_of = ingest.load_scene(os.path.join(PBR, "furnace-test.scene"), ROOTS)
+    assert len(scn.objects) == 22 and len(scn.materials) == 22 and not scn.lights
+    assert scn.background.mode == S.BG_SKYBOX
+    for f in scn.background.cube:
+        assert np.all(scn.textures[f][..., :3] == 0.5)
+    rough = [float(scn.textures[m.textures[2]][0, 0, 0]) for m in scn.materials]
+    metal = [float(scn.textures[m.textures[2]][0, 0, 1]) for m in scn.materials]
+    assert rough == pytest.approx([k / 10 for k in range(11)] * 2)
+    assert metal == [0.0] * 11 + [1.0] * 11
+    assert all(np.all(scn.textures[m.textures[0]][..., :3] == 1.0) for m in scn.materials)
+    ys = sorted({round(float(np.asarray(o.model)[1, 3]), 3) for o in scn.objects})
+    assert ys == [-4.5, -1.5, 1.5, 4.5]
+    assert not any(o.use_texcoords for o in scn.objects)
+    cam = cam_of(64, 48)
+    assert np.allclose(cam.position, [23.2, 0, 0]) and cam.fov == pytest.approx(0.57)
+
+
+def test_emissive_cornellbox_loads():
+    """resources/pbrdemo/cornellbox-emissive.scene: three emissive walls (radiance 1.25 in one
+    channel), an emissive box textured with light_grid.exr (uses its UVs), no analytic light."""
+    scn, _ = ingest.load_scene(os.path.join(PBR, "cornellbox-emissive.scene"), ROOTS)
+    assert len(scn.objects) == 7 and not scn.lights
+    emissive = [m for m in scn.materials if m.type == S.MATERIAL_EMISSIVE]
+    assert len(emissive) == 4
+    cols = sorted(tuple(scn.textures[m.textures[0]][0, 0, :3].round(3)) for m in emissive)
+    assert (0.0, 0.0, 1.25) in cols and (0.0, 1.25, 0.0) in cols and (1.25, 0.0, 0.0) in cols
+    grid = [m for m in emissive if scn.textures[m.textures[0]].shape[:2] == (32, 32)]
+    assert len(grid) == 1 and scn.textures[grid[0].textures[0]][..., 0].max() > 3.0
+    # Objects are sorted by material type: the emissive ones come last, the grid box uses UVs.
+    types = [scn.materials[o.material].type for o in scn.objects]
+    assert types == sorted(types) and sum(o.use_texcoords for o in scn.objects) == 1
+
+
+def test_cutout_cornellbox_loads():
+    """resources/pbrdemo/cornellbox-cutout.scene: an alpha-masked leaves material (palette PNG
+    with transparency, JPEG normal and roughness maps), a vase, two spot lights with their cones."""
+    scn, _ = ingest.load_scene(os.path.join(PBR, "cornellbox-cutout.scene"), ROOTS)
+    masked = [m for m in scn.materials if m.masked]
+    assert len(masked) == 1
+    alpha = scn.textures[masked[0].textures[0]][..., 3]
+    assert alpha.min() == 0.0 and alpha.max() == 1.0
+    assert [l.type for l in scn.lights] == [S.LIGHT_SPOT, S.LIGHT_SPOT]
+    assert scn.lights[0].angles == pytest.approx((0.5, 0.6)) and scn.lights[1].angles == pytest.approx((0.35, 0.4))
+    assert scn.lights[0].radius == 5.0 and scn.lights[0].intensity == pytest.approx((6.0, 5.4, 3.0))
