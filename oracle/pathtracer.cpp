@@ -84,6 +84,26 @@ struct Object {
 	m3 invtp;
 };
 
+// Extension (extended estimator only): light sampling of the lat-long environment and of emissive
+// triangles, combined with BSDF sampling by the power heuristic. Tables are built in double
+// precision in a fixed order, as the product builds them, so both pick the same samples.
+struct AreaLights {
+	int envW = 0, envH = 0;
+	std::vector<float> envPmf;    // per texel: probability of the texel
+	std::vector<float> envRowCdf; // envH + 1
+	std::vector<float> envColCdf; // envH rows of envW + 1
+	struct Tri {
+		v3 v0, e1, e2, n; // n = cross(e1, e2)
+		float area;
+		uint32_t object, localId;
+	};
+	std::vector<Tri> tris;
+	std::vector<float> triPmf, triCdf;  // triCdf: tris + 1
+	std::vector<int32_t> objectBase;    // first emissive triangle of each object, or -1
+	float pEnv = 0.0f;                  // environment vs triangles, within area-light sampling
+	float pArea = 0.0f;                 // area lights vs analytic lights
+};
+
 struct Scene {
 	std::vector<Texture> textures;
 	std::vector<MeshData> meshes;
@@ -94,6 +114,7 @@ struct Scene {
 	std::vector<rdu_light_desc> lights;
 	rdu_background_desc bg{};
 	Raycaster rc;
+	AreaLights area;
 	bool committed = false;
 };
 
@@ -179,7 +200,7 @@ v3 sampleCosineHemisphere(float u0, float u1) {
 	return {xy.x, xy.y, z};
 }
 // u: [lobe choice, dir0, dir1]
-v3 ggxSampleAndEval(v3 wo, v3 base, float roughness, float metallic, const float * u, v3 & wi) {
+v3 ggxSampleAndEval(v3 wo, v3 base, float roughness, float metallic, const float * u, v3 & wi, float * pdfOut = nullptr) {
 	const float probaSpecular = mixf(1.0f / (dot(base, v3(1.0f)) / 3.0f + 1.0f), 1.0f, metallic);
 	const float alpha = alphaFromRoughness(roughness);
 	if(u[0] < probaSpecular) {
@@ -201,9 +222,20 @@ v3 ggxSampleAndEval(v3 wo, v3 base, float roughness, float metallic, const float
 	float pdfSpec = 0.0f;
 	const v3 brdf = ggxBrdf(wo, base, alpha, metallic, wi, &pdfSpec);
 	const float pdf = mixf(kOneOverPi * gmax(wi.z, 0.0f), pdfSpec, probaSpecular);
+	if(pdfOut)
+		*pdfOut = pdf;
 	if(pdf == 0.0f)
 		return v3(0.0f);
 	return brdf / pdf;
+}
+// Extension: the mixture density of sampleAndEval for a given direction (0 below the horizon).
+float ggxPdf(v3 wo, v3 base, float roughness, float metallic, v3 wi) {
+	if(wi.z < 0.0f)
+		return 0.0f;
+	const float probaSpecular = mixf(1.0f / (dot(base, v3(1.0f)) / 3.0f + 1.0f), 1.0f, metallic);
+	float pdfSpec = 0.0f;
+	ggxBrdf(wo, base, alphaFromRoughness(roughness), metallic, wi, &pdfSpec);
+	return mixf(kOneOverPi * gmax(wi.z, 0.0f), pdfSpec, probaSpecular);
 }
 v3 ggxEval(v3 wo, v3 base, float roughness, float metallic, v3 wi) {
 	if(wi.z < 0.0f)
@@ -343,6 +375,187 @@ v3 sampleLatLong(const Texture & t, v3 d) {
 	return t.rgbl(u, v);
 }
 
+// ---------------------------------------------------------------------------------------------
+// Extension: area-light sampling (extended estimator)
+// ---------------------------------------------------------------------------------------------
+inline double lumd(double r, double g, double b) { return 0.2126 * r + 0.7152 * g + 0.0722 * b; }
+inline float lumf(v3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+constexpr float kTwoPiSq = 19.7392088021787171f; // 2 pi^2
+
+// Largest k in [0, n) with cdf[k] <= u (cdf: n + 1 entries from 0 to 1, u in [0, 1)).
+int cdfSearch(const float * cdf, int n, float u) {
+	int lo = 0, hi = n;
+	while(hi - lo > 1) {
+		const int mid = (lo + hi) >> 1;
+		if(cdf[mid] <= u)
+			lo = mid;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+
+// Tables: lat-long texel weights = the largest luminance of the texel's bilinear footprint times
+// sin(theta) at the texel centre; emissive triangles weighted by area times the mean luminance of
+// their emission texture.
+void buildAreaLights(Scene & s) {
+	AreaLights & A = s.area;
+	A = AreaLights{};
+	double envPower = 0.0;
+	if(s.bg.mode == RDU_BACKGROUND_LATLONG) {
+		const Texture & t = s.textures[size_t(s.bg.texture)];
+		const int W = t.w, H = t.h;
+		A.envW = W;
+		A.envH = H;
+		std::vector<double> f(size_t(W) * size_t(H)), rows(static_cast<size_t>(H));
+		double total = 0.0;
+		for(int j = 0; j < H; ++j) {
+			const double sinT = std::sin(3.14159265358979323846 * (double(j) + 0.5) / double(H));
+			double row = 0.0;
+			for(int i = 0; i < W; ++i) {
+				double m = 0.0;
+				for(int dj = 0; dj < 2; ++dj)
+					for(int di = 0; di < 2; ++di) {
+						const v4 c = t.at((i + di) % W, (j + dj) % H);
+						m = std::max(m, lumd(c.x, c.y, c.z));
+					}
+				f[size_t(j) * W + i] = m * sinT;
+				row += m * sinT;
+			}
+			rows[size_t(j)] = row;
+			total += row;
+		}
+		if(total > 0.0) {
+			A.envPmf.resize(size_t(W) * H);
+			A.envRowCdf.resize(size_t(H) + 1);
+			A.envColCdf.resize(size_t(H) * (W + 1));
+			double cum = 0.0;
+			A.envRowCdf[0] = 0.0f;
+			for(int j = 0; j < H; ++j) {
+				cum += rows[size_t(j)];
+				A.envRowCdf[size_t(j) + 1] = j + 1 == H ? 1.0f : float(cum / total);
+				float * cc = &A.envColCdf[size_t(j) * (W + 1)];
+				cc[0] = 0.0f;
+				double c2 = 0.0;
+				for(int i = 0; i < W; ++i) {
+					c2 += f[size_t(j) * W + i];
+					cc[i + 1] = i + 1 == W ? 1.0f : (rows[size_t(j)] > 0.0 ? float(c2 / rows[size_t(j)]) : float(double(i + 1) / W));
+					A.envPmf[size_t(j) * W + i] = float(f[size_t(j) * W + i] / total);
+				}
+			}
+			envPower = total;
+		} else {
+			A.envW = A.envH = 0;
+		}
+	}
+	// Emissive triangles, objects in order, triangles in mesh order; world vertices as the
+	// raycaster transformed them (objects were added in order, so object o's vertices follow
+	// those of the objects before it).
+	A.objectBase.assign(s.objects.size(), -1);
+	std::vector<double> w;
+	size_t vbase = 0;
+	for(size_t o = 0; o < s.objects.size(); ++o) {
+		const Object & obj = s.objects[o];
+		const MeshData & m = s.meshes[obj.mesh];
+		const size_t nv = s.meshPos[obj.mesh].size() / 3;
+		const rdu_material_desc & mat = s.materials[obj.mat];
+		if(mat.type == RDU_MATERIAL_EMISSIVE) {
+			const Texture & t = s.textures[size_t(mat.textures[0])];
+			double mean = 0.0;
+			for(int y = 0; y < t.h; ++y)
+				for(int x = 0; x < t.w; ++x) {
+					const v4 c = t.at(x, y);
+					mean += lumd(c.x, c.y, c.z);
+				}
+			mean /= double(t.w) * double(t.h);
+			A.objectBase[o] = int32_t(A.tris.size());
+			for(size_t k = 0; k + 2 < m.idx.size(); k += 3) {
+				AreaLights::Tri tr;
+				tr.v0 = s.rc.vertices[vbase + m.idx[k]];
+				const v3 v1 = s.rc.vertices[vbase + m.idx[k + 1]];
+				const v3 v2 = s.rc.vertices[vbase + m.idx[k + 2]];
+				tr.e1 = v1 - tr.v0;
+				tr.e2 = v2 - tr.v0;
+				tr.n = cross(tr.e1, tr.e2);
+				const double nx = tr.n.x, ny = tr.n.y, nz = tr.n.z;
+				const double area = 0.5 * std::sqrt(nx * nx + ny * ny + nz * nz);
+				tr.area = float(area);
+				tr.object = uint32_t(o);
+				tr.localId = uint32_t(k);
+				A.tris.push_back(tr);
+				w.push_back(area * mean);
+			}
+		}
+		vbase += nv;
+	}
+	double triPower = 0.0;
+	for(double x : w)
+		triPower += x;
+	if(triPower > 0.0) {
+		A.triPmf.resize(w.size());
+		A.triCdf.resize(w.size() + 1);
+		double cum = 0.0;
+		A.triCdf[0] = 0.0f;
+		for(size_t k = 0; k < w.size(); ++k) {
+			cum += w[k];
+			A.triPmf[k] = float(w[k] / triPower);
+			A.triCdf[k + 1] = k + 1 == w.size() ? 1.0f : float(cum / triPower);
+		}
+	} else {
+		A.tris.clear();
+		A.objectBase.assign(s.objects.size(), -1);
+	}
+	const bool env = envPower > 0.0, tri = !A.tris.empty();
+	A.pEnv = env ? (tri ? 0.5f : 1.0f) : 0.0f;
+	A.pArea = (env || tri) ? (s.lights.empty() ? 1.0f : 0.5f) : 0.0f;
+}
+
+// Density (solid angle) of the environment strategy for direction d.
+float envPdf(const Scene & s, v3 d) {
+	const AreaLights & A = s.area;
+	if(A.envW == 0)
+		return 0.0f;
+	const float u = 0.5f + std::atan2(d.x, -d.z) * (0.5f * kOneOverPi);
+	const float v = std::acos(clampf(d.y, -1.0f, 1.0f)) * kOneOverPi;
+	int i = int(u * float(A.envW)), j = int(v * float(A.envH));
+	i = i < 0 ? 0 : (i >= A.envW ? A.envW - 1 : i);
+	j = j < 0 ? 0 : (j >= A.envH ? A.envH - 1 : j);
+	const float sinT = std::sqrt(gmax(0.0f, 1.0f - d.y * d.y));
+	if(!(sinT > 0.0f))
+		return 0.0f;
+	return A.envPmf[size_t(j) * A.envW + i] * float(A.envW * A.envH) / (kTwoPiSq * sinT);
+}
+
+// One direction towards the environment; pdf = the density it was drawn with.
+v3 envSample(const Scene & s, float u1, float u2, float & pdf) {
+	const AreaLights & A = s.area;
+	const int j = cdfSearch(A.envRowCdf.data(), A.envH, u1);
+	const float a = A.envRowCdf[size_t(j)], b = A.envRowCdf[size_t(j) + 1];
+	const float dv = (u1 - a) / (b - a);
+	const float * cc = &A.envColCdf[size_t(j) * (A.envW + 1)];
+	const int i = cdfSearch(cc, A.envW, u2);
+	const float du = (u2 - cc[i]) / (cc[i + 1] - cc[i]);
+	const float v = (float(j) + dv) / float(A.envH), u = (float(i) + du) / float(A.envW);
+	const float theta = v * kPi, phi = (u - 0.5f) * kTwoPi;
+	const float sinT = std::sin(theta), cosT = std::cos(theta);
+	pdf = sinT > 0.0f ? A.envPmf[size_t(j) * A.envW + i] * float(A.envW * A.envH) / (kTwoPiSq * sinT) : 0.0f;
+	return {sinT * std::sin(phi), cosT, -sinT * std::cos(phi)};
+}
+
+// Density (solid angle, from distance dist along a ray of direction dir) of the triangle strategy
+// for a point on emissive triangle t.
+float triPdf(const AreaLights & A, int t, float dist, v3 dir) {
+	const AreaLights::Tri & T = A.tris[size_t(t)];
+	const float cosL = std::abs(dot(T.n, dir)) / length(T.n);
+	return A.triPmf[size_t(t)] / T.area * (dist * dist) / cosL;
+}
+
+// Power heuristic weight of a strategy with density p against one with density q.
+inline float powerWeight(float p, float q) {
+	const float r = q / p;
+	return 1.0f / (1.0f + r * r);
+}
+
 // PathTracer::evalBackground (PathTracer.cpp:23-55).
 v3 evalBackground(const Scene & s, v3 rayDir, v3 rayPos, v2 ndc, bool direct) {
 	const rdu_background_desc & b = s.bg;
@@ -433,12 +646,13 @@ m3 buildLocalFrame(const Scene & s, const Object & obj, const Hit & hit, v3 rayD
 	return tbn;
 }
 
-// PathTracer::checkVisibility (PathTracer.cpp:124-157).
-bool checkVisibility(const Scene & s, v3 startPos, v3 rayDir, float maxDist, uint64_t & rays) {
+// PathTracer::checkVisibility (PathTracer.cpp:124-157). `mini` is Rendu's 0.001; the extension's
+// area-light rays pass the 1e-4 of the path's own rays.
+bool checkVisibility(const Scene & s, v3 startPos, v3 rayDir, float maxDist, uint64_t & rays, float mini = 0.001f) {
 	v3 lpos = startPos;
 	while(maxDist > 0.0f) {
 		++rays;
-		const Hit lhit = s.rc.traverse(lpos, rayDir, 0.001f, maxDist, false, nullptr);
+		const Hit lhit = s.rc.traverse(lpos, rayDir, mini, maxDist, false, nullptr);
 		if(!lhit.hit)
 			return true;
 		const Object & lobj = s.objects[lhit.meshId];
@@ -482,11 +696,20 @@ v3 tracePath(const Scene & s, const rdu_camera_desc & cam, v3 corner, v3 dx, v3 
 	v3 sampleColor(0.0f);
 	v3 attenuation(1.0f);
 	const size_t nL = s.lights.size();
+	// Extension: area lights and multiple importance sampling (extended estimator only).
+	const AreaLights & A = s.area;
+	const bool extended = opt.estimator == RDU_ESTIMATOR_EXTENDED;
+	const float pArea = extended ? A.pArea : 0.0f;
+	float prevPdf = 0.0f; // BSDF density of the last scattering direction (0: no MIS weight)
+	float misDist = 0.0f; // distance travelled since that scattering vertex, through cut-outs
 	for(size_t did = 0; did < depth; ++did) {
 		++ext;
 		const Hit hit = s.rc.traverse(rayPos, rayDir, 0.0001f, 1e8f, false, nullptr);
 		if(!hit.hit) {
-			sampleColor = sampleColor + attenuation * evalBackground(s, rayDir, rayPos, ndc, did == 0);
+			v3 bg = evalBackground(s, rayDir, rayPos, ndc, did == 0);
+			if(prevPdf > 0.0f && A.pEnv > 0.0f)
+				bg = powerWeight(prevPdf, pArea * A.pEnv * envPdf(s, normalize(rayDir))) * bg;
+			sampleColor = sampleColor + attenuation * bg;
 			break;
 		}
 		const Object & obj = s.objects[hit.meshId];
@@ -498,10 +721,16 @@ v3 tracePath(const Scene & s, const rdu_camera_desc & cam, v3 corner, v3 dx, v3 
 		const v4 bCol = s.textures[size_t(mat.textures[0])].rgbal(uv.x, uv.y);
 		if(mat.masked && bCol.w < 0.01f) {
 			rayPos = p;
+			misDist = misDist + hit.dist;
 			continue;
 		}
 		if(mat.type == RDU_MATERIAL_EMISSIVE) {
-			sampleColor = sampleColor + attenuation * v3(bCol.x, bCol.y, bCol.z);
+			v3 le(bCol.x, bCol.y, bCol.z);
+			if(prevPdf > 0.0f && A.pEnv < 1.0f && A.objectBase[hit.meshId] >= 0) {
+				const int t = A.objectBase[hit.meshId] + int(hit.localId / 3);
+				le = powerWeight(prevPdf, pArea * (1.0f - A.pEnv) * triPdf(A, t, misDist + hit.dist, rayDir)) * le;
+			}
+			sampleColor = sampleColor + attenuation * le;
 			break;
 		}
 		const m3 tbn = buildLocalFrame(s, obj, hit, rayDir, uv);
@@ -510,10 +739,72 @@ v3 tracePath(const Scene & s, const rdu_camera_desc & cam, v3 corner, v3 dx, v3 
 		const v3 baseColor(std::pow(bCol.x, 2.2f), std::pow(bCol.y, 2.2f), std::pow(bCol.z, 2.2f));
 		const v4 rmao = s.textures[size_t(mat.textures[2])].rgbal(uv.x, uv.y);
 		rng.draw4(1 + uint32_t(did), u);
-		if(nL > 0) {
-			const size_t lid = std::min(size_t(u[0] * float(nL)), nL - 1); // Random::Int(0, nL-1)
+		const v3 pShift = p + 0.001f * tbn.c[2];
+		// Area lights are sampled at every vertex but the last: there, the reference's path ends
+		// without tracing a next segment, so light one segment further never counts (Rendu's own
+		// analytic lights are sampled at the last vertex too, as the reference does). The pick
+		// is the same at every vertex, so the analytic lights keep their probability.
+		const bool areaHere = pArea > 0.0f && did + 1 < depth;
+		if(pArea > 0.0f && u[0] < pArea) {
+			if(areaHere) {
+				// Extension: one sample of the environment or of an emissive triangle, weighted
+				// against BSDF sampling by the power heuristic. Its visibility ray leaves from the hit
+				// point with the interval of the path's own next ray (1e-4, 1e8), so that a light
+				// point counts as visible exactly when a BSDF-sampled ray towards it would reach it
+				// (with shading normals, an offset origin would see past surfaces that the path's
+				// rays hit, and the two strategies would estimate different integrals).
+				float e[4];
+				rng.draw4(0x20000u + uint32_t(did), e);
+				v3 direction, le;
+				float pLight = 0.0f, pMis = 0.0f, maxDist = 0.0f;
+				if(e[0] < A.pEnv) {
+					direction = envSample(s, e[1], e[2], pLight);
+					pLight = pArea * A.pEnv * pLight;
+					pMis = pArea * A.pEnv * envPdf(s, direction);
+					le = sampleLatLong(s.textures[size_t(s.bg.texture)], normalize(direction));
+					maxDist = 1e8f;
+				} else {
+					const float ut = (e[0] - A.pEnv) / (1.0f - A.pEnv);
+					const int t = cdfSearch(A.triCdf.data(), int(A.tris.size()), ut);
+					const AreaLights::Tri & T = A.tris[size_t(t)];
+					const float su = std::sqrt(e[1]);
+					const float bu = su * (1.0f - e[2]), bv = su * e[2], bw = 1.0f - bu - bv;
+					const v3 y = T.v0 + bu * T.e1 + bv * T.e2;
+					const Object & lo = s.objects[T.object];
+					const MeshData & lm = s.meshes[lo.mesh];
+					v2 luv{0.5f, 0.5f};
+					if(lo.useUV) {
+						const uint32_t i0 = lm.idx[T.localId], i1 = lm.idx[T.localId + 1], i2 = lm.idx[T.localId + 2];
+						luv = {bw * lm.uv[i0].x + bu * lm.uv[i1].x + bv * lm.uv[i2].x, bw * lm.uv[i0].y + bu * lm.uv[i1].y + bv * lm.uv[i2].y};
+					}
+					const v4 ec = s.textures[size_t(s.materials[lo.mat].textures[0])].rgbal(luv.x, luv.y);
+					le = v3(ec.x, ec.y, ec.z);
+					const v3 toLight = y - p;
+					const float dist = length(toLight);
+					direction = toLight / dist;
+					pLight = pArea * (1.0f - A.pEnv) * triPdf(A, t, dist, direction);
+					pMis = pLight;
+					maxDist = dist * 0.999f;
+				}
+				if(pLight > 0.0f && pLight < std::numeric_limits<float>::infinity()) {
+					const v3 lwi = normalize(mul(itbn, direction));
+					const v3 evalLight = ggxEval(wo, baseColor, rmao.x, rmao.y, lwi);
+					const float w = powerWeight(pMis, ggxPdf(wo, baseColor, rmao.x, rmao.y, lwi));
+					const v3 illumination = evalLight * le * (w / pLight);
+					if((illumination.x > 0.0f || illumination.y > 0.0f || illumination.z > 0.0f) &&
+						checkVisibility(s, p, direction, maxDist, shadow, 0.0001f))
+						sampleColor = sampleColor + attenuation * illumination;
+				}
+			}
+		} else if(nL > 0) {
+			float ul = u[0];
+			float lightPdf = 1.0f / float(nL);
+			if(pArea > 0.0f) { // extension: the analytic lights share the pick with the area lights
+				ul = (u[0] - pArea) / (1.0f - pArea);
+				lightPdf = (1.0f - pArea) / float(nL);
+			}
+			const size_t lid = std::min(size_t(ul * float(nL)), nL - 1); // Random::Int(0, nL-1)
 			const rdu_light_desc & light = s.lights[lid];
-			const v3 pShift = p + 0.001f * tbn.c[2];
 			float maxDist, falloff;
 			const v3 direction = lightSample(light, pShift, maxDist, falloff);
 			bool visible = falloff > 0.0f;
@@ -523,15 +814,17 @@ v3 tracePath(const Scene & s, const rdu_camera_desc & cam, v3 corner, v3 dx, v3 
 			if(visible) {
 				const v3 lwi = normalize(mul(itbn, direction));
 				const v3 evalLight = ggxEval(wo, baseColor, rmao.x, rmao.y, lwi);
-				const float lightPdf = 1.0f / float(nL);
 				const v3 illumination = falloff * evalLight * v3(light.intensity[0], light.intensity[1], light.intensity[2]) / lightPdf;
 				sampleColor = sampleColor + attenuation * illumination;
 			}
 		}
 		v3 wi;
-		const v3 eval = ggxSampleAndEval(wo, baseColor, rmao.x, rmao.y, u + 1, wi);
+		float bsdfPdf = 0.0f;
+		const v3 eval = ggxSampleAndEval(wo, baseColor, rmao.x, rmao.y, u + 1, wi, &bsdfPdf);
 		const v3 nextRayDir = normalize(mul(tbn, wi));
 		attenuation = attenuation * eval;
+		prevPdf = pArea > 0.0f ? bsdfPdf : 0.0f;
+		misDist = 0.0f;
 		if(opt.estimator == RDU_ESTIMATOR_EXTENDED && int(did) + 1 >= opt.rr_start && did + 1 < depth) {
 			// Russian roulette (extension): survive with p = min(max component, 0.95), reweight by 1/p.
 			const float pmax = gmax(attenuation.x, gmax(attenuation.y, attenuation.z));
@@ -625,6 +918,7 @@ int orc_pt_commit(void * h) {
 		if(s->rc.hierarchy[m].right == 0)
 			return -2;
 	s->rc.updateHierarchy();
+	buildAreaLights(*s);
 	s->committed = true;
 	return 0;
 }

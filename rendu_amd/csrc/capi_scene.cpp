@@ -22,7 +22,7 @@ namespace rdu {
 struct RenderPaths {
 	int device = 0;
 	DevBuffer pRays, pHits, pAtten, pColor, pQ0, pQ1, pCounts, pTotals, pSRays, pSContrib, pSQ0, pSQ1, pOcc, pSHits,
-		pGstack, pImage;
+		pGstack, pImage, pMis;
 	size_t pathCap = 0;             // paths every path buffer above holds (0 while a growth is unfinished)
 	uint32_t * walkLeft = nullptr;  // pinned host word for the shadow walk's count (scenes with cut-outs)
 };
@@ -107,6 +107,7 @@ struct rdu_scene {
 	rdu_raycaster * rc = nullptr;
 	rdu::DevScene dev;
 	rdu::DevBuffer dTexels, dTextures, dNrm, dTan, dBit, dUv, dIdx, dMeshes, dObjects, dMaterials, dLights;
+	rdu::DevBuffer dEnvPmf, dEnvRow, dEnvCol, dEmTris, dEmCdf, dEmBase; // extended estimator's light sampling
 	// Radiance-cache mode buffers, grown on demand.
 	rdu::DevBuffer cQuery, cQAtten, cQPath, cPred, cRec, cRecPath, cTrainQ, cTrainT, cHoldQ, cHoldT, cCounts;
 	size_t cacheCap = 0;
@@ -140,7 +141,8 @@ struct rdu_scene {
 		if(stream)
 			(void)hipStreamSynchronize(stream);
 		for(rdu::DevBuffer * b : {&dTexels, &dTextures, &dNrm, &dTan, &dBit, &dUv, &dIdx, &dMeshes, &dObjects, &dMaterials,
-				 &dLights, &cQuery, &cQAtten, &cQPath, &cPred, &cRec, &cRecPath, &cTrainQ, &cTrainT, &cHoldQ, &cHoldT, &cCounts})
+				 &dLights, &dEnvPmf, &dEnvRow, &dEnvCol, &dEmTris, &dEmCdf, &dEmBase, &cQuery, &cQAtten, &cQPath, &cPred, &cRec,
+				 &cRecPath, &cTrainQ, &cTrainT, &cHoldQ, &cHoldT, &cCounts})
 			b->releaseOn(nullptr);
 		if(rc)
 			rdu_raycaster_destroy(rc);
@@ -198,7 +200,8 @@ int ensurePaths(RenderPaths & R, const rdu_scene * s, size_t P, hipStream_t stre
 			(st = R.pCounts.reserveOn(kCountSlots * 4, stream)) || (st = R.pTotals.reserveOn(2 * 8, stream)) ||
 			(st = R.pSRays.reserveOn(P * 32, stream)) || (st = R.pSContrib.reserveOn(P * 16, stream)) ||
 			(st = R.pSQ0.reserveOn(P * 4, stream)) || (st = R.pSQ1.reserveOn(P * 4, stream)) ||
-			(st = R.pOcc.reserveOn(P, stream)) || (st = R.pSHits.reserveOn(P * sizeof(rdu_hit), stream)))
+			(st = R.pOcc.reserveOn(P, stream)) || (st = R.pSHits.reserveOn(P * sizeof(rdu_hit), stream)) ||
+			(st = R.pMis.reserveOn(P * 8, stream)))
 			return st;
 		R.pathCap = P;
 	}
@@ -234,6 +237,132 @@ int ensureCacheBuffers(rdu_scene * s, size_t P, size_t records) {
 	return RDU_OK;
 }
 
+
+// Extension (extended estimator): the light-sampling tables, built in double precision in a fixed
+// order: lat-long texel weights = the largest luminance of the texel's bilinear footprint times
+// sin(theta) at the texel centre (row and column CDFs); emissive triangles (objects in order, world
+// vertices as the raycaster transformed them) weighted by area times the mean luminance of their
+// emission texture; environment vs triangles 1/2 each when both exist, area lights vs analytic
+// lights 1/2 each when both exist.
+struct AreaTables {
+	std::vector<float> envPmf, envRow, envCol, emCdf;
+	std::vector<float4> emTris;
+	std::vector<int32_t> emBase;
+	int32_t envW = 0, envH = 0;
+	float pEnv = 0.0f, pArea = 0.0f;
+};
+
+inline double lumd(double r, double g, double b) { return 0.2126 * r + 0.7152 * g + 0.0722 * b; }
+
+AreaTables buildAreaTables(const rdu_scene * s) {
+	AreaTables A;
+	auto texel = [&](const rdu_scene::Tex & t, int x, int y) {
+		return s->texels.data() + 4 * (size_t(t.offset) + size_t(y) * t.w + size_t(x));
+	};
+	double envPower = 0.0;
+	if(s->bg.mode == RDU_BACKGROUND_LATLONG) {
+		const rdu_scene::Tex & t = s->textures[size_t(s->bg.texture)];
+		const int W = int(t.w), H = int(t.h);
+		std::vector<double> f(size_t(W) * size_t(H)), rows(static_cast<size_t>(H));
+		double total = 0.0;
+		for(int j = 0; j < H; ++j) {
+			const double sinT = std::sin(3.14159265358979323846 * (double(j) + 0.5) / double(H));
+			double row = 0.0;
+			for(int i = 0; i < W; ++i) {
+				double m = 0.0;
+				for(int dj = 0; dj < 2; ++dj)
+					for(int di = 0; di < 2; ++di) {
+						const float * c = texel(t, (i + di) % W, (j + dj) % H);
+						m = std::max(m, lumd(c[0], c[1], c[2]));
+					}
+				f[size_t(j) * W + i] = m * sinT;
+				row += m * sinT;
+			}
+			rows[size_t(j)] = row;
+			total += row;
+		}
+		if(total > 0.0) {
+			A.envW = W;
+			A.envH = H;
+			A.envPmf.resize(size_t(W) * H);
+			A.envRow.resize(size_t(H) + 1);
+			A.envCol.resize(size_t(H) * (W + 1));
+			double cum = 0.0;
+			A.envRow[0] = 0.0f;
+			for(int j = 0; j < H; ++j) {
+				cum += rows[size_t(j)];
+				A.envRow[size_t(j) + 1] = j + 1 == H ? 1.0f : float(cum / total);
+				float * cc = &A.envCol[size_t(j) * (W + 1)];
+				cc[0] = 0.0f;
+				double c2 = 0.0;
+				for(int i = 0; i < W; ++i) {
+					c2 += f[size_t(j) * W + i];
+					cc[i + 1] = i + 1 == W ? 1.0f : (rows[size_t(j)] > 0.0 ? float(c2 / rows[size_t(j)]) : float(double(i + 1) / W));
+					A.envPmf[size_t(j) * W + i] = float(f[size_t(j) * W + i] / total);
+				}
+			}
+			envPower = total;
+		}
+	}
+	A.emBase.assign(s->objects.size(), -1);
+	std::vector<double> w;
+	const std::vector<f3> & verts = s->rc->scene.vertices;
+	size_t vbase = 0;
+	for(size_t o = 0; o < s->objects.size(); ++o) {
+		const rdu_object_desc & obj = s->objects[o];
+		const rdu_scene::MeshRec & m = s->meshes[obj.mesh];
+		const rdu_material_desc & mat = s->materials[obj.material];
+		if(mat.type == RDU_MATERIAL_EMISSIVE) {
+			const rdu_scene::Tex & t = s->textures[size_t(mat.textures[0])];
+			double mean = 0.0;
+			for(uint32_t y = 0; y < t.h; ++y)
+				for(uint32_t x = 0; x < t.w; ++x) {
+					const float * c = texel(t, int(x), int(y));
+					mean += lumd(c[0], c[1], c[2]);
+				}
+			mean /= double(t.w) * double(t.h);
+			A.emBase[o] = int32_t(w.size());
+			for(uint32_t k = 0; k + 2 < m.ni; k += 3) {
+				const uint32_t * ix = s->indices.data() + m.idx + k;
+				const f3 v0 = verts[vbase + ix[0]], v1 = verts[vbase + ix[1]], v2 = verts[vbase + ix[2]];
+				const f3 e1 = v1 - v0, e2 = v2 - v0;
+				const f3 n = cross(e1, e2);
+				const double nx = n.x, ny = n.y, nz = n.z;
+				const double area = 0.5 * std::sqrt(nx * nx + ny * ny + nz * nz);
+				uint32_t obits = uint32_t(o), kbits = k;
+				float of, kf;
+				std::memcpy(&of, &obits, 4);
+				std::memcpy(&kf, &kbits, 4);
+				A.emTris.push_back(make_float4(v0.x, v0.y, v0.z, float(area)));
+				A.emTris.push_back(make_float4(e1.x, e1.y, e1.z, 0.0f)); // pmf below
+				A.emTris.push_back(make_float4(e2.x, e2.y, e2.z, of));
+				A.emTris.push_back(make_float4(n.x, n.y, n.z, kf));
+				w.push_back(area * mean);
+			}
+		}
+		vbase += m.nv;
+	}
+	double triPower = 0.0;
+	for(double x : w)
+		triPower += x;
+	if(triPower > 0.0) {
+		A.emCdf.resize(w.size() + 1);
+		double cum = 0.0;
+		A.emCdf[0] = 0.0f;
+		for(size_t k = 0; k < w.size(); ++k) {
+			cum += w[k];
+			A.emTris[4 * k + 1].w = float(w[k] / triPower);
+			A.emCdf[k + 1] = k + 1 == w.size() ? 1.0f : float(cum / triPower);
+		}
+	} else {
+		A.emTris.clear();
+		A.emBase.assign(s->objects.size(), -1);
+	}
+	const bool env = envPower > 0.0, tri = !A.emTris.empty();
+	A.pEnv = env ? (tri ? 0.5f : 1.0f) : 0.0f;
+	A.pArea = (env || tri) ? (s->lights.empty() ? 1.0f : 0.5f) : 0.0f;
+	return A;
+}
 
 template<typename T>
 void appendBytes(std::vector<unsigned char> & k, const T & v) {
@@ -498,6 +627,22 @@ int rdu_scene_commit(rdu_scene * s) {
 	d.nLights = uint32_t(s->lights.size());
 	d.anyMasked = anyMasked;
 	d.bg = s->bg;
+	const AreaTables A = buildAreaTables(s);
+	if((st = uploadVec(s->dEmBase, A.emBase)) || (!A.envPmf.empty() && ((st = uploadVec(s->dEnvPmf, A.envPmf)) ||
+			(st = uploadVec(s->dEnvRow, A.envRow)) || (st = uploadVec(s->dEnvCol, A.envCol)))) ||
+		(!A.emTris.empty() && ((st = uploadVec(s->dEmTris, A.emTris)) || (st = uploadVec(s->dEmCdf, A.emCdf)))))
+		return st;
+	d.envPmf = static_cast<const float *>(s->dEnvPmf.ptr);
+	d.envRowCdf = static_cast<const float *>(s->dEnvRow.ptr);
+	d.envColCdf = static_cast<const float *>(s->dEnvCol.ptr);
+	d.envW = A.envW;
+	d.envH = A.envH;
+	d.emTris = static_cast<const float4 *>(s->dEmTris.ptr);
+	d.emCdf = static_cast<const float *>(s->dEmCdf.ptr);
+	d.emBase = static_cast<const int32_t *>(s->dEmBase.ptr);
+	d.nEm = uint32_t(A.emTris.size() / 4);
+	d.pEnv = A.pEnv;
+	d.pArea = A.pArea;
 	s->committed = true;
 	return RDU_OK;
 }
@@ -605,6 +750,7 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 	P.shadowQueue[1] = static_cast<uint32_t *>(R->pSQ1.ptr);
 	P.occluded = static_cast<uint8_t *>(R->pOcc.ptr);
 	P.shadowHits = static_cast<rdu_hit *>(R->pSHits.ptr);
+	P.mis = static_cast<float2 *>(R->pMis.ptr);
 	PtScratch scratch;
 	scratch.gstack = static_cast<int *>(R->pGstack.ptr);
 	scratch.walkLeft = R->walkLeft;

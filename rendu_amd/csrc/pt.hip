@@ -119,6 +119,7 @@ __global__ __launch_bounds__(kPtBlock) void ptGenerate(PtFrame f, PtPaths P, uin
 	putRay(P.rays, p, eye, dir, 0.0001f, 1e8f);
 	P.atten[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
 	P.color[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	P.mis[p] = make_float2(0.0f, 0.0f);
 	P.queue[0][p] = p;
 }
 
@@ -189,9 +190,15 @@ __global__ __launch_bounds__(kPtBlock) void ptShade(DevScene S, PtFrame f, PtPat
 		f3 atten = {a4.x, a4.y, a4.z};
 		float4 c4 = P.color[p];
 		f3 color = {c4.x, c4.y, c4.z};
+		// Extension (extended estimator): area lights and multiple importance sampling.
+		const float pArea = opt.estimator == RDU_ESTIMATOR_EXTENDED ? S.pArea : 0.0f;
+		const float2 mis = P.mis[p];
 		if(!hit.hit) {
 			const f2 ndc = samplePos(f, seed, pix % f.width, pix / f.width, sid);
-			color = color + atten * evalBackground(S, rayDir, rayPos, ndc, bounce == 0);
+			f3 bg = evalBackground(S, rayDir, rayPos, ndc, bounce == 0);
+			if(mis.x > 0.0f && S.pEnv > 0.0f)
+				bg = powerWeight(mis.x, pArea * S.pEnv * envPdf(S, normalize(rayDir))) * bg;
+			color = color + atten * bg;
 		} else {
 			const DevObject obj = S.objects[hit.mesh_id];
 			const DevMaterial mat = S.materials[obj.mat];
@@ -204,10 +211,16 @@ __global__ __launch_bounds__(kPtBlock) void ptShade(DevScene S, PtFrame f, PtPat
 				// Cut-out: continue from the hit point in the same direction; this consumes a bounce.
 				if(bounce + 1 < opt.depth) {
 					putRay(P.rays, p, pt, rayDir, 0.0001f, 1e8f);
+					P.mis[p] = make_float2(mis.x, mis.y + hit.t);
 					cont = true;
 				}
 			} else if(mat.type == RDU_MATERIAL_EMISSIVE) {
-				color = color + atten * mk(bCol.x, bCol.y, bCol.z);
+				f3 le = mk(bCol.x, bCol.y, bCol.z);
+				if(mis.x > 0.0f && S.pEnv < 1.0f && S.emBase[hit.mesh_id] >= 0) {
+					const uint32_t t = uint32_t(S.emBase[hit.mesh_id]) + hit.local_id / 3u;
+					le = powerWeight(mis.x, pArea * (1.0f - S.pEnv) * triPdf(S, t, mis.y + hit.t, rayDir)) * le;
+				}
+				color = color + atten * le;
 			} else {
 				bool terminate = false;
 				if(CACHE && int(bounce) >= C.terminateBounce) {
@@ -229,8 +242,67 @@ __global__ __launch_bounds__(kPtBlock) void ptShade(DevScene S, PtFrame f, PtPat
 				const f4v rmao = rgbal(S, mat.tex[2], uv.x, uv.y);
 				float u[4];
 				draw4(seed, pix, sid, 1 + bounce, u);
-				if(S.nLights > 0) {
-					uint32_t lid = uint32_t(u[0] * float(S.nLights));
+				if(pArea > 0.0f && u[0] < pArea) {
+					// Extension: one sample of the environment or of an emissive triangle, at every
+					// vertex but the last (the reference's paths end there without a next segment),
+					// weighted against BSDF sampling by the power heuristic. The visibility ray
+					// leaves from the hit point with the path's own interval (1e-4, 1e8), so a light
+					// point is visible exactly when a BSDF ray towards it would reach it.
+					if(bounce + 1 < opt.depth) {
+						float e[4];
+						draw4(seed, pix, sid, 0x20000u + bounce, e);
+						f3 direction, le;
+						float pLight = 0.0f, pMis = 0.0f, maxDist = 0.0f;
+						if(e[0] < S.pEnv) {
+							direction = envSample(S, e[1], e[2], pLight);
+							pLight = pArea * S.pEnv * pLight;
+							pMis = pArea * S.pEnv * envPdf(S, direction);
+							le = sampleLatLong(S, S.bg.texture, normalize(direction));
+							maxDist = 1e8f;
+						} else {
+							const float ut = (e[0] - S.pEnv) / (1.0f - S.pEnv);
+							const uint32_t t = uint32_t(cdfSearch(S.emCdf, int(S.nEm), ut));
+							const float4 t0 = S.emTris[4 * t], t1 = S.emTris[4 * t + 1], t2 = S.emTris[4 * t + 2],
+										 t3 = S.emTris[4 * t + 3];
+							const float su = sqrt_rn(e[1]);
+							const float bu = su * (1.0f - e[2]), bv = su * e[2], bw = 1.0f - bu - bv;
+							const f3 y = mk(t0.x, t0.y, t0.z) + bu * mk(t1.x, t1.y, t1.z) + bv * mk(t2.x, t2.y, t2.z);
+							const DevObject lo = S.objects[__float_as_uint(t2.w)];
+							f2 luv = {0.5f, 0.5f};
+							if(lo.useUV) {
+								const HitVerts lv = hitVerts(S, lo, __float_as_uint(t3.w));
+								luv = interp2(S.uv, lv.i0, lv.i1, lv.i2, bw, bu, bv);
+							}
+							const f4v ec = rgbal(S, S.materials[lo.mat].tex[0], luv.x, luv.y);
+							le = mk(ec.x, ec.y, ec.z);
+							const f3 toLight = y - pt;
+							const float dist = length(toLight);
+							direction = toLight / dist;
+							pLight = pArea * (1.0f - S.pEnv) * triPdf(S, t, dist, direction);
+							pMis = pLight;
+							maxDist = dist * 0.999f;
+						}
+						if(pLight > 0.0f && pLight < __builtin_inff()) {
+							const f3 lwi = normalize(mulT(tbn, direction));
+							const f3 evalLight = ggxEval(wo, baseColor, rmao.x, rmao.y, lwi);
+							const float w = powerWeight(pMis, ggxPdf(wo, baseColor, rmao.x, rmao.y, lwi));
+							const f3 illum = evalLight * le * (w / pLight);
+							if(illum.x > 0.0f || illum.y > 0.0f || illum.z > 0.0f) {
+								const f3 contrib = atten * illum;
+								putRay(P.shadowRays, p, pt, direction, 0.0001f, maxDist);
+								P.shadowContrib[p] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+								shadow = true;
+							}
+						}
+					}
+				} else if(S.nLights > 0) {
+					float ul = u[0];
+					float lightPdf = 1.0f / float(S.nLights);
+					if(pArea > 0.0f) { // extension: the analytic lights share the pick with the area lights
+						ul = (u[0] - pArea) / (1.0f - pArea);
+						lightPdf = (1.0f - pArea) / float(S.nLights);
+					}
+					uint32_t lid = uint32_t(ul * float(S.nLights));
 					lid = lid < S.nLights ? lid : S.nLights - 1;
 					const rdu_light_desc & light = S.lights[lid];
 					const f3 pShift = pt + 0.001f * tbn.c[2];
@@ -239,7 +311,6 @@ __global__ __launch_bounds__(kPtBlock) void ptShade(DevScene S, PtFrame f, PtPat
 					if(falloff > 0.0f) {
 						const f3 lwi = normalize(mulT(tbn, direction));
 						const f3 evalLight = ggxEval(wo, baseColor, rmao.x, rmao.y, lwi);
-						const float lightPdf = 1.0f / float(S.nLights);
 						const f3 illum = falloff * evalLight * mk(light.intensity[0], light.intensity[1], light.intensity[2]) / lightPdf;
 						const f3 contrib = atten * illum;
 						// checkVisibility (PathTracer.cpp:124-157) starts only while maxDist > 0.
@@ -253,7 +324,8 @@ __global__ __launch_bounds__(kPtBlock) void ptShade(DevScene S, PtFrame f, PtPat
 					}
 				}
 				f3 wi;
-				const f3 eval = ggxSampleAndEval(wo, baseColor, rmao.x, rmao.y, u + 1, wi);
+				float bsdfPdf = 0.0f;
+				const f3 eval = ggxSampleAndEval(wo, baseColor, rmao.x, rmao.y, u + 1, wi, &bsdfPdf);
 				const f3 nextDir = normalize(mul(tbn, wi));
 				atten = atten * eval;
 				bool alive = bounce + 1 < opt.depth;
@@ -271,6 +343,7 @@ __global__ __launch_bounds__(kPtBlock) void ptShade(DevScene S, PtFrame f, PtPat
 				if(alive && !(atten.x == 0.0f && atten.y == 0.0f && atten.z == 0.0f)) {
 					putRay(P.rays, p, pt, normalize(nextDir), 0.0001f, 1e8f);
 					P.atten[p] = make_float4(atten.x, atten.y, atten.z, 0.0f);
+					P.mis[p] = make_float2(pArea > 0.0f ? bsdfPdf : 0.0f, 0.0f);
 					cont = true;
 				}
 				}
@@ -422,7 +495,7 @@ __global__ __launch_bounds__(kPtBlock) void ptShadowWalk(DevScene S, PtPaths P, 
 					const float maxDist = rd.w - h.t;
 					const f3 lpos = mk(ro.x, ro.y, ro.z) + h.t * mk(rd.x, rd.y, rd.z);
 					if(maxDist > 0.0f) {
-						putRay(P.shadowRays, p, lpos, mk(rd.x, rd.y, rd.z), 0.001f, maxDist);
+						putRay(P.shadowRays, p, lpos, mk(rd.x, rd.y, rd.z), ro.w, maxDist); // the walk keeps its interval start
 						again = true;
 					} else {
 						visible = true;
@@ -514,7 +587,8 @@ int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, cons
 		else
 			hipLaunchKernelGGL(ptShade<false>, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, S, f, P, P.queue[q],
 				P.queue[1 - q], P.counts, b, opt, C);
-		if(S.nLights > 0) {
+		// Shadow rays exist when there are analytic lights, or area lights under the extended estimator.
+		if(S.nLights > 0 || (opt.estimator == RDU_ESTIMATOR_EXTENDED && S.pArea > 0.0f)) {
 			if(!S.anyMasked) {
 				TraceOptions so;
 				applyTuning(so, tn);

@@ -44,6 +44,8 @@ struct PtPaths {
 	uint32_t * shadowQueue[2] = {nullptr, nullptr};
 	uint8_t * occluded = nullptr;
 	rdu_hit * shadowHits = nullptr;
+	float2 * mis = nullptr;         // extension: BSDF density of the last scattering direction (0: no
+	                                // MIS weight) and the distance travelled since, through cut-outs
 };
 
 struct PtOptions {

@@ -49,6 +49,19 @@ struct DevScene {
 	uint32_t nLights = 0;
 	int32_t anyMasked = 0;
 	rdu_background_desc bg{};
+	// Extension (extended estimator): light sampling of the lat-long environment and of emissive
+	// triangles, combined with BSDF sampling by the power heuristic.
+	const float * envPmf = nullptr;    // envW x envH texel probabilities
+	const float * envRowCdf = nullptr; // envH + 1
+	const float * envColCdf = nullptr; // envH rows of envW + 1
+	int32_t envW = 0, envH = 0;
+	const float4 * emTris = nullptr;   // 4 float4 per emissive triangle: {v0, area} {e1, pmf}
+	                                   // {e2, object bits} {n = cross(e1, e2), first index bits}
+	const float * emCdf = nullptr;     // nEm + 1
+	const int32_t * emBase = nullptr;  // per object: its first emissive triangle, or -1
+	uint32_t nEm = 0;
+	float pEnv = 0.0f;                 // environment vs triangles within area-light sampling
+	float pArea = 0.0f;                // area lights vs analytic lights
 };
 
 struct f2 {
@@ -246,6 +259,67 @@ __device__ inline f3 sampleLatLong(const DevScene & s, int tex, f3 d) {
 	return rgbl(s, tex, u, v);
 }
 
+// ---- extension: area-light sampling ----------------------------------------------------------
+constexpr float kTwoPiSq = 19.7392088021787171f; // 2 pi^2
+
+// Largest k in [0, n) with cdf[k] <= u (cdf: n + 1 entries from 0 to 1, u in [0, 1)).
+__device__ inline int cdfSearch(const float * cdf, int n, float u) {
+	int lo = 0, hi = n;
+	while(hi - lo > 1) {
+		const int mid = (lo + hi) >> 1;
+		if(cdf[mid] <= u)
+			lo = mid;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+
+// Density (solid angle) of the environment strategy for direction d.
+__device__ inline float envPdf(const DevScene & s, f3 d) {
+	if(s.envW == 0)
+		return 0.0f;
+	const float u = 0.5f + atan2f(d.x, -d.z) * (0.5f * kOneOverPi);
+	const float v = acosf(clampf(d.y, -1.0f, 1.0f)) * kOneOverPi;
+	int i = int(u * float(s.envW)), j = int(v * float(s.envH));
+	i = i < 0 ? 0 : (i >= s.envW ? s.envW - 1 : i);
+	j = j < 0 ? 0 : (j >= s.envH ? s.envH - 1 : j);
+	const float sinT = sqrt_rn(gmax(0.0f, 1.0f - d.y * d.y));
+	if(!(sinT > 0.0f))
+		return 0.0f;
+	return s.envPmf[j * s.envW + i] * float(s.envW * s.envH) / (kTwoPiSq * sinT);
+}
+
+// One direction towards the environment; pdf = the density it was drawn with.
+__device__ inline f3 envSample(const DevScene & s, float u1, float u2, float & pdf) {
+	const int j = cdfSearch(s.envRowCdf, s.envH, u1);
+	const float a = s.envRowCdf[j], b = s.envRowCdf[j + 1];
+	const float dv = (u1 - a) / (b - a);
+	const float * cc = s.envColCdf + size_t(j) * (s.envW + 1);
+	const int i = cdfSearch(cc, s.envW, u2);
+	const float du = (u2 - cc[i]) / (cc[i + 1] - cc[i]);
+	const float v = (float(j) + dv) / float(s.envH), u = (float(i) + du) / float(s.envW);
+	const float theta = v * kPi, phi = (u - 0.5f) * kTwoPi;
+	const float sinT = sinf(theta), cosT = cosf(theta);
+	pdf = sinT > 0.0f ? s.envPmf[j * s.envW + i] * float(s.envW * s.envH) / (kTwoPiSq * sinT) : 0.0f;
+	return {sinT * sinf(phi), cosT, -sinT * cosf(phi)};
+}
+
+// Density (solid angle, at distance dist along direction dir) of the triangle strategy for a point
+// on emissive triangle t.
+__device__ inline float triPdf(const DevScene & s, uint32_t t, float dist, f3 dir) {
+	const float4 a = s.emTris[4 * t], b = s.emTris[4 * t + 1], n4 = s.emTris[4 * t + 3];
+	const f3 n = {n4.x, n4.y, n4.z};
+	const float cosL = fabsf(dot(n, dir)) / length(n);
+	return b.w / a.w * (dist * dist) / cosL;
+}
+
+// Power heuristic weight of a strategy with density p (> 0) against one with density q.
+RDU_HD float powerWeight(float p, float q) {
+	const float r = q / p;
+	return 1.0f / (1.0f + r * r);
+}
+
 // PathTracer::evalBackground (PathTracer.cpp:23-55).
 __device__ inline f3 evalBackground(const DevScene & s, f3 rayDir, f3 rayPos, f2 ndc, bool direct) {
 	const rdu_background_desc & b = s.bg;
@@ -312,8 +386,9 @@ RDU_HD f2 sampleDisk(float u0, float u1) {
 	}
 	return {radius * cosf(angle), radius * sinf(angle)};
 }
-// u: [lobe choice, dir0, dir1]
-RDU_HD f3 ggxSampleAndEval(f3 wo, f3 base, float roughness, float metallic, const float * u, f3 & wi) {
+// u: [lobe choice, dir0, dir1]; pdfOut (optional) receives the mixture density of wi.
+RDU_HD f3 ggxSampleAndEval(f3 wo, f3 base, float roughness, float metallic, const float * u, f3 & wi,
+	float * pdfOut = nullptr) {
 	const float probaSpecular = mixf(1.0f / (dot(base, splat(1.0f)) / 3.0f + 1.0f), 1.0f, metallic);
 	const float alpha = alphaFromRoughness(roughness);
 	if(u[0] < probaSpecular) {
@@ -337,9 +412,20 @@ RDU_HD f3 ggxSampleAndEval(f3 wo, f3 base, float roughness, float metallic, cons
 	float pdfSpec = 0.0f;
 	const f3 brdf = ggxBrdf(wo, base, alpha, metallic, wi, &pdfSpec);
 	const float pdf = mixf(kOneOverPi * gmax(wi.z, 0.0f), pdfSpec, probaSpecular);
+	if(pdfOut)
+		*pdfOut = pdf;
 	if(pdf == 0.0f)
 		return splat(0.0f);
 	return brdf / pdf;
+}
+// Extension: the mixture density of ggxSampleAndEval for a given direction (0 below the horizon).
+RDU_HD float ggxPdf(f3 wo, f3 base, float roughness, float metallic, f3 wi) {
+	if(wi.z < 0.0f)
+		return 0.0f;
+	const float probaSpecular = mixf(1.0f / (dot(base, splat(1.0f)) / 3.0f + 1.0f), 1.0f, metallic);
+	float pdfSpec = 0.0f;
+	(void)ggxBrdf(wo, base, alphaFromRoughness(roughness), metallic, wi, &pdfSpec);
+	return mixf(kOneOverPi * gmax(wi.z, 0.0f), pdfSpec, probaSpecular);
 }
 RDU_HD f3 ggxEval(f3 wo, f3 base, float roughness, float metallic, f3 wi) {
 	if(wi.z < 0.0f)

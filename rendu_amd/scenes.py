@@ -314,3 +314,35 @@ def background_probe(mode: int, level: float = 1.0):
         return cm
 
     return scn, cam
+
+
+def sky_spheres(seed: int = 5, sun: bool = False, point_light: bool = False, emissive: float = 0.0):
+    """Extended-estimator probe: a floor and 5 spheres (roughness 0.2 ... 1, one metal) under a
+    64 x 32 lat-long sky (horizon-to-zenith gradient, radiance below 1, plus a sun of radiance 25
+    when `sun`), optionally a point light and an emissive quad of radiance `emissive` overhead."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    scn = S.Scene()
+    m = scn.add_mesh(quad((-6, 0, 6), (6, 0, 6), (6, 0, -6), (-6, 0, -6)))
+    scn.objects.append(S.Object(m, _regular(scn, [0.5, 0.5, 0.5], 0.9, 0.0)))
+    sph = scn.add_mesh(icosphere(2, radius=1.0))
+    for k in range(5):
+        model = np.diag([0.6, 0.6, 0.6, 1.0]).astype(F32)
+        model[:3, 3] = (-2.4 + 1.2 * k, 0.6, float(rng.uniform(-1.0, 1.0)))
+        rough = (0.2, 0.4, 0.6, 0.8, 1.0)[k]
+        scn.objects.append(S.Object(sph, _regular(scn, rng.uniform(0.2, 0.8, 3), rough, 1.0 if k == 1 else 0.0), model))
+    if emissive > 0.0:
+        m = scn.add_mesh(quad((-1, 3, -1), (1, 3, -1), (1, 3, 1), (-1, 3, 1)))
+        scn.objects.append(S.Object(m, _emissive(scn, [emissive, 0.9 * emissive, 0.7 * emissive])))
+    sky = latlong_sky(64, 32, sun=(0.4, 0.6, -0.7), sun_radiance=25.0 if sun else 0.0, sun_angle=0.12)
+    sky[..., :3] *= 0.8
+    scn.background = S.Background(S.BG_LATLONG, texture=scn.add_texture(sky))
+    if point_light:
+        scn.lights.append(S.Light(S.LIGHT_POINT, (3.0, 3.0, 3.0), (1.5, 2.5, 1.5), radius=8.0))
+
+    def cam(width, height):
+        c = Camera()
+        c.pose([0.0, 2.2, 6.0], [0.0, 0.5, 0.0], [0.0, 1.0, 0.0])
+        c.projection(ratio=width / height, fov=0.9)
+        return c
+
+    return scn, cam
