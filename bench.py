@@ -96,7 +96,9 @@ def cache_experiment(pt, scn, cam, ref_spp):
     included in the wall time) at 16 / 32 / 64 spp, with the trust test and with every vertex from
     the termination bounce (2, 1, or the primary vertex) ending in the cache; also a cache warmed
     by one earlier 64-spp frame.
-    Reports the wall-time ratio of the fastest cached render whose RMSE reaches plain 64 spp."""
+    Reports the wall-time ratio of the fastest cached render whose RMSE reaches plain 64 spp, and
+    what the tail past bounce 2 holds (its share of the mean and of the rays, and the time of the
+    render truncated there, the floor for a cache that terminates paths at bounce 2)."""
     from rendu_amd import scene as S
     from rendu_amd.rcache import RadianceCache
     ext = S.ESTIMATOR_EXTENDED
@@ -108,12 +110,14 @@ def cache_experiment(pt, scn, cam, ref_spp):
     ref_s = time.perf_counter() - t0
     rmse = lambda a: float(np.sqrt(np.mean((a.astype(np.float64) - ref) ** 2)))
 
-    def plain(spp):
+    def plain(spp, depth=8):
         img = np.zeros((h, w, 3), np.float32)
         t = time.perf_counter()
-        pt.render(cam, spp, 8, img, seed=7, estimator=ext, rr_start=3)
+        pt.render(cam, spp, depth, img, seed=7, estimator=ext, rr_start=3)
+        st = pt.last_stats()
         return {"spp": spp, "s": round(time.perf_counter() - t, 4), "rmse": round(rmse(img), 6),
-                "mean": round(float(img.mean()), 5)}
+                "mean": round(float(img.mean()), 5),
+                "rays_per_sample": round((st["extension_rays"] + st["shadow_rays"]) / (w * h * spp), 4)}
 
     def cached(spp, cache=None, **kw):
         cache = cache or RadianceCache(lo, hi, seed=5)
@@ -129,6 +133,10 @@ def cache_experiment(pt, scn, cam, ref_spp):
     pt.render(cam, 1, 8, np.zeros((h, w, 3), np.float32), seed=1, estimator=ext, rr_start=3)  # warm-up
     p64 = plain(64)
     runs = {"plain": [plain(16), plain(32), p64]}
+    # The tracer without the network: paths end after bounce 2 with nothing added (depth 3). Its time
+    # bounds from below what any cache terminating at bounce 2 can reach at this spp.
+    trunc = plain(64, depth=3)
+    runs["plain_truncated_after_bounce2"] = [trunc]
     runs["cached"] = [cached(s) for s in (16, 32, 64)]
     runs["cached_always"] = [cached(s, always_trust=True) for s in (16, 32, 64)]
     runs["cached_always_b1"] = [cached(s, always_trust=True, terminate_bounce=1) for s in (16, 32, 64)]
@@ -143,6 +151,9 @@ def cache_experiment(pt, scn, cam, ref_spp):
                 best = (key, r)
     return {"ref_spp": ref_spp, "ref_s": round(ref_s, 3), "ref_mean": round(float(ref.mean()), 5), "runs": runs,
             "plain64_rmse": p64["rmse"], "plain64_s": p64["s"],
+            "tail_after_bounce2": {"share_of_mean": round(1.0 - trunc["mean"] / p64["mean"], 4),
+                                   "share_of_rays": round(1.0 - trunc["rays_per_sample"] / p64["rays_per_sample"], 4),
+                                   "time_ratio_truncated_over_plain64": round(trunc["s"] / p64["s"], 3)},
             "fastest_reaching_plain64_rmse": None if best is None else
             {"mode": best[0], **best[1], "time_ratio": round(best[1]["s"] / p64["s"], 3)}}
 

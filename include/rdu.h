@@ -389,7 +389,9 @@ void rdu_rcache_destroy(rdu_rcache* cache);
 int rdu_rcache_set_learning_rate(rdu_rcache* cache, float lr);
 /* queries: n x 8 floats; radiance: n x 4 floats out (rgb, unused). flags: RDU_DEVICE_POINTERS. */
 int rdu_rcache_infer(rdu_rcache* cache, const float* queries, size_t n, float* radiance, uint32_t flags, void* stream);
-/* steps optimisation steps on one batch of (query, target radiance) pairs (n x 8 and n x 4 floats). */
+/* steps optimisation steps on one batch of (query, target radiance) pairs (n x 8 and n x 4 floats):
+ * Adam (lr, beta1 0.9, beta2 0.99, eps 1e-15) on the MLP weights with an l2 term of 1e-6, and lazy Adam
+ * on the hash tables: only the entries the batch touches update their moments and weights. */
 int rdu_rcache_train(rdu_rcache* cache, const float* queries, const float* targets, size_t n, uint32_t steps,
 	uint32_t flags, void* stream, float* mean_loss);
 /* Gradients of the mean relative L2 loss for one batch, without an update (host arrays):
@@ -442,6 +444,26 @@ int rdu_scene_last_cache_stats(const rdu_scene* scene, rdu_cache_stats* out);
 /* The training pairs of the last pass of the last cached render (host arrays of `capacity` pairs:
  * queries 8 floats, targets 4 floats each); *count receives the number of pairs available. */
 int rdu_scene_last_cache_pairs(const rdu_scene* scene, float* queries, float* targets, size_t capacity, size_t* count);
+
+/* Tangent frames of a triangle mesh as Mesh::computeTangentsAndBitangents(force) makes them
+ * (Mesh.cpp:356-508), host code. positions and normals: n_vertices x 3 floats; texcoords:
+ * n_vertices x 2 floats, or NULL for the constant (0.5, 0.5) of the forced frame (Mesh.cpp:362-366);
+ * indices: n_indices (a multiple of 3) vertex indices, rewritten in place. Per triangle corner,
+ * MikkTSpace's tangent (genTangSpaceDefault of the mikktspace library, Mesh.cpp:386-394); then each
+ * vertex keeps its first corner's frame, a later corner whose frame differs from the corner before
+ * it gets a new vertex, appended after the existing ones. Outputs (capacity >= n_vertices +
+ * n_indices entries): source_vertex[v] = the input vertex that vertex v copies (identity for the
+ * first n_vertices), tangents and bitangents (3 floats per vertex, unit length, bitangent = -sign *
+ * cross(n, t) normalised); *out_vertices = the vertex count after the split. No triangle: every
+ * vertex gets tangent +x and bitangent +y (Mesh.cpp:388-392). */
+int rdu_mesh_tangent_frames(const float* positions, const float* normals, const float* texcoords, size_t n_vertices,
+	uint32_t* indices, size_t n_indices, uint32_t* source_vertex, float* tangents, float* bitangents, size_t capacity,
+	size_t* out_vertices);
+/* The MikkTSpace step alone (the callbacks of Mesh.cpp:25-91 over an indexed triangle list):
+ * corner_tangents receives n_indices x 4 floats, the tangent and the bitangent sign (+1 or -1) of
+ * every triangle corner, as the library's setTSpaceBasic callback reports them. */
+int rdu_mesh_corner_tangents(const float* positions, const float* normals, const float* texcoords, size_t n_vertices,
+	const uint32_t* indices, size_t n_indices, float* corner_tangents);
 
 /* Last error message of the calling thread ("" if none). */
 const char* rdu_last_error(void);

@@ -426,6 +426,32 @@ __global__ void rcAdam(float * w, uint16_t * w16, float * gsum, float * m, float
 	gsum[i] = 0.0f;
 }
 
+// Adam on the hash-table entries one step touched (the keys of the reduction, each with its summed
+// gradient pair), read straight from the reduction: entries no sample touched keep their moments
+// and weights. This is the lazy Adam of instant neural graphics primitives (Mueller et al. 2022,
+// section 4: table entries with a zero gradient are skipped), and it makes a step's cost follow the
+// batch instead of the 2^19 x 16 levels x 2 table parameters.
+__global__ void rcAdamTouched(const uint32_t * __restrict__ keys, const RcGradPair * __restrict__ sums,
+	const uint32_t * __restrict__ count, float * w, uint16_t * w16, float * m, float * v, float lr, float b1, float b2,
+	float eps, float c1, float c2) {
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= *count)
+		return;
+	const size_t k = keys[i];
+	const RcGradPair g = sums[i];
+	const float gr[2] = {g.a, g.b};
+	for(int j = 0; j < 2; ++j) {
+		const size_t e = 2 * k + j;
+		const float mi = b1 * m[e] + (1.0f - b1) * gr[j];
+		const float vi = b2 * v[e] + (1.0f - b2) * gr[j] * gr[j];
+		m[e] = mi;
+		v[e] = vi;
+		const float wi = w[e] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+		w[e] = wi;
+		w16[e] = f2h(wi);
+	}
+}
+
 template<bool BF16>
 __global__ void rcCopy(const float * w, uint16_t * w16, size_t n) {
 	const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -433,9 +459,10 @@ __global__ void rcCopy(const float * w, uint16_t * w16, size_t n) {
 		w16[i] = BF16 ? f2bf(w[i]) : f2h(w[i]);
 }
 
-// Sums the table-gradient records of n samples per entry into gt: radix sort by key (stable), then
-// a deterministic reduction by key, then one add per touched entry.
-int reduceTableGradients(RcParams & p, RcTrainBuffers & tb, uint32_t n, hipStream_t stream) {
+// Sums the table-gradient records of n samples per entry: radix sort by key (stable), then a
+// deterministic reduction by key into (tb.uniqKey, tb.uniqVal, *tb.uniqCount); with intoGt, one add
+// per touched entry into gt.
+int reduceTableGradients(RcParams & p, RcTrainBuffers & tb, uint32_t n, bool intoGt, hipStream_t stream) {
 	const uint32_t recs = n * kRcRecordsPerSample;
 	rocprim::double_buffer<uint32_t> keys(tb.recKey, tb.recKeyAlt);
 	rocprim::double_buffer<RcGradPair> vals(tb.recVal, tb.recValAlt);
@@ -448,6 +475,8 @@ int reduceTableGradients(RcParams & p, RcTrainBuffers & tb, uint32_t n, hipStrea
 		tb.uniqVal, tb.uniqCount, GradPairSum(), rocprim::equal_to<uint32_t>(), stream);
 	if(e != hipSuccess)
 		return int(e);
+	if(!intoGt)
+		return 0;
 	hipLaunchKernelGGL(rcApplyTableGrad, dim3((recs + 255) / 256), dim3(256), 0, stream, tb.uniqKey, tb.uniqVal,
 		tb.uniqCount, p.gt);
 	return int(hipGetLastError());
@@ -491,8 +520,10 @@ int rcInfer(const RcGrid & grid, const RcParams & p, const float4 * queries, con
 	return int(hipGetLastError());
 }
 
-int rcGradients(const RcGrid & grid, RcParams & p, RcTrainBuffers & tb, const float4 * queries, const float4 * targets,
-	uint32_t n, hipStream_t stream) {
+namespace {
+
+int gradientsImpl(const RcGrid & grid, RcParams & p, RcTrainBuffers & tb, const float4 * queries, const float4 * targets,
+	uint32_t n, bool intoGt, hipStream_t stream) {
 	if(n == 0 || n > tb.capacity)
 		return int(hipErrorInvalidValue);
 	const float lossScale = 1.0f / (3.0f * float(n));
@@ -501,14 +532,21 @@ int rcGradients(const RcGrid & grid, RcParams & p, RcTrainBuffers & tb, const fl
 	int e = int(hipGetLastError());
 	if(e)
 		return e;
-	if((e = reduceTableGradients(p, tb, n, stream)))
+	if((e = reduceTableGradients(p, tb, n, intoGt, stream)))
 		return e;
 	return weightGradients(p, tb, n, stream);
 }
 
+} // namespace
+
+int rcGradients(const RcGrid & grid, RcParams & p, RcTrainBuffers & tb, const float4 * queries, const float4 * targets,
+	uint32_t n, hipStream_t stream) {
+	return gradientsImpl(grid, p, tb, queries, targets, n, true, stream);
+}
+
 int rcTrainStep(const RcGrid & grid, RcParams & p, RcTrainBuffers & tb, RcAdam & adam, const float4 * queries,
 	const float4 * targets, uint32_t n, hipStream_t stream) {
-	int e = rcGradients(grid, p, tb, queries, targets, n, stream);
+	int e = gradientsImpl(grid, p, tb, queries, targets, n, false, stream);
 	if(e)
 		return e;
 	adam.step += 1;
@@ -516,8 +554,9 @@ int rcTrainStep(const RcGrid & grid, RcParams & p, RcTrainBuffers & tb, RcAdam &
 	const float c2 = 1.0f - powf(adam.beta2, float(adam.step));
 	hipLaunchKernelGGL((rcAdam<true>), dim3((kRcMlpParams + 255) / 256), dim3(256), 0, stream, p.w32, p.w16, p.gw, p.mw,
 		p.vw, size_t(kRcMlpParams), adam.lr, adam.beta1, adam.beta2, adam.eps, adam.l2, c1, c2);
-	hipLaunchKernelGGL((rcAdam<false>), dim3(unsigned((kRcTableParams + 255) / 256)), dim3(256), 0, stream, p.t32, p.t16,
-		p.gt, p.mt, p.vt, kRcTableParams, adam.lr, adam.beta1, adam.beta2, adam.eps, 0.0f, c1, c2);
+	const uint32_t recs = n * kRcRecordsPerSample;
+	hipLaunchKernelGGL(rcAdamTouched, dim3((recs + 255) / 256), dim3(256), 0, stream, tb.uniqKey, tb.uniqVal, tb.uniqCount,
+		p.t32, p.t16, p.mt, p.vt, adam.lr, adam.beta1, adam.beta2, adam.eps, c1, c2);
 	return int(hipGetLastError());
 }
 

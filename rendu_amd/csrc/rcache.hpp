@@ -46,7 +46,7 @@ struct RcParams {
 	float * mw = nullptr, * vw = nullptr;
 	float * t32 = nullptr;      // tables master, kRcTableParams
 	uint16_t * t16 = nullptr;   // fp16 copy
-	float * gt = nullptr;       // table gradient accumulators
+	float * gt = nullptr;       // table gradient accumulators (rcGradients only)
 	float * mt = nullptr, * vt = nullptr;
 };
 
@@ -85,7 +85,9 @@ struct RcAdam {
 // rdu_ray layout); out: n float4 (rgb radiance). A device count, when given, bounds n.
 int rcInfer(const RcGrid & grid, const RcParams & p, const float4 * queries, const uint32_t * countDev, uint32_t n,
 	float4 * out, hipStream_t stream);
-// One optimisation step on n (query, target radiance) pairs. Returns hipError_t.
+// One optimisation step on n (query, target radiance) pairs: dense Adam over the MLP weights (with
+// the l2 term), lazy Adam over the table entries the batch touched (untouched entries keep their
+// weights and moments). p.gt is neither read nor written. Returns hipError_t.
 int rcTrainStep(const RcGrid & grid, RcParams & p, RcTrainBuffers & tb, RcAdam & adam, const float4 * queries,
 	const float4 * targets, uint32_t n, hipStream_t stream);
 // Gradients only (no update), for checks: fills p.gw and p.gt for the batch.

@@ -5,7 +5,8 @@ The cache mode has no Rendu counterpart. What pins it:
 * a frozen cache that predicts exactly zero and is trusted everywhere turns the render into the
   plain render truncated after bounce 2 (depth 3) in a scene without analytic lights, bit for bit;
 * the prediction enters linearly: image(2 L) - image(0) = 2 (image(L) - image(0));
-* trained online on a Cornell box, the cached render stays close to a converged plain render.
+* trained online on a Cornell box, the cached render has less error per sample than the plain
+  render (the single-frame scene is covered by test_gpu_cache_single_frame.py).
 """
 import numpy as np
 import pytest
@@ -93,13 +94,16 @@ def test_prediction_enters_linearly():
     assert ok.mean() >= 0.99, ok.mean()
 
 
-def test_trained_cache_close_to_converged_render():
-    """Online training on a Cornell box (128 x 128, 8 passes of 8 spp per render, two warm-up
-    renders): against a 1024-spp plain render, the cached render (trust test on) is neither darker
-    or brighter by more than 3% nor noisier than the plain render at the same spp by more than 5%;
-    with every vertex ending in the cache (no trust test) the bias stays under 5%."""
+def test_trained_cache_gains_per_sample_on_the_cornell_box():
+    """Online training on a Cornell box, where most light is indirect (128 x 128, depth 8, 8 passes
+    of 8 spp per render; per mode a fresh cache and two warm-up renders): against a 1024-spp plain
+    render, at the same 64 spp, the cached render beats the plain render's RMSE in every mode, and
+    the more path ends the cache takes the larger the gain (measured: 0.98x with the trust test,
+    0.94x with every bounce-2 vertex ending in the cache, 0.83x from bounce 1, i.e. 1.45x less
+    variance per sample), while the mean stays within 1% of the converged render (2% when every
+    bounce-1 vertex ends in the cache; measured 0.3-0.9%)."""
     scn, cam_of, lo, hi = _cornell(subdiv=3)
-    log_case("cached Cornell 128x128 vs 1024-spp plain")
+    log_case("cached Cornell 128x128 vs 1024-spp plain, three cache modes")
     pt = PathTracer(scn)
     w = h = 128
     mp = w * h * 8
@@ -107,20 +111,22 @@ def test_trained_cache_close_to_converged_render():
     pt.render(cam_of(w, h), 1024, 8, ref, seed=100, estimator=EXT, rr_start=3)
     plain = np.zeros_like(ref)
     pt.render(cam_of(w, h), 64, 8, plain, seed=1, estimator=EXT, rr_start=3, max_paths=mp)
-    cache = RadianceCache(lo, hi, seed=3)
-    for warm_seed in (2, 3):
-        warm = np.zeros_like(ref)
-        pt.render_cached(cache, cam_of(w, h), 64, 8, warm, seed=warm_seed, estimator=EXT, rr_start=3, max_paths=mp)
-    got = np.zeros_like(ref)
-    st = pt.render_cached(cache, cam_of(w, h), 64, 8, got, seed=1, estimator=EXT, rr_start=3, max_paths=mp)
-    allc = np.zeros_like(ref)
-    st2 = pt.render_cached(cache, cam_of(w, h), 64, 8, allc, seed=1, estimator=EXT, rr_start=3, max_paths=mp,
-                           always_trust=True)
     rmse = lambda a: float(np.sqrt(np.mean((a - ref) ** 2)))
     bias = lambda a: float(abs(a.mean() - ref.mean()) / ref.mean())
-    print(f"[cornell cache] trust: bias {bias(got):.4f} rmse {rmse(got):.5f} queries {st['cache_queries']} "
-          f"trusted {st['trusted_cells']} | always: bias {bias(allc):.4f} rmse {rmse(allc):.5f} "
-          f"queries {st2['cache_queries']} | plain rmse {rmse(plain):.5f} pairs {st['train_pairs']}", flush=True)
-    assert st["train_pairs"] > 0 and st2["cache_queries"] > 0
-    assert bias(got) < 0.03 and rmse(got) < 1.05 * rmse(plain)
-    assert bias(allc) < 0.05
+    ratios = {}
+    for name, kw, bound, max_bias in (("trust test, bounce 2", {}, 1.0, 0.01),
+                                      ("always, bounce 2", {"always_trust": True}, 0.97, 0.01),
+                                      ("always, bounce 1", {"always_trust": True, "terminate_bounce": 1}, 0.9, 0.02)):
+        cache = RadianceCache(lo, hi, seed=3)
+        for warm_seed in (2, 3):
+            pt.render_cached(cache, cam_of(w, h), 64, 8, np.zeros_like(ref), seed=warm_seed, estimator=EXT, rr_start=3,
+                             max_paths=mp, **kw)
+        got = np.zeros_like(ref)
+        st = pt.render_cached(cache, cam_of(w, h), 64, 8, got, seed=1, estimator=EXT, rr_start=3, max_paths=mp, **kw)
+        ratios[name] = rmse(got) / rmse(plain)
+        print(f"[cornell cache] {name}: rmse {rmse(got):.5f} (plain {rmse(plain):.5f}, ratio {ratios[name]:.3f}) "
+              f"bias {bias(got):.4f} queries {st['cache_queries']} pairs {st['train_pairs']}", flush=True)
+        assert st["train_pairs"] > 0 and st["cache_queries"] > 0
+        assert bias(got) < max_bias, (name, bias(got))
+        assert ratios[name] < bound, (name, ratios[name])
+    assert ratios["always, bounce 1"] < ratios["always, bounce 2"] < ratios["trust test, bounce 2"]

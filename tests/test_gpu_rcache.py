@@ -197,3 +197,60 @@ def test_training_fits_smooth_field(log_case):
     err1 = np.mean((c.infer(test_pos, test_dir) - test_tgt) ** 2)
     assert losses[-1] < 0.05 * losses[0], (losses[0], losses[-1])
     assert err1 < 0.01 * err0 and err1 < 1e-3, (err0, err1)
+
+
+def _adam(w, g, m, v, step, lr, b1=0.9, b2=0.99, eps=1e-15):
+    """One Adam step (Kingma and Ba 2015, algorithm 1) in float64; returns the new (w, m, v)."""
+    m = b1 * m + (1 - b1) * g
+    v = b2 * v + (1 - b2) * g * g
+    return w - lr * (m / (1 - b1 ** step)) / (np.sqrt(v / (1 - b2 ** step)) + eps), m, v
+
+
+def test_adam_is_lazy_on_table_entries(log_case):
+    """Two training steps on batches that touch different table entries: the MLP takes dense Adam
+    with its l2 term, the tables take Adam only where the step's batch lands (an entry touched by the
+    first batch and not the second keeps its weight exactly; dense Adam would move it by its decayed
+    first moment)."""
+    log_case("rcache lazy table Adam, two steps, n=2000 then a sub-box of 1000")
+    c = _cache(8)
+    lr, l2 = 3e-3, 1e-6
+    c.set_learning_rate(lr)
+    _set_random(c, 41, table_scale=0.5)
+    p0m, p0t, _ = c.get_params()
+    rng = np.random.default_rng(42)
+    pos_a, dir_a = _positions(2000, 43), _directions(2000, 44)
+    tgt_a = rng.random((2000, 3)).astype(np.float32)
+    sub = HI - 0.75 * (HI - LO)
+    pos_b = (LO + np.random.default_rng(45).random((1000, 3)) * (sub - LO)).astype(np.float32)
+    dir_b, tgt_b = _directions(1000, 46), rng.random((1000, 3)).astype(np.float32)
+
+    ga_m, ga_t = c.gradients(pos_a, tgt_a, dir_a)
+    c.train(pos_a, tgt_a, dir_a)
+    p1m, p1t, _ = c.get_params()
+    gb_m, gb_t = c.gradients(pos_b, tgt_b, dir_b)
+    c.train(pos_b, tgt_b, dir_b)
+    p2m, p2t, _ = c.get_params()
+
+    w, g = p0t.astype(np.float64), ga_t.astype(np.float64)
+    ta, tb = ga_t != 0, gb_t != 0
+    e1, m1, v1 = _adam(w, g, 0.0, 0.0, 1, lr)
+    assert np.array_equal(p1t[~ta], p0t[~ta])
+    assert np.allclose(p1t[ta], e1[ta], rtol=0, atol=2e-6)
+    w, g = p1t.astype(np.float64), gb_t.astype(np.float64)
+    m1, v1 = np.where(ta, m1, 0.0), np.where(ta, v1, 0.0)
+    e2, _, _ = _adam(w, g, m1, v1, 2, lr)
+    only_a = ta & ~tb
+    assert only_a.sum() > 1000 and tb.sum() > 1000
+    assert np.array_equal(p2t[~tb], p1t[~tb])
+    assert np.allclose(p2t[tb], e2[tb], rtol=0, atol=2e-6)
+    dense, _, _ = _adam(w, g, m1, v1, 2, lr)
+    assert np.abs(dense[only_a] - p1t[only_a]).max() > 1e-4  # what dense Adam would have done
+
+    # MLP: dense, gradient plus l2 * w; its weight gradients are summed with atomics, so a few
+    # near-zero components may round to the other sign between the two passes.
+    live = np.ones(N_MLP, bool)
+    live[-(16 - 3) * 64:] = False
+    e1m, mm, vm = _adam(p0m.astype(np.float64), ga_m + l2 * p0m.astype(np.float64), 0.0, 0.0, 1, lr)
+    e2m, _, _ = _adam(p1m.astype(np.float64), gb_m + l2 * p1m.astype(np.float64), mm, vm, 2, lr)
+    for got, exp in ((p1m, e1m), (p2m, e2m)):
+        assert np.mean(np.abs(got[live] - exp[live]) > 1e-3 * lr) < 1e-3
