@@ -18,14 +18,17 @@ Host-side restatement of what the path-tracer app does before PathTracer::render
   (Image::loadHDR); cube faces are flipped vertically;
 * the atmosphere background ('sun'): Rendu's precomputed scattering table as its texture.
 
-Differences: tangent frames come from per-triangle UV derivatives accumulated per vertex (Rendu
-runs MikkTSpace); they matter only for normal-mapped materials. PNG (8-bit, non-interlaced) and
-uncompressed EXR are decoded here; JPEG goes through Pillow's decoder (libjpeg), whose inverse DCT
+* Mesh::computeTangentsAndBitangents(force) (Mesh.cpp:356-508), forced as the path-tracer app
+  loads its scenes (pathtracer/main.cpp:90): MikkTSpace corner frames and Rendu's vertex split, in
+  the library (rdu_mesh_tangent_frames).
+
+Differences: PNG (8-bit, non-interlaced) and uncompressed EXR are decoded here; JPEG goes through Pillow's decoder (libjpeg), whose inverse DCT
 and chroma upsampling can differ from stb_image's (Rendu's) by a level or two in some texels.
 Animations are not supported (scenes load in their initial state).
 """
 from __future__ import annotations
 
+import ctypes
 import math
 import os
 from dataclasses import dataclass, field
@@ -208,31 +211,34 @@ def compute_normals(mesh: MeshData) -> None:
 
 
 def compute_tangents(mesh: MeshData) -> None:
-    """Per-vertex tangent frames from per-triangle UV derivatives (constant UVs 0.5 when absent),
-    Gram-Schmidt against the normal; bitangent = cross(normal, tangent) with the UV handedness."""
-    P = mesh.positions.astype(np.float64)
-    N = mesh.normals.astype(np.float64)
-    uv = mesh.texcoords.astype(np.float64) if mesh.texcoords is not None else np.full((P.shape[0], 2), 0.5)
-    I = mesh.indices.reshape(-1, 3)
-    e1, e2 = P[I[:, 1]] - P[I[:, 0]], P[I[:, 2]] - P[I[:, 0]]
-    d1, d2 = uv[I[:, 1]] - uv[I[:, 0]], uv[I[:, 2]] - uv[I[:, 0]]
-    det = d1[:, 0] * d2[:, 1] - d2[:, 0] * d1[:, 1]
-    r = np.where(np.abs(det) > 1e-20, 1.0 / np.where(det == 0, 1, det), 0.0)[:, None]
-    t = (e1 * d2[:, 1:2] - e2 * d1[:, 1:2]) * r
-    b = (e2 * d1[:, 0:1] - e1 * d2[:, 0:1]) * r
-    T = np.zeros_like(P)
-    B = np.zeros_like(P)
-    for k in range(3):
-        np.add.at(T, I[:, k], t)
-        np.add.at(B, I[:, k], b)
-    T = T - N * (T * N).sum(axis=1, keepdims=True)
-    ln = np.linalg.norm(T, axis=1, keepdims=True)
-    fallback = np.where(np.abs(N[:, :1]) < 0.9, np.array([[1.0, 0, 0]]), np.array([[0, 1.0, 0]]))
-    fb = fallback - N * (fallback * N).sum(axis=1, keepdims=True)
-    T = np.where(ln > 1e-12, T / np.maximum(ln, 1e-30), fb / np.linalg.norm(fb, axis=1, keepdims=True))
-    hand = np.where((np.cross(N, T) * B).sum(axis=1, keepdims=True) < 0, -1.0, 1.0)
-    mesh.tangents = T.astype(F32)
-    mesh.bitangents = (np.cross(N, T) * hand).astype(F32)
+    """Mesh::computeTangentsAndBitangents(force = true) (Mesh.cpp:356-508) through the library's
+    rdu_mesh_tangent_frames: MikkTSpace corner frames, then Rendu's vertex split (positions, normals
+    and texture coordinates of split vertices are copies). A mesh without texture coordinates gets
+    frames from the constant (0.5, 0.5) and keeps no texture coordinates (its objects do not use
+    them, Object.cpp:7)."""
+    from . import _lib
+    P = np.ascontiguousarray(mesh.positions, F32).reshape(-1, 3)
+    N = np.ascontiguousarray(mesh.normals, F32).reshape(-1, 3)
+    T = None if mesh.texcoords is None else np.ascontiguousarray(mesh.texcoords, F32).reshape(-1, 2)
+    I = np.ascontiguousarray(mesh.indices, np.uint32).reshape(-1).copy()
+    nv, ni = P.shape[0], I.size
+    cap = nv + ni
+    src = np.zeros(cap, np.uint32)
+    tan = np.zeros((cap, 3), F32)
+    bit = np.zeros((cap, 3), F32)
+    count = ctypes.c_size_t(0)
+    _lib.check(_lib.lib.rdu_mesh_tangent_frames(
+        P.ctypes.data, N.ctypes.data, None if T is None else T.ctypes.data, nv, I.ctypes.data, ni, src.ctypes.data,
+        tan.ctypes.data, bit.ctypes.data, cap, ctypes.byref(count)))
+    n = count.value
+    src = src[:n]
+    mesh.positions = P[src]
+    mesh.normals = N[src]
+    if T is not None:
+        mesh.texcoords = T[src]
+    mesh.indices = I
+    mesh.tangents = tan[:n].copy()
+    mesh.bitangents = bit[:n].copy()
 
 
 class Resources:

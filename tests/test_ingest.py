@@ -63,8 +63,9 @@ def test_compute_normals_and_tangent_frame():
     np.testing.assert_allclose(m.normals, [[0, 0, 1]] * 3, atol=1e-7)
     ingest.compute_tangents(m)
     np.testing.assert_allclose(m.tangents, [[1, 0, 0]] * 3, atol=1e-6)
-    # v is flipped on load (v' = 1 - v): the bitangent follows increasing v', here -y.
-    np.testing.assert_allclose(m.bitangents, [[0, -1, 0]] * 3, atol=1e-6)
+    # v is flipped on load (v' = 1 - v): the texture area is negative, MikkTSpace's sign -1, and
+    # Rendu's bitangent -sign * cross(n, t) (Mesh.cpp:465) is +y.
+    np.testing.assert_allclose(m.bitangents, [[0, 1, 0]] * 3, atol=1e-6)
 
 
 def test_transformation_trs():
