@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--profile-frame", action="store_true",
                     help="only the single frame (1920x1080, 64 spp, depth 8), plain then with a fresh radiance "
                          "cache: the workload of the path-tracer kernel profiles")
+    ap.add_argument("--jobs-only", action="store_true",
+                    help="only the 64 batched jobs: a warm-up batch, a 1.5 s pause, then the timed batch with "
+                         "2 build and 6 render threads (the workload of the job-batch kernel trace)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device three times: the reference reading of the "
                          "FETCH_SIZE / WRITE_SIZE counters that converts them to bytes")
@@ -88,6 +91,25 @@ def profile_frame():
     t2 = time.perf_counter()
     print(json.dumps({"plain_s": round(t1 - t0, 4), "cached_s": round(t2 - t1, 4), "paths": st["paths"],
                       "cache_queries": cst["cache_queries"], "train_steps": cst["train_steps"]}), flush=True)
+
+
+def job_list_64():
+    from rendu_amd import jobs as J
+    from rendu_amd import scenes
+    out = []
+    for s in range(64):
+        scn_j, cam_j, w_j, h_j, spp_j, depth_j = scenes.random_job(s)
+        out.append(J.Job(scn_j, cam_j(w_j, h_j), w_j, h_j, spp_j, depth_j, seed=s))
+    return out
+
+
+def jobs_only():
+    from rendu_amd import jobs as J
+    job_list = job_list_64()
+    J.run_batch(job_list, workers=8, order="lpt")
+    time.sleep(1.5)
+    res, wall = J.run_batch(job_list, workers=6, order="lpt", build_workers=2)
+    print(json.dumps(dict(J.summary(res, wall), workers=6, build_workers=2, order="longest first")), flush=True)
 
 
 def cache_experiment(pt, scn, cam, ref_spp):
@@ -311,6 +333,9 @@ def main():
         return
     if args.profile_frame:
         profile_frame()
+        return
+    if args.jobs_only:
+        jobs_only()
         return
 
     import torch
@@ -538,10 +563,7 @@ def main():
     # Batched jobs: 64 seeded scenes (1e4..5e5 triangles, 128^2..1024^2 pixels, 4..256 spp, depth 6),
     # each built (host BVH + upload) and rendered; one HIP stream per scene.
     from rendu_amd import jobs as J
-    job_list = []
-    for s in range(64):
-        scn_j, cam_j, w_j, h_j, spp_j, depth_j = scenes.random_job(s)
-        job_list.append(J.Job(scn_j, cam_j(w_j, h_j), w_j, h_j, spp_j, depth_j, seed=s))
+    job_list = job_list_64()
     # Throughput configuration: longest estimated job first, host builds in a pool of 2 threads
     # running ahead of 6 render threads. Reported beside it: builds inside each job's own thread
     # (8 threads; lower p50 latency), and 4 threads in list order (lower p99 latency).
