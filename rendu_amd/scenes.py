@@ -190,16 +190,20 @@ def terrain_spheres(seed: int = 11, grid: int = 512, spheres: int = 2000, subdiv
 
 def random_job(seed: int):
     """One job of the batched workload: (scene, camera factory, width, height, spp, depth) with a
-    triangle count log-uniform on [1e4, 5e5] (a ground quad plus icospheres at subdivision 3, 1280
-    triangles each, one shared mesh placed by model matrix), a square image whose pixel count is
-    log-uniform on [128^2, 1024^2], spp uniform on {4..256} and depth 6. Materials as in
-    terrain_spheres; a 256 x 128 lat-long sky."""
+    triangle count log-uniform on [1e4, 5e5], split as in the single frame: a fBm heightfield
+    terrain of about a fifth of the triangles (2 (g-1)^2 of them) and icospheres at subdivision 3
+    (1280 triangles each, one shared mesh placed by model matrix) resting on it for the rest; a
+    square image whose pixel count is log-uniform on [128^2, 1024^2], spp uniform on {4..256} and
+    depth 6. Materials as in terrain_spheres; a 256 x 128 lat-long sky."""
     rng = np.random.Generator(np.random.PCG64(seed))
     tris = float(np.exp(rng.uniform(np.log(1e4), np.log(5e5))))
     side = int(round(np.exp(rng.uniform(np.log(128.0), np.log(1024.0)))))
     spp = int(rng.integers(4, 257))
-    n_sph = max(1, int(round((tris - 2) / 1280.0)))
-    extent = 4.0 * np.sqrt(n_sph)
+    g = max(8, int(round(np.sqrt(0.2 * tris / 2.0))) + 1)
+    n_sph = max(1, int(round((tris - 2 * (g - 1) ** 2) / 1280.0)))
+    size = 8.0 * np.sqrt(n_sph) + 8.0
+    amp = 0.12 * size
+    heights = fbm_heights(g, seed + 1000)
     scn = S.Scene()
     t_flat = scn.add_texture(S.constant_texture([0.5, 0.5, 1.0, 1.0]))
 
@@ -210,18 +214,20 @@ def random_job(seed: int):
         t2 = scn.add_texture(S.constant_texture([rough, metal, 1.0, 1.0]))
         return scn.add_material(S.Material(S.MATERIAL_REGULAR, (t0, t_flat, t2), False))
 
-    e = extent
-    m = scn.add_mesh(quad((-e, 0, e), (e, 0, e), (e, 0, -e), (-e, 0, -e)))
-    scn.objects.append(S.Object(m, material([0.5, 0.5, 0.5], 0.8, 0.0)))
+    m = scn.add_mesh(heightfield_mesh(heights, size, amp))
+    scn.objects.append(S.Object(m, material([0.45, 0.4, 0.3], 0.9, 0.0)))
     sph = scn.add_mesh(icosphere(3, radius=1.0))
     for k in range(n_sph):
         r = float(rng.uniform(0.4, 1.2))
-        x, z = rng.uniform(-0.8 * e, 0.8 * e, 2)
+        x, z = rng.uniform(-0.42 * size, 0.42 * size, 2)
+        i = int(round((z / size + 0.5) * (g - 1)))
+        jj = int(round((x / size + 0.5) * (g - 1)))
         model = np.diag([r, r, r, 1.0]).astype(F32)
-        model[:3, 3] = (x, r, z)
+        model[:3, 3] = (x, float(heights[i, jj] * amp) + 0.7 * r, z)
         scn.objects.append(S.Object(sph, material(rng.uniform(0.1, 0.9, 3), float(rng.uniform(0.05, 1.0)),
                                                   float(rng.random() < 0.3)), model))
     scn.background = S.Background(S.BG_LATLONG, texture=scn.add_texture(latlong_sky(256, 128)))
+    e = 0.5 * size
 
     def cam(width, height):
         c = Camera()
