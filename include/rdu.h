@@ -240,7 +240,8 @@ typedef struct {
 	uint32_t seed;       /* key of the counter-based random stream */
 	int32_t estimator;   /* RDU_ESTIMATOR_* */
 	int32_t rr_start;    /* extended: first bounce subject to Russian roulette */
-	uint32_t max_paths;  /* paths in flight per pass (0 = library default) */
+	uint32_t max_paths;  /* paths in flight per pass (0 = library default: 2^25 for plain renders,
+	                      * 2^22 for cached ones; the image does not depend on it) */
 } rdu_render_opts;
 
 typedef struct rdu_scene rdu_scene;

@@ -57,6 +57,26 @@ def test_cornell_point_light_nee_and_passes():
     assert st["shadow_rays"] > 0
 
 
+def test_pass_split_does_not_change_the_image():
+    """Samples accumulate per pixel in sample order whatever the pass split, so one pass of all
+    samples (the default pass size), one sample per pass and uneven passes give the same bits
+    (extended estimator, point light and Russian roulette: every kernel of a pass)."""
+    from rendu_amd.pathtracer import PathTracer
+    scn, cam_of = scenes.cornell_spheres(seed=4, subdivisions=2, point_light=True)
+    log_case("pass split: 1, 16 and 6 passes of a 64x48 16-spp render")
+    pt = PathTracer(scn)
+    w, h = 64, 48
+    imgs, passes = [], []
+    for mp in (0, w * h, w * h * 3):
+        img = np.zeros((h, w, 3), np.float32)
+        st = pt.render(cam_of(w, h), 16, 5, img, seed=9, estimator=S.ESTIMATOR_EXTENDED, rr_start=2, max_paths=mp)
+        imgs.append(img)
+        passes.append(st["passes"])
+    assert passes == [1, 16, 6]
+    assert imgs[0].mean() > 0.01
+    assert np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[0], imgs[2])
+
+
 def feature_scene(bg_mode):
     """Masked cut-outs with UVs, normal maps, a transformed object, spot + directional + point lights."""
     rng = np.random.default_rng(7)
