@@ -206,3 +206,32 @@ def test_terrain_spheres_full_geometry():
     got, exp, st = render_both(scn, cam_of(96, 54), 96, 54, 4, 8, seed=2, estimator=S.ESTIMATOR_EXTENDED, rr_start=3)
     assert_close(got, exp)
     assert st["paths"] == 96 * 54 * 4
+
+
+def test_single_frame_at_full_resolution():
+    """The single frame at its configured size, 1920x1080 over the full scene: 2 spp sample for
+    sample against the oracle (about 4M paths on the host cores), then the configured 64 spp on
+    the GPU, where size-independent properties hold: the same bits from 4 passes of 2^25 paths and
+    from 32 passes of 2^22, every pixel finite and non-negative, and a mean within 5 standard errors
+    of the mean of the 2-spp oracle render (the two are independent estimates of one image)."""
+    scn, cam_of = scenes.terrain_spheres()
+    w, h = 1920, 1080
+    cam = cam_of(w, h)
+    log_case("single frame 1920x1080 2spp depth8 extended vs oracle")
+    got, exp, st = render_both(scn, cam, w, h, 2, 8, seed=4, estimator=S.ESTIMATOR_EXTENDED, rr_start=3)
+    assert_close(got, exp)
+    assert st["paths"] == w * h * 2
+    log_case("single frame 1920x1080 64spp depth8 extended: 4 vs 32 passes")
+    pt = PathTracer(scn)
+    imgs = []
+    for mp in (0, 1 << 22):
+        img = np.zeros((h, w, 3), F32)
+        st = pt.render(cam, 64, 8, img, seed=11, estimator=S.ESTIMATOR_EXTENDED, rr_start=3, max_paths=mp)
+        imgs.append(img)
+        assert st["passes"] == (4 if mp == 0 else 32)
+    assert np.array_equal(imgs[0], imgs[1])
+    full = imgs[0].astype(np.float64)
+    assert np.all(np.isfinite(full)) and full.min() >= 0.0
+    # Standard error of the 2-spp image mean, from its per-pixel spread (pixels are independent).
+    se = exp.astype(np.float64).std() / np.sqrt(exp.size)
+    assert abs(full.mean() - exp.mean()) < 5 * se + 1e-4, (full.mean(), exp.mean(), se)
