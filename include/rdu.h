@@ -297,6 +297,30 @@ typedef struct {
 } rdu_render_stats;
 int rdu_scene_last_render_stats(const rdu_scene* scene, rdu_render_stats* out);
 
+/* One job of a batch render: a committed scene, its camera (a description, or the frame Rendu's
+ * render reads; exactly one of the two non-NULL), samples, depth, and the caller's image as in
+ * rdu_pathtracer_render (accumulated into, then divided by the power-of-two sample count). */
+typedef struct {
+	rdu_scene* scene;
+	const rdu_camera_desc* camera;
+	const rdu_camera_frame* frame;
+	size_t samples, depth;
+	float* image;
+	uint32_t width, height, components;
+	rdu_render_opts opts; /* seed, estimator, rr_start; max_paths is ignored */
+} rdu_batch_job;
+
+/* Batched render jobs (no reference counterpart): every job of the list in one set of wavefront
+ * launches per bounce. Each pass takes samples of several jobs (longest jobs first) up to max_paths
+ * paths (0 = 2^25); their trees are copied into one node arena, rays start at their own job's
+ * root, shading reads each wavefront's job descriptor once. Every image equals, bit for bit, what
+ * rdu_pathtracer_render gives that scene alone with the same options. job_seconds (n entries, or
+ * NULL) receives each job's completion time from the call's start; stats (or NULL) the totals
+ * (seconds, paths, rays, passes). The scenes must live on one device and not be rendered
+ * elsewhere during the call. */
+int rdu_pathtracer_render_batch(const rdu_batch_job* jobs, size_t n, uint32_t max_paths, double* job_seconds,
+	rdu_render_stats* stats);
+
 /* HIP graph replay of render passes (no reference counterpart). When enabled, a plain render's
  * full passes (max_paths paths each; a shorter last pass runs eagerly) are captured once into a
  * HIP graph and replayed: a pass whose launch configuration (camera frame, size, depth,

@@ -114,6 +114,7 @@ def _load() -> ctypes.CDLL:
                                                  c_uint32, c_uint32, c_uint32, c_void_p, c_void_p]),
         "rdu_scene_last_cache_stats": (c_int, [c_void_p, c_void_p]),
         "rdu_scene_last_cache_pairs": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, POINTER(c_size_t)]),
+        "rdu_pathtracer_render_batch": (c_int, [c_void_p, c_size_t, c_uint32, c_void_p, c_void_p]),
         "rdu_mesh_tangent_frames": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p,
                                             c_void_p, c_void_p, c_size_t, POINTER(c_size_t)]),
         "rdu_mesh_corner_tangents": (c_int, [c_void_p, c_void_p, c_void_p, c_size_t, c_void_p, c_size_t, c_void_p]),

@@ -23,6 +23,7 @@ struct RenderPaths {
 	int device = 0;
 	DevBuffer pRays, pHits, pAtten, pColor, pQ0, pQ1, pCounts, pTotals, pSRays, pSContrib, pSQ0, pSQ1, pOcc, pSHits,
 		pGstack, pImage, pMis;
+	DevBuffer pChunk, pJob; // batch passes: each path's chunk and job (grown by the first batch render)
 	size_t pathCap = 0;             // paths every path buffer above holds (0 while a growth is unfinished)
 	uint32_t * walkLeft = nullptr;  // pinned host word for the shadow walk's count (scenes with cut-outs)
 };
@@ -190,7 +191,11 @@ int uploadVec(DevBuffer & b, const std::vector<T> & v) {
 // set's previous render drained its own stream before the lease, so the old blocks are idle. While
 // a growth is unfinished pathCap is 0, so a set whose growth failed half-way (some buffers freed,
 // some small) is regrown in full by the next render that leases it instead of being used as is.
+int ensurePathsFor(RenderPaths & R, const DevBVH & bvh, bool masked, size_t P, bool batch, hipStream_t stream);
 int ensurePaths(RenderPaths & R, const rdu_scene * s, size_t P, hipStream_t stream) {
+	return ensurePathsFor(R, s->rc->dev, s->dev.anyMasked != 0, P, false, stream);
+}
+int ensurePathsFor(RenderPaths & R, const DevBVH & bvh, bool masked, size_t P, bool batch, hipStream_t stream) {
 	int st;
 	if(P > R.pathCap) {
 		R.pathCap = 0;
@@ -205,13 +210,16 @@ int ensurePaths(RenderPaths & R, const rdu_scene * s, size_t P, hipStream_t stre
 			return st;
 		R.pathCap = P;
 	}
-	if(!R.walkLeft && s->dev.anyMasked &&
+	if(batch && (R.pChunk.bytes < P * 4 || R.pJob.bytes < P * 4) &&
+		((st = R.pChunk.reserveOn(P * 4, stream)) || (st = R.pJob.reserveOn(P * 4, stream))))
+		return st;
+	if(!R.walkLeft && masked &&
 		(st = hipCheck(hipHostMalloc(reinterpret_cast<void **>(&R.walkLeft), sizeof(uint32_t), hipHostMallocDefault),
 			 "hipHostMalloc")))
 		return st;
 	TraceOptions o;
 	o.wide = true;
-	const size_t g = globalStackInts(s->rc->dev, o, P);
+	const size_t g = globalStackInts(bvh, o, P);
 	if(g && (st = R.pGstack.reserveOn(g * sizeof(int), stream)))
 		return st;
 	return RDU_OK;
@@ -653,30 +661,10 @@ const rdu_raycaster * rdu_scene_raycaster(const rdu_scene * s) { return s ? s->r
 // tuneOut: instead of rendering, time one pass per candidate of the traversal launch parameters
 // (image unused) and keep the fastest for the scene.
 // camFrame (when not null) gives the camera frame directly instead of `cam`'s pose inputs.
-static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples, size_t depth, float * image,
-	uint32_t width, uint32_t height, uint32_t components, const rdu_render_opts * opts, rdu_rcache * cache,
-	const rdu_cache_opts * copts, TuneResult * tuneOut = nullptr, const rdu_camera_frame * camFrame = nullptr) {
-	clearError();
-	const auto t0 = std::chrono::steady_clock::now();
-	if(!s)
-		return fail(RDU_ERR_INVALID, "[PathTracer] No scene available.");
-	if(!s->committed)
-		return fail(RDU_ERR_STATE, "scene not committed");
-	if((!cam && !camFrame) || (!image && !tuneOut))
-		return fail(RDU_ERR_INVALID, "null camera or image");
-	if(components < 3)
-		return fail(RDU_ERR_INVALID, "[PathTracer] Expected a RGB image.");
-	if(width == 0 || height == 0 || samples == 0 || depth == 0)
-		return fail(RDU_ERR_INVALID, "empty image, zero samples or zero depth");
-	// PathTracer.cpp:169-173: round to a power of two.
-	const size_t spp = size_t(std::pow(2.0, std::round(std::log2(float(samples)))));
-	rdu_render_opts opt{};
-	if(opts)
-		opt = *opts;
-	PtOptions po;
-	po.depth = uint32_t(depth);
-	po.estimator = opt.estimator;
-	po.rrStart = opt.rr_start;
+// Camera::pose + Camera::pixelShifts of a camera description (or the caller's own frame) and the
+// sample grid of spp samples (PathTracer.cpp:57-69, 176-196).
+static PtFrame makeFrame(const rdu_camera_desc * cam, const rdu_camera_frame * camFrame, uint32_t width,
+	uint32_t height, size_t spp) {
 	PtFrame f;
 	f.width = width;
 	f.height = height;
@@ -711,6 +699,34 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 	f.cellX = cx;
 	f.cellW = 1.0f / float(cx);
 	f.cellH = 1.0f / float(cy);
+	return f;
+}
+
+static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples, size_t depth, float * image,
+	uint32_t width, uint32_t height, uint32_t components, const rdu_render_opts * opts, rdu_rcache * cache,
+	const rdu_cache_opts * copts, TuneResult * tuneOut = nullptr, const rdu_camera_frame * camFrame = nullptr) {
+	clearError();
+	const auto t0 = std::chrono::steady_clock::now();
+	if(!s)
+		return fail(RDU_ERR_INVALID, "[PathTracer] No scene available.");
+	if(!s->committed)
+		return fail(RDU_ERR_STATE, "scene not committed");
+	if((!cam && !camFrame) || (!image && !tuneOut))
+		return fail(RDU_ERR_INVALID, "null camera or image");
+	if(components < 3)
+		return fail(RDU_ERR_INVALID, "[PathTracer] Expected a RGB image.");
+	if(width == 0 || height == 0 || samples == 0 || depth == 0)
+		return fail(RDU_ERR_INVALID, "empty image, zero samples or zero depth");
+	// PathTracer.cpp:169-173: round to a power of two.
+	const size_t spp = size_t(std::pow(2.0, std::round(std::log2(float(samples)))));
+	rdu_render_opts opt{};
+	if(opts)
+		opt = *opts;
+	PtOptions po;
+	po.depth = uint32_t(depth);
+	po.estimator = opt.estimator;
+	po.rrStart = opt.rr_start;
+	const PtFrame f = makeFrame(cam, camFrame, width, height, spp);
 
 	const size_t pixels = size_t(width) * height;
 	// Default pass size: 2^25 paths (about 6 GB of path state) for plain renders, so that the sparse
@@ -961,6 +977,293 @@ int rdu_pathtracer_render_frame(rdu_scene * s, const rdu_camera_frame * frame, s
 		return fail(RDU_ERR_INVALID, "null camera frame");
 	}
 	return renderImpl(s, nullptr, samples, depth, image, width, height, components, opts, nullptr, nullptr, nullptr, frame);
+}
+
+int rdu_pathtracer_render_batch(const rdu_batch_job * jobs, size_t n, uint32_t max_paths, double * job_seconds,
+	rdu_render_stats * stats) {
+	clearError();
+	const auto t0 = std::chrono::steady_clock::now();
+	if(!jobs || n == 0)
+		return fail(RDU_ERR_INVALID, "null or empty job list");
+	if(n > 65536)
+		return fail(RDU_ERR_UNSUPPORTED, "more than 65536 jobs in one batch");
+	int device = -1;
+	for(size_t j = 0; j < n; ++j) {
+		const rdu_batch_job & b = jobs[j];
+		if(!b.scene || !b.scene->committed)
+			return fail(RDU_ERR_STATE, "batch job with no committed scene");
+		if((!b.camera && !b.frame) || !b.image)
+			return fail(RDU_ERR_INVALID, "batch job with a null camera or image");
+		if(b.components < 3)
+			return fail(RDU_ERR_INVALID, "[PathTracer] Expected a RGB image.");
+		if(b.width == 0 || b.height == 0 || b.samples == 0 || b.depth == 0 || b.depth > 1024)
+			return fail(RDU_ERR_INVALID, "batch job with an empty image, zero samples or a depth outside 1..1024");
+		if(size_t(b.width) * b.height > (size_t(1) << 26))
+			return fail(RDU_ERR_UNSUPPORTED, "batch job image larger than 2^26 pixels");
+		if(device >= 0 && b.scene->rc->device != device)
+			return fail(RDU_ERR_INVALID, "batch jobs on different devices");
+		device = b.scene->rc->device;
+	}
+	int st = setDevice(device);
+	if(st)
+		return st;
+	const size_t budget = max_paths ? max_paths : (size_t(1) << 25);
+
+	// Per job: sample count (power of two), frame, options; the tree's node and triangle counts.
+	struct JobInfo {
+		size_t spp, pixels, next = 0, imgOffset = 0;
+		uint32_t nodes = 0, tris = 0, nodeOffset = 0, triOffset = 0;
+		int32_t code = 0;
+		float top[6];
+	};
+	std::vector<JobInfo> info(n);
+	std::vector<PtJob> table(n);
+	size_t nodesTotal = 0, trisTotal = 0, imgFloats = 0;
+	uint32_t maxDepth = 0, wideDepth = 0;
+	bool shadows = false, masked = false;
+	for(size_t j = 0; j < n; ++j) {
+		const rdu_batch_job & b = jobs[j];
+		const rdu_raycaster * rc = b.scene->rc;
+		JobInfo & I = info[j];
+		I.spp = size_t(std::pow(2.0, std::round(std::log2(float(b.samples)))));
+		I.pixels = size_t(b.width) * b.height;
+		I.nodes = rc->gpuBuilt ? rc->gpu.wideCount : uint32_t(rc->layout.wide.size() / 32);
+		I.tris = rc->gpuBuilt ? rc->gpu.triangles : uint32_t(rc->layout.tris.size() / 12);
+		I.nodeOffset = uint32_t(nodesTotal);
+		I.triOffset = uint32_t(trisTotal);
+		I.imgOffset = imgFloats;
+		nodesTotal += I.nodes;
+		trisTotal += I.tris;
+		imgFloats += I.pixels * b.components;
+		const int32_t c = rc->dev.wideTopCode;
+		I.code = c >= 0 ? c + int32_t(I.nodeOffset) : c - 2 * int32_t(I.triOffset);
+		for(int a = 0; a < 3; ++a) {
+			I.top[a] = rc->dev.wideTopMin[a];
+			I.top[3 + a] = rc->dev.wideTopMax[a];
+		}
+		wideDepth = std::max(wideDepth, rc->dev.wideDepth);
+		const rdu_render_opts o = b.opts;
+		PtJob & J = table[j];
+		J.S = b.scene->dev;
+		J.f = makeFrame(b.camera, b.frame, b.width, b.height, I.spp);
+		J.opt.depth = uint32_t(b.depth);
+		J.opt.estimator = o.estimator;
+		J.opt.rrStart = o.rr_start;
+		J.components = b.components;
+		maxDepth = std::max(maxDepth, uint32_t(b.depth));
+		shadows = shadows || J.S.nLights > 0 || (o.estimator == RDU_ESTIMATOR_EXTENDED && J.S.pArea > 0.0f);
+		masked = masked || J.S.anyMasked != 0;
+	}
+	if(nodesTotal >= size_t(0x7fffffff) || trisTotal >= (size_t(1) << 30))
+		return fail(RDU_ERR_UNSUPPORTED, "batch trees too large for one arena");
+
+	hipStream_t stream = nullptr;
+	if((st = hipCheck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate")))
+		return st;
+	// Device memory of the batch; freed in stream order when the call returns.
+	struct Local {
+		hipStream_t s;
+		DevBuffer arenaNodes, arenaTris, roots, jobsDev, chunks, images;
+		std::vector<hipEvent_t> events;
+		~Local() {
+			(void)hipStreamSynchronize(s);
+			for(DevBuffer * b : {&arenaNodes, &arenaTris, &roots, &jobsDev, &chunks, &images})
+				b->releaseOn(s);
+			for(hipEvent_t e : events)
+				(void)hipEventDestroy(e);
+			(void)hipStreamSynchronize(s);
+			(void)hipStreamDestroy(s);
+		}
+	} L;
+	L.s = stream;
+	if((st = L.arenaNodes.reserveOn(std::max<size_t>(nodesTotal, 1) * 128, stream)) ||
+		(st = L.arenaTris.reserveOn(std::max<size_t>(trisTotal, 1) * 48, stream)) || (st = L.roots.reserveOn(n * 32, stream)) ||
+		(st = L.jobsDev.reserveOn(n * sizeof(PtJob), stream)) || (st = L.images.reserveOn(imgFloats * 4, stream)))
+		return st;
+	float4 * arenaNodes = static_cast<float4 *>(L.arenaNodes.ptr);
+	float4 * arenaTris = static_cast<float4 *>(L.arenaTris.ptr);
+	float * images = static_cast<float *>(L.images.ptr);
+	std::vector<float4> roots(2 * n);
+	for(size_t j = 0; j < n; ++j) {
+		const JobInfo & I = info[j];
+		const rdu_raycaster * rc = jobs[j].scene->rc;
+		if((st = hipCheck(hipError_t(ptRebaseWide(arenaNodes + 8 * size_t(I.nodeOffset), rc->dev.wide, I.nodes,
+								 int32_t(I.nodeOffset), int32_t(I.triOffset), stream)),
+				"arena nodes")) ||
+			(I.tris && (st = hipCheck(hipMemcpyAsync(arenaTris + 3 * size_t(I.triOffset), rc->dev.tris, size_t(I.tris) * 48,
+								 hipMemcpyDeviceToDevice, stream),
+							"arena triangles"))))
+			return st;
+		roots[2 * j] = make_float4(I.top[0], I.top[1], I.top[2], __builtin_bit_cast(float, I.code));
+		roots[2 * j + 1] = make_float4(I.top[3], I.top[4], I.top[5], 0.0f);
+		table[j].image = images + I.imgOffset;
+		const rdu_batch_job & b = jobs[j];
+		if((st = hipCheck(hipMemcpyAsync(table[j].image, b.image, I.pixels * b.components * 4, hipMemcpyHostToDevice, stream),
+				"image upload")))
+			return st;
+	}
+	if((st = hipCheck(hipMemcpyAsync(L.roots.ptr, roots.data(), roots.size() * 16, hipMemcpyHostToDevice, stream), "roots")) ||
+		(st = hipCheck(hipMemcpyAsync(L.jobsDev.ptr, table.data(), n * sizeof(PtJob), hipMemcpyHostToDevice, stream), "jobs")))
+		return st;
+	DevBVH arena;
+	arena.wide = arenaNodes;
+	arena.tris = arenaTris;
+	arena.wideDepth = wideDepth;
+
+	// Passes: longest jobs first (pixels x samples x depth), each pass taking as many samples of
+	// each unfinished job as fit in the path budget; a job's samples run in order over its passes.
+	std::vector<size_t> order(n);
+	for(size_t j = 0; j < n; ++j)
+		order[j] = j;
+	std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+		return info[a].pixels * info[a].spp * jobs[a].depth > info[b].pixels * info[b].spp * jobs[b].depth;
+	});
+	size_t maxPass = 0;
+	for(size_t j = 0; j < n; ++j)
+		maxPass = std::max(maxPass, std::min(info[j].pixels * info[j].spp, std::max(budget, info[j].pixels)));
+	const size_t passCap = std::max(std::min(budget, [&] {
+		size_t t = 0;
+		for(size_t j = 0; j < n; ++j)
+			t += info[j].pixels * info[j].spp;
+		return t;
+	}()), maxPass);
+	if(passCap > 0xffffffffull)
+		return fail(RDU_ERR_UNSUPPORTED, "batch pass above 2^32 paths");
+	PathsLease R(device, passCap, stream);
+	if((st = ensurePathsFor(*R, arena, masked, passCap, true, stream)))
+		return st;
+	if((st = L.chunks.reserveOn(n * sizeof(PtChunk), stream)) ||
+		(st = hipCheck(hipMemsetAsync(R->pTotals.ptr, 0, 16, stream), "hipMemsetAsync")))
+		return st;
+	PtPaths P;
+	P.rays = static_cast<float4 *>(R->pRays.ptr);
+	P.hits = static_cast<rdu_hit *>(R->pHits.ptr);
+	P.atten = static_cast<float4 *>(R->pAtten.ptr);
+	P.color = static_cast<float4 *>(R->pColor.ptr);
+	P.queue[0] = static_cast<uint32_t *>(R->pQ0.ptr);
+	P.queue[1] = static_cast<uint32_t *>(R->pQ1.ptr);
+	P.counts = static_cast<uint32_t *>(R->pCounts.ptr);
+	P.totals = static_cast<unsigned long long *>(R->pTotals.ptr);
+	P.shadowRays = static_cast<float4 *>(R->pSRays.ptr);
+	P.shadowContrib = static_cast<float4 *>(R->pSContrib.ptr);
+	P.shadowQueue[0] = static_cast<uint32_t *>(R->pSQ0.ptr);
+	P.shadowQueue[1] = static_cast<uint32_t *>(R->pSQ1.ptr);
+	P.occluded = static_cast<uint8_t *>(R->pOcc.ptr);
+	P.shadowHits = static_cast<rdu_hit *>(R->pSHits.ptr);
+	P.mis = static_cast<float2 *>(R->pMis.ptr);
+	PtScratch scratch;
+	scratch.gstack = static_cast<int *>(R->pGstack.ptr);
+	scratch.walkLeft = R->walkLeft;
+	PtBatch B;
+	B.jobs = static_cast<const PtJob *>(L.jobsDev.ptr);
+	B.chunks = static_cast<const PtChunk *>(L.chunks.ptr);
+	B.pathChunk = static_cast<uint32_t *>(R->pChunk.ptr);
+	B.pathJob = static_cast<uint32_t *>(R->pJob.ptr);
+	B.roots = static_cast<const float4 *>(L.roots.ptr);
+	hipEvent_t start;
+	if((st = hipCheck(hipEventCreate(&start), "hipEventCreate")))
+		return st;
+	L.events.push_back(start);
+	if((st = hipCheck(hipEventRecord(start, stream), "hipEventRecord")))
+		return st;
+	std::vector<hipEvent_t> doneAt(n, nullptr);
+	std::vector<std::vector<PtChunk>> passChunks; // kept until the stream has read them
+	uint32_t passes = 0;
+	size_t remainingJobs = n;
+	const TraceTuning tune;
+	while(remainingJobs > 0) {
+		std::vector<PtChunk> chunks;
+		std::vector<size_t> finished;
+		size_t used = 0, slots = 0;
+		for(size_t j : order) {
+			JobInfo & I = info[j];
+			if(I.next >= I.spp)
+				continue;
+			size_t take = (budget - std::min(budget, used)) / I.pixels;
+			if(take == 0 && chunks.empty())
+				take = 1; // an image larger than the budget: one sample per pass
+			take = std::min(take, I.spp - I.next);
+			if(take == 0)
+				continue;
+			PtChunk c;
+			c.job = uint32_t(j);
+			c.pathBegin = uint32_t(used);
+			c.samples = uint32_t(take);
+			c.sampleBase = uint32_t(I.next);
+			c.seed = jobs[j].opts.seed;
+			c.slotBegin = uint32_t(slots);
+			chunks.push_back(c);
+			used += take * I.pixels;
+			slots += I.pixels;
+			I.next += take;
+			if(I.next >= I.spp)
+				finished.push_back(j);
+		}
+		passChunks.push_back(chunks);
+		const std::vector<PtChunk> & pc = passChunks.back();
+		if((st = hipCheck(hipMemcpyAsync(L.chunks.ptr, pc.data(), pc.size() * sizeof(PtChunk), hipMemcpyHostToDevice, stream),
+				"chunks")))
+			return st;
+		B.nChunks = uint32_t(pc.size());
+		B.nSlots = uint32_t(slots);
+		if((st = hipCheck(hipError_t(ptRenderBatchPass(B, arena, P, uint32_t(used), maxDepth, shadows, masked, 16, scratch, tune,
+								 stream)),
+				"batch pass")))
+			return st;
+		++passes;
+		for(size_t j : finished) {
+			const JobInfo & I = info[j];
+			const rdu_batch_job & b = jobs[j];
+			hipEvent_t e;
+			if((st = hipCheck(hipEventCreate(&e), "hipEventCreate")))
+				return st;
+			L.events.push_back(e);
+			if((st = hipCheck(hipError_t(ptNormalizeImage(table[j].image, uint32_t(I.pixels), b.components, uint32_t(I.spp),
+									 stream)),
+					"normalise")) ||
+				(st = hipCheck(hipMemcpyAsync(b.image, table[j].image, I.pixels * b.components * 4, hipMemcpyDeviceToHost,
+								   stream),
+						"image download")) ||
+				(st = hipCheck(hipEventRecord(e, stream), "hipEventRecord")))
+				return st;
+			doneAt[j] = e;
+			--remainingJobs;
+		}
+	}
+	unsigned long long totals[2] = {0, 0};
+	if((st = hipCheck(hipMemcpyAsync(totals, R->pTotals.ptr, 16, hipMemcpyDeviceToHost, stream), "stats download")) ||
+		(st = hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize")))
+		return st;
+	// Completion time of each job from the call's start: host time to the first enqueued work plus
+	// the device time from there to the job's last copy.
+	const double hostLead = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+	float lastMs = 0.0f;
+	for(size_t j = 0; j < n; ++j) {
+		float ms = 0.0f;
+		if((st = hipCheck(hipEventElapsedTime(&ms, start, doneAt[j]), "hipEventElapsedTime")))
+			return st;
+		lastMs = std::max(lastMs, ms);
+		if(job_seconds)
+			job_seconds[j] = double(ms) * 1e-3;
+	}
+	if(job_seconds) {
+		// Shift to the call's clock: the last job finished when the call's wait returned.
+		const double shift = hostLead - double(lastMs) * 1e-3;
+		for(size_t j = 0; j < n; ++j)
+			job_seconds[j] += shift > 0.0 ? shift : 0.0;
+	}
+	if(stats) {
+		size_t paths = 0;
+		for(size_t j = 0; j < n; ++j)
+			paths += info[j].pixels * info[j].spp;
+		*stats = rdu_render_stats{};
+		stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+		stats->paths = paths;
+		stats->extension_rays = totals[0];
+		stats->shadow_rays = totals[1];
+		stats->passes = passes;
+	}
+	return RDU_OK;
 }
 
 int rdu_scene_set_graph(rdu_scene * s, int enable) {

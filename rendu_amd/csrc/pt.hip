@@ -164,14 +164,14 @@ __global__ void ptShadowSwap(uint32_t * counts, unsigned long long * totals) {
 	}
 }
 
-// One bounce of PathTracer::render's loop body (PathTracer.cpp:202-286) for every queued path.
+// One bounce of PathTracer::render's loop body (PathTracer.cpp:202-286) for queue entry i of n.
+// The path's sample and pixel come from its id less pathBegin (the first path of its job in a
+// batch pass; 0 otherwise). Called by the active lanes of a wavefront together (the queue appends
+// are wavefront ballots over the active lanes).
 template<bool CACHE>
-__global__ __launch_bounds__(kPtBlock) void ptShade(DevScene S, PtFrame f, PtPaths P, const uint32_t * queueIn,
-	uint32_t * queueOut, uint32_t * counts, uint32_t bounce, PtOptions opt, PtCache C) {
-	const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
-	const uint32_t n = counts[kCountActive];
-	const uint32_t sampleBase = counts[kCountBase];
-	const uint32_t seed = counts[kCountSeed];
+__device__ __forceinline__ void shadeEntry(const DevScene & S, const PtFrame & f, const PtPaths & P,
+	const uint32_t * queueIn, uint32_t * queueOut, uint32_t * counts, uint32_t bounce, const PtOptions & opt,
+	const PtCache & C, uint32_t i, uint32_t n, uint32_t sampleBase, uint32_t seed, uint32_t pathBegin) {
 	bool cont = false, shadow = false;
 	bool doQuery = false, doRecord = false; // radiance-cache mode only
 	f3 cPos{0, 0, 0}, cDir{0, 0, 0}, cColor{0, 0, 0}, cAtten{0, 0, 0};
@@ -179,7 +179,8 @@ __global__ __launch_bounds__(kPtBlock) void ptShade(DevScene S, PtFrame f, PtPat
 	if(i < n) {
 		p = queueIn[i];
 		const uint32_t pixels = f.width * f.height;
-		const uint32_t s = p / pixels, pix = p % pixels;
+		const uint32_t local = p - pathBegin;
+		const uint32_t s = local / pixels, pix = local % pixels;
 		const uint32_t sid = sampleBase + s;
 		const float4 ro = P.rays[2 * size_t(p)];
 		const float4 rd = P.rays[2 * size_t(p) + 1];
@@ -372,6 +373,13 @@ __global__ __launch_bounds__(kPtBlock) void ptShade(DevScene S, PtFrame f, PtPat
 	}
 }
 
+template<bool CACHE>
+__global__ __launch_bounds__(kPtBlock) void ptShade(DevScene S, PtFrame f, PtPaths P, const uint32_t * queueIn,
+	uint32_t * queueOut, uint32_t * counts, uint32_t bounce, PtOptions opt, PtCache C) {
+	shadeEntry<CACHE>(S, f, P, queueIn, queueOut, counts, bounce, opt, C, blockIdx.x * kPtBlock + threadIdx.x,
+		counts[kCountActive], counts[kCountBase], counts[kCountSeed], 0u);
+}
+
 // Terminated paths add throughput x cache prediction.
 __global__ __launch_bounds__(kPtBlock) void ptCacheApply(PtPaths P, PtCache C) {
 	const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
@@ -470,12 +478,11 @@ __global__ __launch_bounds__(kPtBlock) void ptShadowApply(PtPaths P, const uint3
 // Rendu's shadow walk through alpha cut-outs (PathTracer.cpp:124-157), one closest hit per step:
 // no hit -> visible; an unmasked (or UV-less) occluder -> occluded; a transparent texel ->
 // advance past it and query again.
-__global__ __launch_bounds__(kPtBlock) void ptShadowWalk(DevScene S, PtPaths P, const uint32_t * queueIn,
-	uint32_t * queueOut, uint32_t * counts) {
-	const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
+__device__ __forceinline__ void walkEntry(const DevScene & S, const PtPaths & P, const uint32_t * queueIn,
+	uint32_t * queueOut, uint32_t * counts, uint32_t i, uint32_t n) {
 	bool again = false;
 	uint32_t p = 0;
-	if(i < counts[kCountShadow]) {
+	if(i < n) {
 		p = queueIn[i];
 		const rdu_hit h = P.shadowHits[p];
 		bool visible = false;
@@ -511,6 +518,11 @@ __global__ __launch_bounds__(kPtBlock) void ptShadowWalk(DevScene S, PtPaths P, 
 	appendQueue(again, p, queueOut, counts + kCountShadowNext);
 }
 
+__global__ __launch_bounds__(kPtBlock) void ptShadowWalk(DevScene S, PtPaths P, const uint32_t * queueIn,
+	uint32_t * queueOut, uint32_t * counts) {
+	walkEntry(S, P, queueIn, queueOut, counts, blockIdx.x * kPtBlock + threadIdx.x, counts[kCountShadow]);
+}
+
 // image.rgb += min(color, 5) over the pass's samples in sample order (PathTracer.cpp:288).
 __global__ __launch_bounds__(kPtBlock) void ptAccumulate(PtFrame f, PtPaths P, float * image, uint32_t components,
 	uint32_t samplesInPass) {
@@ -543,6 +555,137 @@ __global__ __launch_bounds__(kPtBlock) void ptNormalize(float * image, uint32_t 
 }
 
 inline unsigned blocksFor(size_t n) { return unsigned((n + kPtBlock - 1) / kPtBlock); }
+
+// ---- batch passes ---------------------------------------------------------------------------------
+
+// Index of the last entry of a sorted array `begin` (of n >= 1 entries, begin[0] = 0) not above v.
+__device__ __forceinline__ uint32_t lastNotAbove(const PtChunk * chunks, uint32_t n, uint32_t v, bool slots) {
+	uint32_t lo = 0, hi = n; // answer in [lo, hi)
+	while(hi - lo > 1) {
+		const uint32_t mid = (lo + hi) >> 1;
+		const uint32_t b = slots ? chunks[mid].slotBegin : chunks[mid].pathBegin;
+		if(b <= v)
+			lo = mid;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+
+// Camera rays of every chunk's paths (ptGenerate per job), with the path's chunk and job.
+__global__ __launch_bounds__(kPtBlock) void ptGenerateBatch(PtBatch B, PtPaths P, uint32_t nPaths) {
+	const uint32_t p = blockIdx.x * kPtBlock + threadIdx.x;
+	if(p >= nPaths)
+		return;
+	const uint32_t k = lastNotAbove(B.chunks, B.nChunks, p, false);
+	const PtChunk c = B.chunks[k];
+	const PtFrame f = B.jobs[c.job].f;
+	const uint32_t pixels = f.width * f.height;
+	const uint32_t local = p - c.pathBegin;
+	const uint32_t s = local / pixels, pix = local % pixels;
+	const uint32_t x = pix % f.width, y = pix / f.width;
+	const f2 ndc = samplePos(f, c.seed, x, y, c.sampleBase + s);
+	const f3 eye = f.eye;
+	const f3 worldPos = f.corner + ndc.x * f.dx + ndc.y * f.dy;
+	const f3 dir = normalize(worldPos - eye);
+	putRay(P.rays, p, eye, dir, 0.0001f, 1e8f);
+	P.atten[p] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+	P.color[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	P.mis[p] = make_float2(0.0f, 0.0f);
+	P.queue[0][p] = p;
+	B.pathChunk[p] = k;
+	B.pathJob[p] = c.job;
+}
+
+// ptShade over a batch queue: the lanes of a wavefront are served one chunk at a time (the chunk's
+// and its job's descriptors are wavefront-uniform, so they are read with scalar loads); paths of
+// one chunk are contiguous in the queues, so a wavefront usually needs one round.
+__global__ __launch_bounds__(kPtBlock) void ptShadeBatch(const PtJob * __restrict__ jobs,
+	const PtChunk * __restrict__ chunks, const uint32_t * __restrict__ pathChunk, PtPaths P, const uint32_t * queueIn,
+	uint32_t * queueOut, uint32_t * counts, uint32_t bounce) {
+	const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
+	const uint32_t n = counts[kCountActive];
+	bool todo = i < n;
+	const uint32_t k = todo ? pathChunk[queueIn[i]] : 0u;
+	for(;;) {
+		const uint64_t m = __ballot(todo);
+		if(!m)
+			break;
+		const uint32_t leader = uint32_t(__ffsll((unsigned long long)m) - 1);
+		const uint32_t kk = uint32_t(__builtin_amdgcn_readlane(int(k), int(leader)));
+		if(todo && k == kk) {
+			const PtChunk c = chunks[kk];
+			const PtJob & J = jobs[c.job];
+			shadeEntry<false>(J.S, J.f, P, queueIn, queueOut, counts, bounce, J.opt, PtCache{}, i, n, c.sampleBase, c.seed,
+				c.pathBegin);
+			todo = false;
+		}
+	}
+}
+
+// ptShadowWalk over a batch queue, one job at a time per wavefront (as ptShadeBatch).
+__global__ __launch_bounds__(kPtBlock) void ptShadowWalkBatch(PtBatch B, PtPaths P, const uint32_t * queueIn,
+	uint32_t * queueOut, uint32_t * counts) {
+	const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
+	const uint32_t n = counts[kCountShadow];
+	bool todo = i < n;
+	const uint32_t j = todo ? B.pathJob[queueIn[i]] : 0u;
+	for(;;) {
+		const uint64_t m = __ballot(todo);
+		if(!m)
+			break;
+		const uint32_t leader = uint32_t(__ffsll((unsigned long long)m) - 1);
+		const uint32_t jj = uint32_t(__builtin_amdgcn_readlane(int(j), int(leader)));
+		if(todo && j == jj) {
+			walkEntry(B.jobs[jj].S, P, queueIn, queueOut, counts, i, n);
+			todo = false;
+		}
+	}
+}
+
+// ptAccumulate for every chunk: slot t is pixel t - slotBegin of its chunk's job image; the chunk's
+// samples are added in sample order.
+__global__ __launch_bounds__(kPtBlock) void ptAccumulateBatch(PtBatch B, PtPaths P) {
+	const uint32_t t = blockIdx.x * kPtBlock + threadIdx.x;
+	if(t >= B.nSlots)
+		return;
+	const uint32_t k = lastNotAbove(B.chunks, B.nChunks, t, true);
+	const PtChunk c = B.chunks[k];
+	const PtJob & J = B.jobs[c.job];
+	const uint32_t pixels = J.f.width * J.f.height;
+	const uint32_t pix = t - c.slotBegin;
+	float * px = J.image + size_t(pix) * J.components;
+	float r = px[0], g = px[1], b = px[2];
+	for(uint32_t s = 0; s < c.samples; ++s) {
+		const float4 col = P.color[size_t(c.pathBegin) + size_t(s) * pixels + pix];
+		r += gmin(col.x, 5.0f);
+		g += gmin(col.y, 5.0f);
+		b += gmin(col.z, 5.0f);
+	}
+	px[0] = r;
+	px[1] = g;
+	px[2] = b;
+}
+
+// A tree's 4-wide nodes (8 float4: six plane records, the four child codes, padding) copied into an
+// arena: node codes move by nodeOffset, leaf codes ~(first << 1 | count - 1) by triOffset triangles.
+__global__ __launch_bounds__(kPtBlock) void ptRebaseWideKernel(float4 * __restrict__ dst, const float4 * __restrict__ src,
+	uint32_t nodes, int32_t nodeOffset, int32_t triOffset) {
+	const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
+	if(i >= nodes * 8u)
+		return;
+	float4 v = src[i];
+	if((i & 7u) == 6u) {
+		int32_t c[4] = {__float_as_int(v.x), __float_as_int(v.y), __float_as_int(v.z), __float_as_int(v.w)};
+		for(int k = 0; k < 4; ++k) {
+			if(c[k] == kWideEmpty)
+				continue;
+			c[k] = c[k] >= 0 ? c[k] + nodeOffset : c[k] - 2 * triOffset;
+		}
+		v = make_float4(__int_as_float(c[0]), __int_as_float(c[1]), __int_as_float(c[2]), __int_as_float(c[3]));
+	}
+	dst[i] = v;
+}
 
 } // namespace
 
@@ -655,6 +798,93 @@ int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, cons
 		hipLaunchKernelGGL(ptCacheDecay, dim3(blocksFor(cells)), dim3(kPtBlock), 0, stream, C.cells, cells);
 		hipLaunchKernelGGL(ptCacheHeldOut, dim3(blocksFor(holdCap)), dim3(kPtBlock), 0, stream, C);
 	}
+	return int(hipGetLastError());
+}
+
+int ptRebaseWide(float4 * dst, const float4 * src, uint32_t nodes, int32_t nodeOffset, int32_t triOffset,
+	hipStream_t stream) {
+	if(nodes == 0)
+		return 0;
+	hipLaunchKernelGGL(ptRebaseWideKernel, dim3(blocksFor(size_t(nodes) * 8)), dim3(kPtBlock), 0, stream, dst, src, nodes,
+		nodeOffset, triOffset);
+	return int(hipGetLastError());
+}
+
+int ptRenderBatchPass(const PtBatch & B, const DevBVH & arena, const PtPaths & P, uint32_t nPaths, uint32_t maxDepth,
+	bool shadows, bool masked, int maxShadowLayers, PtScratch & scratch, const TraceTuning & tune, hipStream_t stream) {
+	if(nPaths == 0 || B.nChunks == 0)
+		return 0;
+	TraceTuning tn = tune;
+	if(tn.chunk == 0)
+		tn.chunk = 64;
+	if(tn.refillMin == 0)
+		tn.refillMin = 8;
+	hipLaunchKernelGGL(ptResetCounts, dim3(1), dim3(64), 0, stream, P.counts, nPaths, 0u);
+	hipLaunchKernelGGL(ptGenerateBatch, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, B, P, nPaths);
+	int q = 0;
+	for(uint32_t b = 0; b < maxDepth; ++b) {
+		TraceOptions to;
+		applyTuning(to, tn);
+		to.wide = true;
+		to.nearFar = b > 0;
+		to.perm = P.queue[q];
+		to.countDev = P.counts + kCountActive;
+		to.rayJob = B.pathJob;
+		to.jobRoots = B.roots;
+		int e = launchClosest(arena, P.rays, nPaths, P.hits, to, scratch.gstack, nullptr, stream);
+		if(e)
+			return e;
+		hipLaunchKernelGGL(ptShadeBatch, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, B.jobs, B.chunks, B.pathChunk,
+			P, P.queue[q], P.queue[1 - q], P.counts, b);
+		if(shadows) {
+			if(!masked) {
+				TraceOptions so;
+				applyTuning(so, tn);
+				so.wide = true;
+				so.nearFar = true;
+				so.perm = P.shadowQueue[0];
+				so.countDev = P.counts + kCountShadow;
+				so.rayJob = B.pathJob;
+				so.jobRoots = B.roots;
+				e = launchAny(arena, P.shadowRays, nPaths, P.occluded, so, scratch.gstack, nullptr, stream);
+				if(e)
+					return e;
+				hipLaunchKernelGGL(ptShadowApply, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, P, P.counts);
+			} else {
+				int layer = 0;
+				for(;;) {
+					for(int k = 0; k < maxShadowLayers; ++k, ++layer) {
+						TraceOptions so;
+						applyTuning(so, tn);
+						so.wide = true;
+						so.perm = P.shadowQueue[layer & 1];
+						so.countDev = P.counts + kCountShadow;
+						so.rayJob = B.pathJob;
+						so.jobRoots = B.roots;
+						e = launchClosest(arena, P.shadowRays, nPaths, P.shadowHits, so, scratch.gstack, nullptr, stream);
+						if(e)
+							return e;
+						hipLaunchKernelGGL(ptShadowWalkBatch, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, B, P,
+							P.shadowQueue[layer & 1], P.shadowQueue[1 - (layer & 1)], P.counts);
+						hipLaunchKernelGGL(ptShadowSwap, dim3(1), dim3(64), 0, stream, P.counts, P.totals);
+					}
+					if(!scratch.walkLeft)
+						return int(hipErrorInvalidValue);
+					hipError_t he = hipMemcpyAsync(scratch.walkLeft, P.counts + kCountShadow, sizeof(uint32_t),
+						hipMemcpyDeviceToHost, stream);
+					if(he == hipSuccess)
+						he = hipStreamSynchronize(stream);
+					if(he != hipSuccess)
+						return int(he);
+					if(*scratch.walkLeft == 0)
+						break;
+				}
+			}
+		}
+		hipLaunchKernelGGL(ptAdvance, dim3(1), dim3(64), 0, stream, P.counts, P.totals);
+		q = 1 - q;
+	}
+	hipLaunchKernelGGL(ptAccumulateBatch, dim3(blocksFor(B.nSlots)), dim3(kPtBlock), 0, stream, B, P);
 	return int(hipGetLastError());
 }
 

@@ -108,6 +108,46 @@ int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, cons
 	const PtOptions & opt, float * dImage, uint32_t components, PtScratch & scratch, const PtCache * cache,
 	const TraceTuning & tune, hipStream_t stream);
 int ptSetSampleBase(uint32_t * counts, uint32_t base, uint32_t seed, hipStream_t stream);
+
+// Batch passes: the paths of several jobs (scene, camera, image, samples, depth) in one set of
+// launches. A pass holds chunks: some consecutive samples of one job, whose paths are the id range
+// [pathBegin, pathBegin + samples x pixels) laid out like a single render's pass (sample-major).
+// Each job's tree lives in one node arena shared by the pass (codes rebased), so the traversal
+// starts every ray at its job's root. Paths record their chunk and job; the shading kernels read a
+// wavefront's job descriptor with scalar loads (once per distinct job among its lanes).
+struct PtJob {
+	DevScene S;
+	PtFrame f;
+	PtOptions opt;
+	float * image = nullptr; // device accumulation image of the job (pixels x components)
+	uint32_t components = 3;
+};
+struct PtChunk {
+	uint32_t job = 0;
+	uint32_t pathBegin = 0;  // first path id of the chunk in the pass
+	uint32_t samples = 0;    // samples of the chunk
+	uint32_t sampleBase = 0; // index of its first sample in the job's render
+	uint32_t seed = 0;       // the job's random-stream key
+	uint32_t slotBegin = 0;  // first accumulation slot of the chunk (running sum of pixels over chunks)
+};
+struct PtBatch {
+	const PtJob * jobs = nullptr;     // device table
+	const PtChunk * chunks = nullptr; // device table of the pass
+	uint32_t nChunks = 0;
+	uint32_t nSlots = 0;              // pixels summed over the chunks
+	uint32_t * pathChunk = nullptr;   // per path
+	uint32_t * pathJob = nullptr;     // per path (the traversal's root index)
+	const float4 * roots = nullptr;   // per job: {top min, root code bits}, {top max, 0}
+};
+// One batch pass of nPaths paths over the arena tree; maxDepth is the largest job depth;
+// shadows: some job samples lights; masked: some job has cut-outs (then every job's shadow rays
+// take the cut-out walk, which gives the any-hit answer on scenes without cut-outs).
+int ptRenderBatchPass(const PtBatch & B, const DevBVH & arena, const PtPaths & P, uint32_t nPaths, uint32_t maxDepth,
+	bool shadows, bool masked, int maxShadowLayers, PtScratch & scratch, const TraceTuning & tune, hipStream_t stream);
+// Copies a tree's 4-wide nodes into an arena at node offset nodeOffset with its triangles at
+// triangle offset triOffset, rebasing the child codes.
+int ptRebaseWide(float4 * dst, const float4 * src, uint32_t nodes, int32_t nodeOffset, int32_t triOffset,
+	hipStream_t stream);
 int ptNormalizeImage(float * dImage, uint32_t pixels, uint32_t components, uint32_t spp, hipStream_t stream);
 
 } // namespace rdu

@@ -370,11 +370,13 @@ __global__ __launch_bounds__(kSimpleBlock) void traceKernel(DevBVH bvh, const fl
 
 // ALT selects the kernel's variant: for closest hit the near/far plane fetch, for any hit the
 // unsorted visiting order; NFA selects the near/far plane fetch for (sorted) any hit.
-template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA = false>
+// BATCH: rays of several scenes whose trees share one node arena; ray k starts at the root of scene
+// rayJob[k], whose top box and code are jobRoots[2 rayJob[k]] (min, code bits) and [+1] (max).
+template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA = false, bool BATCH = false>
 __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
 	const uint32_t * __restrict__ perm, const uint32_t * __restrict__ countDev, rdu_hit * __restrict__ hits,
 	uint8_t * __restrict__ occ, int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint32_t chunk,
-	uint64_t * stats);
+	const uint32_t * __restrict__ rayJob, const float4 * __restrict__ jobRoots, uint64_t * stats);
 
 // MI355X (gfx950) has 8 XCDs; workgroups are dispatched to them round-robin.
 constexpr uint32_t kXcds = 8;
@@ -398,7 +400,7 @@ uint32_t refillMin(const TraceOptions & opt) {
 	return opt.spatialOrder ? 32u : 16u;
 }
 
-template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA>
+template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA, bool BATCH>
 PersistentGrid persistentGrid(size_t n, uint32_t blocksPerCuCap);
 
 template<bool ANY>
@@ -427,9 +429,9 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 	if(persistent) {
 		if(n > 0xffffffffull)
 			return int(hipErrorInvalidValue);
-#define RDU_PERSIST(S, ST, ALT, NFA)                                                                               \
+#define RDU_PERSIST_B(S, ST, ALT, NFA, B)                                                                         \
 	{                                                                                                            \
-		const PersistentGrid g = persistentGrid<ANY, S, ST, ALT, NFA>(n, opt.blocksPerCu);                            \
+		const PersistentGrid g = persistentGrid<ANY, S, ST, ALT, NFA, B>(n, opt.blocksPerCu);                         \
 		const uint32_t waves = (g.blocks / kXcds) * (kBlock / 64);                                               \
 		const uint64_t slice = (uint64_t(n) + kXcds - 1) / kXcds;                                                \
 		/* sorted batches: interleaved 64-ray chunks; input order: one contiguous run per wavefront; */        \
@@ -438,15 +440,29 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 		chunk = chunk < 64 ? 64 : chunk;                                                                        \
 		if(opt.chunk >= 16 && opt.chunk <= (1u << 20))                                                          \
 			chunk = opt.chunk;                                                                                  \
-		hipLaunchKernelGGL((tracePersistent<ANY, S, ST, ALT, NFA>), dim3(g.blocks), dim3(kBlock), 0, stream, bvh, rays,  \
-			uint32_t(n), perm, opt.countDev, hits, occ, gstack, lds, kXcds, refillMin(opt), uint32_t(chunk), stats); \
+		hipLaunchKernelGGL((tracePersistent<ANY, S, ST, ALT, NFA, B>), dim3(g.blocks), dim3(kBlock), 0, stream, bvh,     \
+			rays, uint32_t(n), perm, opt.countDev, hits, occ, gstack, lds, kXcds, refillMin(opt), uint32_t(chunk),      \
+			opt.rayJob, opt.jobRoots, stats);                                                                   \
 	}
+#define RDU_PERSIST(S, ST, ALT, NFA) RDU_PERSIST_B(S, ST, ALT, NFA, false)
 		// Kernel variant: the near/far plane fetch (fewer instructions per node, more registers:
 		// 5 waves per SIMD instead of 6 for closest hit and of 7 for any hit; pays on incoherent rays), or
 		// for any hit on coherent rays the unsorted visiting order.
 		const bool alt = ANY ? opt.unorderedAny : opt.nearFar;
 		const bool nfa = ANY && !opt.unorderedAny && opt.nearFar;
-		if(alt) {
+		if(opt.jobRoots) {
+			// Batch passes of the path tracer: closest hit with either fetch, any hit with the
+			// near/far fetch; no statistics.
+			if(stats || (ANY && !nfa))
+				return int(hipErrorInvalidValue);
+			if(ANY) {
+				if(spill) RDU_PERSIST_B(true, false, false, true, true) else RDU_PERSIST_B(false, false, false, true, true)
+			} else if(alt) {
+				if(spill) RDU_PERSIST_B(true, false, true, false, true) else RDU_PERSIST_B(false, false, true, false, true)
+			} else {
+				if(spill) RDU_PERSIST_B(true, false, false, false, true) else RDU_PERSIST_B(false, false, false, false, true)
+			}
+		} else if(alt) {
 			if(stats) {
 				if(spill) RDU_PERSIST(true, true, true, false) else RDU_PERSIST(false, true, true, false)
 			} else {
@@ -466,6 +482,7 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 			}
 		}
 #undef RDU_PERSIST
+#undef RDU_PERSIST_B
 		return int(hipGetLastError());
 	}
 	if(stats) {
@@ -513,11 +530,11 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 // so never above the true entry): a popped child whose entry already exceeds the current maxi
 // is dropped without being fetched, exactly as its own box test would drop it (the test fails
 // once entry > maxi). Levels spilled to global memory carry no entry and are always fetched.
-template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA>
+template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA, bool BATCH>
 __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
 	const uint32_t * __restrict__ perm, const uint32_t * __restrict__ countDev, rdu_hit * __restrict__ hits,
 	uint8_t * __restrict__ occ, int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint32_t chunk,
-	uint64_t * stats) {
+	const uint32_t * __restrict__ rayJob, const float4 * __restrict__ jobRoots, uint64_t * stats) {
 	constexpr bool NF = (ALT && !ANY) || (ANY && NFA && !ALT); // near/far plane fetch
 	constexpr bool UNSORTED = ALT && ANY; // children in node order
 	__shared__ int ls[kWideLds][kBlock];
@@ -600,9 +617,18 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 					kk = 0;
 					// The top box of the 4-wide tree (select semantics: correct for any ray).
 					float e;
-					if(slab6<true>(r, bvh.wideTopMin[0], bvh.wideTopMin[1], bvh.wideTopMin[2], bvh.wideTopMax[0],
-						   bvh.wideTopMax[1], bvh.wideTopMax[2], r.mini, maxi, e)) {
+					bool top;
+					if constexpr(BATCH) {
+						const uint32_t jb = rayJob[ri];
+						const float4 lo = jobRoots[2 * size_t(jb)], hi = jobRoots[2 * size_t(jb) + 1];
+						top = slab6<true>(r, lo.x, lo.y, lo.z, hi.x, hi.y, hi.z, r.mini, maxi, e);
+						c = __float_as_int(lo.w);
+					} else {
+						top = slab6<true>(r, bvh.wideTopMin[0], bvh.wideTopMin[1], bvh.wideTopMin[2], bvh.wideTopMax[0],
+							bvh.wideTopMax[1], bvh.wideTopMax[2], r.mini, maxi, e);
 						c = bvh.wideTopCode;
+					}
+					if(top) {
 						has = true;
 					} else {
 						pend = true; // a miss, written at the next refill
@@ -787,7 +813,7 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 	}
 }
 
-template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA>
+template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA, bool BATCH>
 PersistentGrid persistentGrid(size_t n, uint32_t blocksPerCuCap) {
 	struct Occupancy {
 		int blocksPerCu = 1, cus = 1;
@@ -798,7 +824,8 @@ PersistentGrid persistentGrid(size_t n, uint32_t blocksPerCuCap) {
 		int dev = 0;
 		(void)hipGetDevice(&dev);
 		(void)hipDeviceGetAttribute(&o.cus, hipDeviceAttributeMultiprocessorCount, dev);
-		(void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocksPerCu, tracePersistent<ANY, SPILL, STATS, ALT, NFA>, kBlock, 0);
+		(void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocksPerCu, tracePersistent<ANY, SPILL, STATS, ALT, NFA, BATCH>,
+			kBlock, 0);
 		o.blocksPerCu = o.blocksPerCu < 1 ? 1 : o.blocksPerCu;
 		o.cus = o.cus < 1 ? 1 : o.cus;
 		return o;

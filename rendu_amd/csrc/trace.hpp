@@ -85,6 +85,11 @@ struct TraceOptions {
 	                           // uses it when unorderedAny is off
 	bool unorderedAny = false; // any hit: visit the children that pass in node order, no near-first sort
 	                           // (for coherent rays)
+	// Batches of several scenes over one node arena (persistent 4-wide kernels only): ray k starts
+	// at the root of scene rayJob[k], jobRoots[2 j] = {top box min, root code bits}, [2 j + 1] =
+	// {top box max, 0}.
+	const uint32_t * rayJob = nullptr;
+	const float4 * jobRoots = nullptr;
 };
 
 // Launch parameters of the persistent traversal picked by the autotuner (0 = default).

@@ -135,6 +135,52 @@ def run_batch(jobs: List[Job], workers: int = 4, on_done: Optional[Callable[[Job
     return results, wall
 
 
+def render_batch(items, max_paths: int = 0):
+    """Renders several built jobs in one call (rdu_pathtracer_render_batch): every bounce of every
+    job in one set of wavefront launches. items: (PathTracer, Job) pairs. Returns (images, each
+    job's completion seconds from the call's start, render stats)."""
+    import ctypes
+    from ._lib import check, lib
+    n = len(items)
+    arr = (S.BatchJob * n)()
+    images, cams = [], []
+    for k, (pt, job) in enumerate(items):
+        img = np.zeros((job.height, job.width, 3), np.float32)
+        images.append(img)
+        cams.append(S.camera_desc(job.camera))
+        arr[k] = S.BatchJob(pt._h.value, ctypes.pointer(cams[-1]), None, job.samples, job.depth,
+                            img.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), job.width, job.height, 3,
+                            S.RenderOpts(job.seed, job.estimator, job.rr_start, 0))
+    secs = np.zeros(n, np.float64)
+    st = S.RenderStats()
+    check(lib.rdu_pathtracer_render_batch(ctypes.cast(arr, ctypes.c_void_p), n, int(max_paths),
+                                          secs.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st)))
+    return images, secs, {k: getattr(st, k) for k, _ in S.RenderStats._fields_}
+
+
+def run_merged(jobs: List[Job], build_workers: int = 8, max_paths: int = 0):
+    """The batch as one renderer call: the scenes are built on `build_workers` host threads (GPU
+    builds where a job asks for one), then every job renders in one rdu_pathtracer_render_batch
+    call. Returns (results in job order, wall seconds); a job's latency is the build phase plus its
+    completion time inside the call."""
+    b0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=max(1, build_workers)) as ex:
+        built = list(ex.map(_build, jobs))
+    t_built = time.perf_counter() - b0
+    images, secs, st = render_batch([(pt, j) for (pt, _, _), j in zip(built, jobs)], max_paths=max_paths)
+    wall = time.perf_counter() - b0
+    results = []
+    for k, ((pt, t0, t1), img) in enumerate(zip(built, images)):
+        r = JobResult(k, img, t_built + float(secs[k]), t1 - t0, float(secs[k]),
+                      jobs[k].width * jobs[k].height * 2 ** round(np.log2(max(jobs[k].samples, 1))), 0)
+        results.append(r)
+    if results:
+        results[0].rays = int(st["extension_rays"] + st["shadow_rays"])
+        results[0].stats = st
+    del built
+    return results, wall
+
+
 def summary(results: List[JobResult], wall: float) -> dict:
     lat = np.array([r.latency_s for r in results])
     return {

@@ -61,6 +61,13 @@ class RenderOpts(ctypes.Structure):
     _fields_ = [("seed", c_uint32), ("estimator", c_int32), ("rr_start", c_int32), ("max_paths", c_uint32)]
 
 
+class BatchJob(ctypes.Structure):
+    """rdu_batch_job."""
+    _fields_ = [("scene", ctypes.c_void_p), ("camera", ctypes.POINTER(CameraDesc)), ("frame", ctypes.c_void_p),
+                ("samples", ctypes.c_size_t), ("depth", ctypes.c_size_t), ("image", ctypes.POINTER(c_float)),
+                ("width", c_uint32), ("height", c_uint32), ("components", c_uint32), ("opts", RenderOpts)]
+
+
 class RenderStats(ctypes.Structure):
     _fields_ = [("seconds", ctypes.c_double), ("paths", ctypes.c_uint64), ("extension_rays", ctypes.c_uint64),
                 ("shadow_rays", ctypes.c_uint64), ("samples", c_uint32), ("passes", c_uint32),
