@@ -23,7 +23,7 @@ struct RenderPaths {
 	int device = 0;
 	DevBuffer pRays, pHits, pAtten, pColor, pQ0, pQ1, pCounts, pTotals, pSRays, pSContrib, pSQ0, pSQ1, pOcc, pSHits,
 		pGstack, pImage, pMis;
-	DevBuffer pChunk, pJob; // batch passes: each path's chunk and job (grown by the first batch render)
+	DevBuffer pChunk, pJob, pOverflow; // batch passes: each path's chunk and job, the shading overflow list
 	size_t pathCap = 0;             // paths every path buffer above holds (0 while a growth is unfinished)
 	uint32_t * walkLeft = nullptr;  // pinned host word for the shadow walk's count (scenes with cut-outs)
 };
@@ -210,8 +210,9 @@ int ensurePathsFor(RenderPaths & R, const DevBVH & bvh, bool masked, size_t P, b
 			return st;
 		R.pathCap = P;
 	}
-	if(batch && (R.pChunk.bytes < P * 4 || R.pJob.bytes < P * 4) &&
-		((st = R.pChunk.reserveOn(P * 4, stream)) || (st = R.pJob.reserveOn(P * 4, stream))))
+	if(batch && (R.pChunk.bytes < P * 4 || R.pJob.bytes < P * 4 || R.pOverflow.bytes < P * 4) &&
+		((st = R.pChunk.reserveOn(P * 4, stream)) || (st = R.pJob.reserveOn(P * 4, stream)) ||
+			(st = R.pOverflow.reserveOn(P * 4, stream))))
 		return st;
 	if(!R.walkLeft && masked &&
 		(st = hipCheck(hipHostMalloc(reinterpret_cast<void **>(&R.walkLeft), sizeof(uint32_t), hipHostMallocDefault),
@@ -1159,6 +1160,7 @@ int rdu_pathtracer_render_batch(const rdu_batch_job * jobs, size_t n, uint32_t m
 	B.chunks = static_cast<const PtChunk *>(L.chunks.ptr);
 	B.pathChunk = static_cast<uint32_t *>(R->pChunk.ptr);
 	B.pathJob = static_cast<uint32_t *>(R->pJob.ptr);
+	B.overflow = static_cast<uint32_t *>(R->pOverflow.ptr);
 	B.roots = static_cast<const float4 *>(L.roots.ptr);
 	hipEvent_t start;
 	if((st = hipCheck(hipEventCreate(&start), "hipEventCreate")))

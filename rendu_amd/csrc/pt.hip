@@ -131,6 +131,7 @@ __global__ void ptResetCounts(uint32_t * counts, uint32_t active, uint32_t sampl
 		counts[kCountNext] = 0;
 		counts[kCountShadow] = 0;
 		counts[kCountShadowNext] = 0;
+		counts[kCountOverflow] = 0;
 		const uint32_t base = counts[kCountBaseNext];
 		counts[kCountBase] = base;
 		counts[kCountBaseNext] = base + samplesInPass;
@@ -153,6 +154,7 @@ __global__ void ptAdvance(uint32_t * counts, unsigned long long * totals) {
 		counts[kCountNext] = 0;
 		counts[kCountShadow] = 0;
 		counts[kCountShadowNext] = 0;
+		counts[kCountOverflow] = 0;
 	}
 }
 
@@ -597,15 +599,85 @@ __global__ __launch_bounds__(kPtBlock) void ptGenerateBatch(PtBatch B, PtPaths P
 	B.pathJob[p] = c.job;
 }
 
-// ptShade over a batch queue: the lanes of a wavefront are served one chunk at a time (the chunk's
-// and its job's descriptors are wavefront-uniform, so they are read with scalar loads); paths of
-// one chunk are contiguous in the queues, so a wavefront usually needs one round.
+// A batch job's descriptor read as a wavefront-uniform value: the descriptor is read through the
+// constant address space (scalar loads, as a kernel argument would be), and the scene's pointers are
+// tagged as global memory, as they are when they arrive as kernel arguments, and kept in scalar
+// registers, so the scene's loads take a scalar base address.
+template <class T> __device__ inline T * globalPtr(T * p) {
+	const uint64_t v = uint64_t(p);
+	const uint32_t lo = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(v))));
+	const uint32_t hi = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(v >> 32))));
+	return (T *)(__attribute__((address_space(1))) T *)((uint64_t(hi) << 32) | lo);
+}
+
+template <class T> __device__ inline const T & constantRef(const T * p) {
+	return *(const T *)(const __attribute__((address_space(4))) T *)p;
+}
+
+// `j` must be the same in every active lane; it is read from the first one so that the compiler
+// keeps the descriptor's address (and so its loads) scalar.
+__device__ inline PtJob uniformJob(const PtJob * jobs, uint32_t j) {
+	PtJob J = constantRef(jobs + __builtin_amdgcn_readfirstlane(j));
+	DevScene & S = J.S;
+	S.texels = globalPtr(S.texels);
+	S.textures = globalPtr(S.textures);
+	S.nrm = globalPtr(S.nrm);
+	S.tan = globalPtr(S.tan);
+	S.bit = globalPtr(S.bit);
+	S.uv = globalPtr(S.uv);
+	S.idx = globalPtr(S.idx);
+	S.meshes = globalPtr(S.meshes);
+	S.objects = globalPtr(S.objects);
+	S.materials = globalPtr(S.materials);
+	S.lights = globalPtr(S.lights);
+	S.envPmf = globalPtr(S.envPmf);
+	S.envRowCdf = globalPtr(S.envRowCdf);
+	S.envColCdf = globalPtr(S.envColCdf);
+	S.emTris = globalPtr(S.emTris);
+	S.emCdf = globalPtr(S.emCdf);
+	S.emBase = globalPtr(S.emBase);
+	return J;
+}
+
+__device__ inline PtChunk uniformChunk(const PtChunk * chunks, uint32_t k) {
+	return constantRef(chunks + __builtin_amdgcn_readfirstlane(k));
+}
+
+// ptShade over a batch queue. A wavefront serves the lanes of its first lane's chunk (whose chunk
+// and job descriptors are then wavefront-uniform and read with scalar loads); lanes of other chunks
+// (where a wavefront's queue slice straddles two chunks) go to the overflow list, which
+// ptShadeBatchOverflow serves chunk by chunk. Paths of one chunk are contiguous in the queues, so
+// the overflow is small, and the main kernel keeps the registers of a single-scene shading step.
 __global__ __launch_bounds__(kPtBlock) void ptShadeBatch(const PtJob * __restrict__ jobs,
-	const PtChunk * __restrict__ chunks, const uint32_t * __restrict__ pathChunk, PtPaths P, const uint32_t * queueIn,
-	uint32_t * queueOut, uint32_t * counts, uint32_t bounce) {
+	const PtChunk * __restrict__ chunks, const uint32_t * __restrict__ pathChunk, uint32_t * __restrict__ overflow,
+	PtPaths P, const uint32_t * queueIn, uint32_t * queueOut, uint32_t * counts, uint32_t bounce) {
 	const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
 	const uint32_t n = counts[kCountActive];
-	bool todo = i < n;
+	const bool todo = i < n;
+	const uint32_t k = todo ? pathChunk[queueIn[i]] : 0u;
+	const uint64_t m = __ballot(todo);
+	if(!m)
+		return;
+	const uint32_t leader = uint32_t(__ffsll((unsigned long long)m) - 1);
+	const uint32_t kk = uint32_t(__builtin_amdgcn_readlane(int(k), int(leader)));
+	const bool mine = todo && k == kk;
+	appendQueue(todo && !mine, i, overflow, counts + kCountOverflow);
+	if(mine) {
+		const PtChunk c = uniformChunk(chunks, kk);
+		const PtJob J = uniformJob(jobs, c.job);
+		shadeEntry<false>(J.S, J.f, P, queueIn, queueOut, counts, bounce, J.opt, PtCache{}, i, n, c.sampleBase, c.seed,
+			c.pathBegin);
+	}
+}
+
+// The overflow entries of a bounce, one chunk at a time per wavefront.
+__global__ __launch_bounds__(kPtBlock) void ptShadeBatchOverflow(const PtJob * __restrict__ jobs,
+	const PtChunk * __restrict__ chunks, const uint32_t * __restrict__ pathChunk, const uint32_t * __restrict__ overflow,
+	PtPaths P, const uint32_t * queueIn, uint32_t * queueOut, uint32_t * counts, uint32_t bounce) {
+	const uint32_t o = blockIdx.x * kPtBlock + threadIdx.x;
+	const uint32_t n = counts[kCountActive];
+	bool todo = o < counts[kCountOverflow];
+	const uint32_t i = todo ? overflow[o] : 0u;
 	const uint32_t k = todo ? pathChunk[queueIn[i]] : 0u;
 	for(;;) {
 		const uint64_t m = __ballot(todo);
@@ -614,8 +686,8 @@ __global__ __launch_bounds__(kPtBlock) void ptShadeBatch(const PtJob * __restric
 		const uint32_t leader = uint32_t(__ffsll((unsigned long long)m) - 1);
 		const uint32_t kk = uint32_t(__builtin_amdgcn_readlane(int(k), int(leader)));
 		if(todo && k == kk) {
-			const PtChunk c = chunks[kk];
-			const PtJob & J = jobs[c.job];
+			const PtChunk c = uniformChunk(chunks, kk);
+			const PtJob J = uniformJob(jobs, c.job);
 			shadeEntry<false>(J.S, J.f, P, queueIn, queueOut, counts, bounce, J.opt, PtCache{}, i, n, c.sampleBase, c.seed,
 				c.pathBegin);
 			todo = false;
@@ -623,7 +695,7 @@ __global__ __launch_bounds__(kPtBlock) void ptShadeBatch(const PtJob * __restric
 	}
 }
 
-// ptShadowWalk over a batch queue, one job at a time per wavefront (as ptShadeBatch).
+// ptShadowWalk over a batch queue, one job at a time per wavefront.
 __global__ __launch_bounds__(kPtBlock) void ptShadowWalkBatch(PtBatch B, PtPaths P, const uint32_t * queueIn,
 	uint32_t * queueOut, uint32_t * counts) {
 	const uint32_t i = blockIdx.x * kPtBlock + threadIdx.x;
@@ -637,7 +709,7 @@ __global__ __launch_bounds__(kPtBlock) void ptShadowWalkBatch(PtBatch B, PtPaths
 		const uint32_t leader = uint32_t(__ffsll((unsigned long long)m) - 1);
 		const uint32_t jj = uint32_t(__builtin_amdgcn_readlane(int(j), int(leader)));
 		if(todo && j == jj) {
-			walkEntry(B.jobs[jj].S, P, queueIn, queueOut, counts, i, n);
+			walkEntry(uniformJob(B.jobs, jj).S, P, queueIn, queueOut, counts, i, n);
 			todo = false;
 		}
 	}
@@ -835,7 +907,9 @@ int ptRenderBatchPass(const PtBatch & B, const DevBVH & arena, const PtPaths & P
 		if(e)
 			return e;
 		hipLaunchKernelGGL(ptShadeBatch, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, B.jobs, B.chunks, B.pathChunk,
-			P, P.queue[q], P.queue[1 - q], P.counts, b);
+			B.overflow, P, P.queue[q], P.queue[1 - q], P.counts, b);
+		hipLaunchKernelGGL(ptShadeBatchOverflow, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, B.jobs, B.chunks,
+			B.pathChunk, B.overflow, P, P.queue[q], P.queue[1 - q], P.counts, b);
 		if(shadows) {
 			if(!masked) {
 				TraceOptions so;

@@ -27,7 +27,8 @@ enum : int {
 	kCountBase = 4,
 	kCountBaseNext = 5,
 	kCountSeed = 6,
-	kCountSlots = 7
+	kCountOverflow = 7, // batch passes: queue entries left for the second shading launch of a bounce
+	kCountSlots = 8
 };
 
 // Path state, structure of arrays indexed by path id p = sample_in_pass * pixels + pixel.
@@ -137,6 +138,7 @@ struct PtBatch {
 	uint32_t nSlots = 0;              // pixels summed over the chunks
 	uint32_t * pathChunk = nullptr;   // per path
 	uint32_t * pathJob = nullptr;     // per path (the traversal's root index)
+	uint32_t * overflow = nullptr;    // per path: queue entries whose wavefront served another chunk
 	const float4 * roots = nullptr;   // per job: {top min, root code bits}, {top max, 0}
 };
 // One batch pass of nPaths paths over the arena tree; maxDepth is the largest job depth;

@@ -177,7 +177,11 @@ __device__ inline f3 sampleCubemap(const DevScene & s, const int32_t * faces, f3
 	}
 	x = 0.5f * (x / denom) + 0.5f;
 	y = 0.5f * (-y / denom) + 0.5f;
-	const int tex = faces[side];
+	// Face texture by masks rather than an indexed read, so a descriptor copied into registers is
+	// not forced to memory.
+	int tex = 0;
+	for(int k = 0; k < 6; ++k)
+		tex |= faces[k] & -int(side == k);
 	const DevTexture t = s.textures[tex];
 	const float eps = 1.0f / float(t.w < t.h ? t.w : t.h);
 	x = clampf(x, 0.0f + eps, 1.0f - eps);
@@ -253,7 +257,7 @@ __device__ inline f3 skyEval(const DevScene & s, int table, f3 rayOrigin, f3 ray
 }
 
 // Extension: latitude-longitude environment (u along the azimuth from -z towards +x, v from +y).
-__device__ inline f3 sampleLatLong(const DevScene & s, int tex, f3 d) {
+__device__ __forceinline__ f3 sampleLatLong(const DevScene & s, int tex, f3 d) {
 	const float u = 0.5f + atan2f(d.x, -d.z) * (0.5f * kOneOverPi);
 	const float v = acosf(clampf(d.y, -1.0f, 1.0f)) * kOneOverPi;
 	return rgbl(s, tex, u, v);
