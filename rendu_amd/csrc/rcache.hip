@@ -1,14 +1,18 @@
 // Radiance cache kernels for gfx950: hash-grid and spherical-harmonics encodings, fused 5 x 64 MLP
 // forward / backward on bf16 MFMA with fp32 accumulation, split-K weight-gradient GEMMs and Adam.
 //
-// MFMA mapping (v_mfma_f32_16x16x16_bf16, one wavefront): D[16x16] += A[16x16] B[16x16] with
-//   A[m][k] in lane m + 16 * (k / 4), element k % 4;   B[k][n] in lane n + 16 * (k / 4), element k % 4;
+// MFMA mapping (one wavefront). gfx950's v_mfma_f32_16x16x32_bf16: D[16x16] += A[16x32] B[32x16], with
+//   A[m][k] in lane m + 16 * (k / 8), element k % 8;   B[k][n] in lane n + 16 * (k / 8), element k % 8;
 //   D[m][n] in lane n + 16 * (m / 4), element m % 4.
 // A wavefront evaluates 16 samples at a time with layers computed transposed, Y^T = W X^T:
 // A = weight tile (m = output feature, k = input feature), B = activations (k = feature,
 // n = sample). The D layout of output tile t (lane = sample + 16 g, element e: feature 16 t + 4 g + e)
-// is exactly the B layout of input chunk t of the next layer, so activations stay in registers
-// between layers. The backward pass uses the same structure with transposed weight tiles.
+// is the layout of 16-feature chunk t of the next layer's input. One 32-deep product takes chunks t
+// and t + 1 side by side (elements 0-3 from chunk t, 4-7 from chunk t + 1, in both A and B): the
+// product sums over k in any order A and B agree on, so activations stay in registers between
+// layers with no shuffle. A remaining 16-deep chunk (the 48-wide input, the 16-row output layer's
+// transpose) uses v_mfma_f32_16x16x16_bf16 (same D layout; k = 4 g + element). The backward pass
+// uses the same structure with transposed weight tiles.
 #include "rcache.hpp"
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -19,6 +23,8 @@ namespace rdu {
 namespace {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4v4 __attribute__((ext_vector_type(4)));
 
 constexpr int kRcBlock = 256; // 4 wavefronts
@@ -66,7 +72,14 @@ __device__ __forceinline__ uint32_t nonzero4(s4 v) {
 	return (v.x != 0 ? 1u : 0u) | (v.y != 0 ? 2u : 0u) | (v.z != 0 ? 4u : 0u) | (v.w != 0 ? 8u : 0u);
 }
 
+// 16-deep product (one chunk of A and B).
 __device__ __forceinline__ f4v4 mfma(s4 a, s4 b, f4v4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+// 32-deep product over two chunks (a0, b0) and (a1, b1).
+__device__ __forceinline__ f4v4 mfma2(s4 a0, s4 a1, s4 b0, s4 b1, f4v4 c) {
+	const s8 a = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+	const s8 b = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+	return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
+}
 
 // A operand: W[mt*16 + m][kt*16 + 4g .. +3] (row-major [out][in], in = ld).
 __device__ __forceinline__ s4 loadA(const uint16_t * W, int ld, int mt, int kt, int lane) {
@@ -275,8 +288,11 @@ __global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t 
 			for(int mt = 0; mt < 4; ++mt) {
 				f4v4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-				for(int kt = 0; kt < kts; ++kt)
-					acc = mfma(loadA(Wl, kInDim[l], mt, kt, lane), X[l][kt], acc);
+				for(int kt = 0; kt + 1 < kts; kt += 2)
+					acc = mfma2(loadA(Wl, kInDim[l], mt, kt, lane), loadA(Wl, kInDim[l], mt, kt + 1, lane), X[l][kt],
+						X[l][kt + 1], acc);
+				if(kts & 1)
+					acc = mfma(loadA(Wl, kInDim[l], mt, kts - 1, lane), X[l][kts - 1], acc);
 				const float r[4] = {acc.x > 0.0f ? acc.x : 0.0f, acc.y > 0.0f ? acc.y : 0.0f, acc.z > 0.0f ? acc.z : 0.0f,
 					acc.w > 0.0f ? acc.w : 0.0f};
 				X[l + 1][mt] = pack4(r);
@@ -288,8 +304,9 @@ __global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t 
 		}
 		// Output layer (rows 0..2 used).
 		f4v4 Y = {0.0f, 0.0f, 0.0f, 0.0f};
-		for(int kt = 0; kt < 4; ++kt)
-			Y = mfma(loadA(W + kOffW[5], kRcWidth, 0, kt, lane), X[5][kt], Y);
+		for(int kt = 0; kt < 4; kt += 2)
+			Y = mfma2(loadA(W + kOffW[5], kRcWidth, 0, kt, lane), loadA(W + kOffW[5], kRcWidth, 0, kt + 1, lane), X[5][kt],
+				X[5][kt + 1], Y);
 		if(!TRAIN) {
 			if(valid && g == 0)
 				out[idx] = make_float4(Y.x, Y.y, Y.z, 0.0f);
@@ -321,8 +338,11 @@ __global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t 
 			for(int mt = 0; mt < 4; ++mt) {
 				f4v4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-				for(int kt = 0; kt < kts; ++kt)
-					acc = mfma(loadAT(Wl, kInDim[l], mt, kt, lane), G[l][kt], acc);
+				for(int kt = 0; kt + 1 < kts; kt += 2)
+					acc = mfma2(loadAT(Wl, kInDim[l], mt, kt, lane), loadAT(Wl, kInDim[l], mt, kt + 1, lane), G[l][kt],
+						G[l][kt + 1], acc);
+				if(kts & 1)
+					acc = mfma(loadAT(Wl, kInDim[l], mt, kts - 1, lane), G[l][kts - 1], acc);
 				// ReLU derivative: X[l][mt] > 0 (post-activation of layer l - 1).
 				float r[4];
 				const float a4[4] = {acc.x, acc.y, acc.z, acc.w};
@@ -336,8 +356,8 @@ __global__ __launch_bounds__(kRcBlock) void rcFused(RcGrid grid, const uint16_t 
 		// harmonics have no parameters).
 		for(int mt = 0; mt < 2; ++mt) {
 			f4v4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-			for(int kt = 0; kt < 4; ++kt)
-				acc = mfma(loadAT(W, kRcIn, mt, kt, lane), G[0][kt], acc);
+			for(int kt = 0; kt < 4; kt += 2)
+				acc = mfma2(loadAT(W, kRcIn, mt, kt, lane), loadAT(W, kRcIn, mt, kt + 1, lane), G[0][kt], G[0][kt + 1], acc);
 			if(valid) {
 				const int l0 = 8 * mt + 2 * g;
 				recordLevel(grid, tb, idx, l0, p, acc.x, acc.y);
@@ -378,22 +398,18 @@ __global__ __launch_bounds__(kRcBlock) void rcWeightGrad(RcTrainBuffers tb, uint
 	const uint32_t cap = tb.capacity;
 	const uint16_t * Gr = tb.g + size_t(kGRow[layer] + mt * 16 + (lane & 15)) * cap;
 	const uint16_t * Xr = tb.x + size_t(kXRow[layer] + nt * 16 + (lane & 15)) * cap;
+	// k = sample: lane group g takes samples s + 4 g .. + 3 of each 16-sample chunk.
+	const auto chunkAt = [&](const uint16_t * row, uint32_t s) {
+		const uint2 u = *reinterpret_cast<const uint2 *>(row + s + 4u * uint32_t(lane >> 4));
+		const s4 r = {short(u.x & 0xffffu), short(u.x >> 16), short(u.y & 0xffffu), short(u.y >> 16)};
+		return r;
+	};
 	f4v4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-	for(uint32_t s = s0; s < s1; s += 16) {
-		const uint32_t k = s + 4u * uint32_t(lane >> 4);
-		const uint2 ga = *reinterpret_cast<const uint2 *>(Gr + k);
-		const uint2 xb = *reinterpret_cast<const uint2 *>(Xr + k);
-		s4 a, b;
-		a.x = short(ga.x & 0xffffu);
-		a.y = short(ga.x >> 16);
-		a.z = short(ga.y & 0xffffu);
-		a.w = short(ga.y >> 16);
-		b.x = short(xb.x & 0xffffu);
-		b.y = short(xb.x >> 16);
-		b.z = short(xb.y & 0xffffu);
-		b.w = short(xb.y >> 16);
-		acc = mfma(a, b, acc);
-	}
+	uint32_t s = s0;
+	for(; s + 32u <= s1; s += 32u)
+		acc = mfma2(chunkAt(Gr, s), chunkAt(Gr, s + 16u), chunkAt(Xr, s), chunkAt(Xr, s + 16u), acc);
+	if(s < s1)
+		acc = mfma(chunkAt(Gr, s), chunkAt(Xr, s), acc);
 	// D[m][n]: lane n + 16 (m / 4), element m % 4 -> gw[layer][mt*16 + m][nt*16 + n].
 	const int ld = kInDim[layer];
 	float * base = gw + kOffW[layer];
