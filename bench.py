@@ -579,6 +579,11 @@ def main():
     res_j, wall_j = J.run_batch(job_list, workers=4)
     extra["jobs64_4_threads_list_order"] = J.summary(res_j, wall_j)
     del res_j
+    # The same jobs as 4 renderer calls of 16 jobs each (rdu_pathtracer_render_batch: one set of
+    # wavefront launches per bounce for all jobs of a call), started as their builds (3 threads) end.
+    res_j, wall_j = J.run_merged(job_list, build_workers=3, groups=4)
+    extra["jobs64_merged_calls"] = dict(J.summary(res_j, wall_j), calls=4, build_workers=3)
+    del res_j
     for j in job_list:
         j.gpu_build = True
     J.run_batch(job_list, workers=8, order="lpt")  # warm-up of the GPU build's own temporaries

@@ -158,26 +158,59 @@ def render_batch(items, max_paths: int = 0):
     return images, secs, {k: getattr(st, k) for k, _ in S.RenderStats._fields_}
 
 
-def run_merged(jobs: List[Job], build_workers: int = 8, max_paths: int = 0):
-    """The batch as one renderer call: the scenes are built on `build_workers` host threads (GPU
-    builds where a job asks for one), then every job renders in one rdu_pathtracer_render_batch
-    call. Returns (results in job order, wall seconds); a job's latency is the build phase plus its
-    completion time inside the call."""
+def merge_groups(jobs: List[Job], groups: int) -> List[List[int]]:
+    """Job indices longest first (estimated_cost), cut into at most `groups` consecutive non-empty
+    groups of about equal estimated cost."""
+    if groups < 1:
+        raise ValueError("groups must be >= 1")
+    seq = sorted(range(len(jobs)), key=lambda i: -estimated_cost(jobs[i]))
+    cost = [estimated_cost(jobs[i]) for i in seq]
+    total = sum(cost) or 1.0
+    parts: List[List[int]] = [[] for _ in range(groups)]
+    acc = 0.0
+    for i, c in zip(seq, cost):
+        parts[min(groups - 1, int(acc / total * groups))].append(i)
+        acc += c
+    return [g for g in parts if g]
+
+
+def run_merged(jobs: List[Job], build_workers: int = 3, groups: int = 4, max_paths: int = 0):
+    """The batch as a few renderer calls that each render many jobs at once
+    (rdu_pathtracer_render_batch: every bounce of every job of the call in one set of wavefront
+    launches). Jobs are taken longest first (estimated_cost) and cut into `groups` consecutive
+    groups of about equal estimated cost; the scenes are built in that order on `build_workers`
+    host threads, and each group's call starts, on a thread of its own, as soon as its own jobs are
+    built, so that builds overlap earlier groups' renders and the groups' calls overlap each other
+    on the GPU. Returns (results in job order, wall seconds); a job's latency runs from the batch
+    start to its completion inside its group's call."""
+    if build_workers < 1:
+        raise ValueError("build_workers must be >= 1")
+    parts = merge_groups(jobs, groups)
+    seq = [i for g in parts for i in g]
+    results: List[Optional[JobResult]] = [None] * len(jobs)
     b0 = time.perf_counter()
-    with ThreadPoolExecutor(max_workers=max(1, build_workers)) as ex:
-        built = list(ex.map(_build, jobs))
-    t_built = time.perf_counter() - b0
-    images, secs, st = render_batch([(pt, j) for (pt, _, _), j in zip(built, jobs)], max_paths=max_paths)
+    with ThreadPoolExecutor(max_workers=build_workers) as bex:
+        builds = {i: bex.submit(_build, jobs[i]) for i in seq}
+
+        def render_group(members):
+            built = [builds[i].result() for i in members]
+            c0 = time.perf_counter() - b0
+            images, secs, st = render_batch([(pt, jobs[i]) for (pt, _, _), i in zip(built, members)],
+                                            max_paths=max_paths)
+            for k, i in enumerate(members):
+                pt, t0, t1 = built[k]
+                spp = 2 ** round(np.log2(max(jobs[i].samples, 1)))
+                r = JobResult(i, images[k], c0 + float(secs[k]), t1 - t0, float(secs[k]),
+                              jobs[i].width * jobs[i].height * spp, 0, started_s=c0)
+                results[i] = r
+            results[members[0]].rays = int(st["extension_rays"] + st["shadow_rays"])
+            results[members[0]].stats = st
+            del built
+
+        with ThreadPoolExecutor(max_workers=len(parts)) as rex:
+            list(rex.map(render_group, parts))
+        del builds
     wall = time.perf_counter() - b0
-    results = []
-    for k, ((pt, t0, t1), img) in enumerate(zip(built, images)):
-        r = JobResult(k, img, t_built + float(secs[k]), t1 - t0, float(secs[k]),
-                      jobs[k].width * jobs[k].height * 2 ** round(np.log2(max(jobs[k].samples, 1))), 0)
-        results.append(r)
-    if results:
-        results[0].rays = int(st["extension_rays"] + st["shadow_rays"])
-        results[0].stats = st
-    del built
     return results, wall
 
 

@@ -56,11 +56,13 @@ def test_batch_images_equal_alone(max_paths):
     assert np.all(secs > 0) and np.all(secs <= st["seconds"] + 1e-6)
 
 
-def test_merged_runner_matches_alone():
+@pytest.mark.parametrize("groups", [1, 2])
+def test_merged_runner_matches_alone(groups):
     js = _jobs()[:5]
-    log_case("jobs.run_merged (threaded builds, one batch call) vs alone")
-    res, wall = J.run_merged(js, build_workers=4)
+    log_case(f"jobs.run_merged (threaded builds, {groups} batch call(s)) vs alone")
+    res, wall = J.run_merged(js, build_workers=4, groups=groups)
     for j, r in zip(js, res):
         np.testing.assert_array_equal(r.image, J.run_job(j).image)
     summ = J.summary(res, wall)
     assert summ["jobs"] == 5 and summ["jobs_per_s"] > 0
+    assert all(0 < r.render_s <= r.latency_s for r in res)

@@ -44,6 +44,37 @@ def test_estimated_cost_orders_by_work():
         J.run_batch([], build_workers=-1)
 
 
+def test_merge_groups_partition():
+    from rendu_amd import jobs as J
+    js = []
+    for s in range(12):
+        scn, cam, w, h, spp, depth = scenes.random_job(s)
+        js.append(J.Job(scn, cam(w, h), w, h, spp, depth, seed=s))
+    cost = [J.estimated_cost(j) for j in js]
+    order = sorted(range(12), key=lambda i: -cost[i])
+    for groups in (1, 2, 3, 5, 12, 20):
+        parts = J.merge_groups(js, groups)
+        assert 1 <= len(parts) <= min(groups, 12) and all(parts)
+        # consecutive slices of the longest-first order, covering every job once
+        assert [i for g in parts for i in g] == order
+        if groups == 1:
+            assert parts == [order]
+        # a job's group is the share of the total cost its start falls in (the last share takes
+        # the rest): one share per group, increasing along the order
+        tot, acc, shares = sum(cost), 0.0, []
+        for g in parts:
+            sh = set()
+            for i in g:
+                sh.add(min(groups - 1, int(acc / tot * groups)))
+                acc += cost[i]
+            assert len(sh) == 1
+            shares.append(sh.pop())
+        assert shares == sorted(set(shares))
+    assert J.merge_groups([], 3) == []
+    with pytest.raises(ValueError):
+        J.merge_groups(js, 0)
+
+
 @pytest.mark.gpu
 def test_longest_first_batch_matches_sequential():
     from conftest import log_case
