@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--jobs-only", action="store_true",
                     help="only the 64 batched jobs: a warm-up batch, a 1.5 s pause, then the timed batch with "
                          "2 build and 6 render threads (the workload of the job-batch kernel trace)")
+    ap.add_argument("--jobs-one-call", action="store_true",
+                    help="only the 64 jobs, built, then rendered in one rdu_pathtracer_render_batch call: a "
+                         "warm-up call, a 1.5 s pause, then the timed call (the workload of its kernel trace)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device three times: the reference reading of the "
                          "FETCH_SIZE / WRITE_SIZE counters that converts them to bytes")
@@ -110,6 +113,21 @@ def jobs_only():
     time.sleep(1.5)
     res, wall = J.run_batch(job_list, workers=6, order="lpt", build_workers=2)
     print(json.dumps(dict(J.summary(res, wall), workers=6, build_workers=2, order="longest first")), flush=True)
+
+
+def jobs_one_call():
+    from rendu_amd import jobs as J
+    from rendu_amd.pathtracer import PathTracer
+    job_list = job_list_64()
+    items = [(PathTracer(j.scene), j) for j in job_list]
+    J.render_batch(items)
+    time.sleep(1.5)
+    t0 = time.perf_counter()
+    _, secs, st = J.render_batch(items)
+    wall = time.perf_counter() - t0
+    print(json.dumps({"jobs": len(items), "render_wall_s": round(wall, 4), "jobs_per_s_render_only":
+                      round(len(items) / wall, 2), "passes": st["passes"], "paths": st["paths"],
+                      "latency_p50_s": round(float(np.percentile(secs, 50)), 4)}), flush=True)
 
 
 def cache_experiment(pt, scn, cam, ref_spp):
@@ -333,6 +351,9 @@ def main():
         return
     if args.profile_frame:
         profile_frame()
+        return
+    if args.jobs_one_call:
+        jobs_one_call()
         return
     if args.jobs_only:
         jobs_only()

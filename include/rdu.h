@@ -312,8 +312,10 @@ typedef struct {
 
 /* Batched render jobs (no reference counterpart): every job of the list in one set of wavefront
  * launches per bounce. Each pass takes samples of several jobs (longest jobs first) up to max_paths
- * paths (0 = 2^25); their trees are copied into one node arena, rays start at their own job's
- * root, shading reads each wavefront's job descriptor once. Every image equals, bit for bit, what
+ * paths (0 = 2^24); their trees are copied into one node arena, rays start at their own job's
+ * root, shading reads each wavefront's job descriptor once. Up to three passes are in flight on
+ * streams of the call's own, each with its own path state; passes add into the images in order.
+ * Every image equals, bit for bit, what
  * rdu_pathtracer_render gives that scene alone with the same options. job_seconds (n entries, or
  * NULL) receives each job's completion time from the call's start; stats (or NULL) the totals
  * (seconds, paths, rays, passes). The scenes must live on one device and not be rendered

@@ -1008,7 +1008,9 @@ int rdu_pathtracer_render_batch(const rdu_batch_job * jobs, size_t n, uint32_t m
 	int st = setDevice(device);
 	if(st)
 		return st;
-	const size_t budget = max_paths ? max_paths : (size_t(1) << 25);
+	// Paths per pass: with passes in flight on three streams, 2^24 renders the 64-job batch as fast
+	// as 2^25 in half the memory (about 3.2 GB of path state per pass).
+	const size_t budget = max_paths ? max_paths : (size_t(1) << 24);
 
 	// Per job: sample count (power of two), frame, options; the tree's node and triangle counts.
 	struct JobInfo {
@@ -1064,15 +1066,20 @@ int rdu_pathtracer_render_batch(const rdu_batch_job * jobs, size_t n, uint32_t m
 	// Device memory of the batch; freed in stream order when the call returns.
 	struct Local {
 		hipStream_t s;
+		std::vector<hipStream_t> more; // the other streams passes run on
 		DevBuffer arenaNodes, arenaTris, roots, jobsDev, chunks, images;
 		std::vector<hipEvent_t> events;
 		~Local() {
+			for(hipStream_t m : more)
+				(void)hipStreamSynchronize(m);
 			(void)hipStreamSynchronize(s);
 			for(DevBuffer * b : {&arenaNodes, &arenaTris, &roots, &jobsDev, &chunks, &images})
 				b->releaseOn(s);
 			for(hipEvent_t e : events)
 				(void)hipEventDestroy(e);
 			(void)hipStreamSynchronize(s);
+			for(hipStream_t m : more)
+				(void)hipStreamDestroy(m);
 			(void)hipStreamDestroy(s);
 		}
 	} L;
@@ -1111,57 +1118,122 @@ int rdu_pathtracer_render_batch(const rdu_batch_job * jobs, size_t n, uint32_t m
 	arena.tris = arenaTris;
 	arena.wideDepth = wideDepth;
 
-	// Passes: longest jobs first (pixels x samples x depth), each pass taking as many samples of
-	// each unfinished job as fit in the path budget; a job's samples run in order over its passes.
+	// Schedule, made up front: longest jobs first (pixels x samples x depth), each pass taking as
+	// many samples of each unfinished job as fit in the path budget; a job's samples run in order
+	// over its passes.
 	std::vector<size_t> order(n);
 	for(size_t j = 0; j < n; ++j)
 		order[j] = j;
 	std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
 		return info[a].pixels * info[a].spp * jobs[a].depth > info[b].pixels * info[b].spp * jobs[b].depth;
 	});
-	size_t maxPass = 0;
-	for(size_t j = 0; j < n; ++j)
-		maxPass = std::max(maxPass, std::min(info[j].pixels * info[j].spp, std::max(budget, info[j].pixels)));
-	const size_t passCap = std::max(std::min(budget, [&] {
-		size_t t = 0;
-		for(size_t j = 0; j < n; ++j)
-			t += info[j].pixels * info[j].spp;
-		return t;
-	}()), maxPass);
+	struct Pass {
+		size_t chunkBegin = 0, chunkCount = 0, used = 0, slots = 0;
+		std::vector<size_t> finished; // jobs whose last samples this pass renders
+	};
+	std::vector<Pass> schedule;
+	std::vector<PtChunk> allChunks;
+	for(size_t remaining = n; remaining > 0;) {
+		Pass ps;
+		ps.chunkBegin = allChunks.size();
+		for(size_t j : order) {
+			JobInfo & I = info[j];
+			if(I.next >= I.spp)
+				continue;
+			size_t take = (budget - std::min(budget, ps.used)) / I.pixels;
+			if(take == 0 && ps.used == 0)
+				take = 1; // an image larger than the budget: one sample per pass
+			take = std::min(take, I.spp - I.next);
+			if(take == 0)
+				continue;
+			PtChunk c;
+			c.job = uint32_t(j);
+			c.pathBegin = uint32_t(ps.used);
+			c.samples = uint32_t(take);
+			c.sampleBase = uint32_t(I.next);
+			c.seed = jobs[j].opts.seed;
+			c.slotBegin = uint32_t(ps.slots);
+			allChunks.push_back(c);
+			ps.used += take * I.pixels;
+			ps.slots += I.pixels;
+			I.next += take;
+			if(I.next >= I.spp)
+				ps.finished.push_back(j);
+		}
+		ps.chunkCount = allChunks.size() - ps.chunkBegin;
+		remaining -= ps.finished.size();
+		schedule.push_back(std::move(ps));
+	}
+	size_t passCap = 1;
+	for(const Pass & ps : schedule)
+		passCap = std::max(passCap, ps.used);
 	if(passCap > 0xffffffffull)
 		return fail(RDU_ERR_UNSUPPORTED, "batch pass above 2^32 paths");
-	PathsLease R(device, passCap, stream);
-	if((st = ensurePathsFor(*R, arena, masked, passCap, true, stream)))
+	if((st = L.chunks.reserveOn(allChunks.size() * sizeof(PtChunk), stream)) ||
+		(st = hipCheck(hipMemcpyAsync(L.chunks.ptr, allChunks.data(), allChunks.size() * sizeof(PtChunk),
+						   hipMemcpyHostToDevice, stream),
+				"chunks")))
 		return st;
-	if((st = L.chunks.reserveOn(n * sizeof(PtChunk), stream)) ||
-		(st = hipCheck(hipMemsetAsync(R->pTotals.ptr, 0, 16, stream), "hipMemsetAsync")))
+
+	// Passes in flight: up to kBatchLanes, each lane with its own stream and path state, so that one
+	// pass's traversal overlaps another's shading. Passes accumulate in schedule order (each waits
+	// for the previous pass's accumulation), so every pixel sums its samples in order.
+	constexpr size_t kBatchLanes = 3;
+	const size_t lanes = std::min(kBatchLanes, schedule.size());
+	struct Lane {
+		hipStream_t s = nullptr;
+		std::unique_ptr<PathsLease> lease;
+		PtPaths P;
+		PtScratch scratch;
+		PtBatch B;
+	};
+	std::vector<Lane> lane(lanes);
+	hipEvent_t setupDone;
+	if((st = hipCheck(hipEventCreateWithFlags(&setupDone, hipEventDisableTiming), "hipEventCreate")))
 		return st;
-	PtPaths P;
-	P.rays = static_cast<float4 *>(R->pRays.ptr);
-	P.hits = static_cast<rdu_hit *>(R->pHits.ptr);
-	P.atten = static_cast<float4 *>(R->pAtten.ptr);
-	P.color = static_cast<float4 *>(R->pColor.ptr);
-	P.queue[0] = static_cast<uint32_t *>(R->pQ0.ptr);
-	P.queue[1] = static_cast<uint32_t *>(R->pQ1.ptr);
-	P.counts = static_cast<uint32_t *>(R->pCounts.ptr);
-	P.totals = static_cast<unsigned long long *>(R->pTotals.ptr);
-	P.shadowRays = static_cast<float4 *>(R->pSRays.ptr);
-	P.shadowContrib = static_cast<float4 *>(R->pSContrib.ptr);
-	P.shadowQueue[0] = static_cast<uint32_t *>(R->pSQ0.ptr);
-	P.shadowQueue[1] = static_cast<uint32_t *>(R->pSQ1.ptr);
-	P.occluded = static_cast<uint8_t *>(R->pOcc.ptr);
-	P.shadowHits = static_cast<rdu_hit *>(R->pSHits.ptr);
-	P.mis = static_cast<float2 *>(R->pMis.ptr);
-	PtScratch scratch;
-	scratch.gstack = static_cast<int *>(R->pGstack.ptr);
-	scratch.walkLeft = R->walkLeft;
-	PtBatch B;
-	B.jobs = static_cast<const PtJob *>(L.jobsDev.ptr);
-	B.chunks = static_cast<const PtChunk *>(L.chunks.ptr);
-	B.pathChunk = static_cast<uint32_t *>(R->pChunk.ptr);
-	B.pathJob = static_cast<uint32_t *>(R->pJob.ptr);
-	B.overflow = static_cast<uint32_t *>(R->pOverflow.ptr);
-	B.roots = static_cast<const float4 *>(L.roots.ptr);
+	L.events.push_back(setupDone);
+	if((st = hipCheck(hipEventRecord(setupDone, stream), "hipEventRecord")))
+		return st;
+	for(size_t k = 0; k < lanes; ++k) {
+		Lane & ln = lane[k];
+		if(k == 0)
+			ln.s = stream;
+		else {
+			if((st = hipCheck(hipStreamCreateWithFlags(&ln.s, hipStreamNonBlocking), "hipStreamCreate")))
+				return st;
+			L.more.push_back(ln.s);
+			if((st = hipCheck(hipStreamWaitEvent(ln.s, setupDone, 0), "hipStreamWaitEvent")))
+				return st;
+		}
+		ln.lease = std::make_unique<PathsLease>(device, passCap, ln.s);
+		RenderPaths & R = **ln.lease;
+		if((st = ensurePathsFor(R, arena, masked, passCap, true, ln.s)) ||
+			(st = hipCheck(hipMemsetAsync(R.pTotals.ptr, 0, 16, ln.s), "hipMemsetAsync")))
+			return st;
+		PtPaths & P = ln.P;
+		P.rays = static_cast<float4 *>(R.pRays.ptr);
+		P.hits = static_cast<rdu_hit *>(R.pHits.ptr);
+		P.atten = static_cast<float4 *>(R.pAtten.ptr);
+		P.color = static_cast<float4 *>(R.pColor.ptr);
+		P.queue[0] = static_cast<uint32_t *>(R.pQ0.ptr);
+		P.queue[1] = static_cast<uint32_t *>(R.pQ1.ptr);
+		P.counts = static_cast<uint32_t *>(R.pCounts.ptr);
+		P.totals = static_cast<unsigned long long *>(R.pTotals.ptr);
+		P.shadowRays = static_cast<float4 *>(R.pSRays.ptr);
+		P.shadowContrib = static_cast<float4 *>(R.pSContrib.ptr);
+		P.shadowQueue[0] = static_cast<uint32_t *>(R.pSQ0.ptr);
+		P.shadowQueue[1] = static_cast<uint32_t *>(R.pSQ1.ptr);
+		P.occluded = static_cast<uint8_t *>(R.pOcc.ptr);
+		P.shadowHits = static_cast<rdu_hit *>(R.pSHits.ptr);
+		P.mis = static_cast<float2 *>(R.pMis.ptr);
+		ln.scratch.gstack = static_cast<int *>(R.pGstack.ptr);
+		ln.scratch.walkLeft = R.walkLeft;
+		ln.B.jobs = static_cast<const PtJob *>(L.jobsDev.ptr);
+		ln.B.pathChunk = static_cast<uint32_t *>(R.pChunk.ptr);
+		ln.B.pathJob = static_cast<uint32_t *>(R.pJob.ptr);
+		ln.B.overflow = static_cast<uint32_t *>(R.pOverflow.ptr);
+		ln.B.roots = static_cast<const float4 *>(L.roots.ptr);
+	}
 	hipEvent_t start;
 	if((st = hipCheck(hipEventCreate(&start), "hipEventCreate")))
 		return st;
@@ -1169,73 +1241,62 @@ int rdu_pathtracer_render_batch(const rdu_batch_job * jobs, size_t n, uint32_t m
 	if((st = hipCheck(hipEventRecord(start, stream), "hipEventRecord")))
 		return st;
 	std::vector<hipEvent_t> doneAt(n, nullptr);
-	std::vector<std::vector<PtChunk>> passChunks; // kept until the stream has read them
-	uint32_t passes = 0;
-	size_t remainingJobs = n;
-	const TraceTuning tune;
-	while(remainingJobs > 0) {
-		std::vector<PtChunk> chunks;
-		std::vector<size_t> finished;
-		size_t used = 0, slots = 0;
-		for(size_t j : order) {
-			JobInfo & I = info[j];
-			if(I.next >= I.spp)
-				continue;
-			size_t take = (budget - std::min(budget, used)) / I.pixels;
-			if(take == 0 && chunks.empty())
-				take = 1; // an image larger than the budget: one sample per pass
-			take = std::min(take, I.spp - I.next);
-			if(take == 0)
-				continue;
-			PtChunk c;
-			c.job = uint32_t(j);
-			c.pathBegin = uint32_t(used);
-			c.samples = uint32_t(take);
-			c.sampleBase = uint32_t(I.next);
-			c.seed = jobs[j].opts.seed;
-			c.slotBegin = uint32_t(slots);
-			chunks.push_back(c);
-			used += take * I.pixels;
-			slots += I.pixels;
-			I.next += take;
-			if(I.next >= I.spp)
-				finished.push_back(j);
-		}
-		passChunks.push_back(chunks);
-		const std::vector<PtChunk> & pc = passChunks.back();
-		if((st = hipCheck(hipMemcpyAsync(L.chunks.ptr, pc.data(), pc.size() * sizeof(PtChunk), hipMemcpyHostToDevice, stream),
-				"chunks")))
-			return st;
-		B.nChunks = uint32_t(pc.size());
-		B.nSlots = uint32_t(slots);
-		if((st = hipCheck(hipError_t(ptRenderBatchPass(B, arena, P, uint32_t(used), maxDepth, shadows, masked, 16, scratch, tune,
-								 stream)),
-				"batch pass")))
-			return st;
-		++passes;
-		for(size_t j : finished) {
+	// A finished job's image: normalised after its last accumulation, copied back, time-stamped.
+	// Enqueued one pass late, so that the copy (which waits on the host for its stream) finds the
+	// next pass already queued on another stream.
+	const auto deliver = [&](const Pass & ps, hipStream_t s) -> int {
+		for(size_t j : ps.finished) {
 			const JobInfo & I = info[j];
 			const rdu_batch_job & b = jobs[j];
 			hipEvent_t e;
-			if((st = hipCheck(hipEventCreate(&e), "hipEventCreate")))
-				return st;
+			int r = hipCheck(hipEventCreate(&e), "hipEventCreate");
+			if(r)
+				return r;
 			L.events.push_back(e);
-			if((st = hipCheck(hipError_t(ptNormalizeImage(table[j].image, uint32_t(I.pixels), b.components, uint32_t(I.spp),
-									 stream)),
+			if((r = hipCheck(hipError_t(ptNormalizeImage(table[j].image, uint32_t(I.pixels), b.components, uint32_t(I.spp), s)),
 					"normalise")) ||
-				(st = hipCheck(hipMemcpyAsync(b.image, table[j].image, I.pixels * b.components * 4, hipMemcpyDeviceToHost,
-								   stream),
+				(r = hipCheck(hipMemcpyAsync(b.image, table[j].image, I.pixels * b.components * 4, hipMemcpyDeviceToHost, s),
 						"image download")) ||
-				(st = hipCheck(hipEventRecord(e, stream), "hipEventRecord")))
-				return st;
+				(r = hipCheck(hipEventRecord(e, s), "hipEventRecord")))
+				return r;
 			doneAt[j] = e;
-			--remainingJobs;
 		}
+		return RDU_OK;
+	};
+	const TraceTuning tune;
+	hipEvent_t prevAccumulated = nullptr;
+	for(size_t p = 0; p < schedule.size(); ++p) {
+		const Pass & ps = schedule[p];
+		Lane & ln = lane[p % lanes];
+		ln.B.chunks = static_cast<const PtChunk *>(L.chunks.ptr) + ps.chunkBegin;
+		ln.B.nChunks = uint32_t(ps.chunkCount);
+		ln.B.nSlots = uint32_t(ps.slots);
+		if((st = hipCheck(hipError_t(ptRenderBatchPass(ln.B, arena, ln.P, uint32_t(ps.used), maxDepth, shadows, masked, 16,
+								 ln.scratch, tune, ln.s, prevAccumulated)),
+				"batch pass")))
+			return st;
+		hipEvent_t acc;
+		if((st = hipCheck(hipEventCreateWithFlags(&acc, hipEventDisableTiming), "hipEventCreate")))
+			return st;
+		L.events.push_back(acc);
+		if((st = hipCheck(hipEventRecord(acc, ln.s), "hipEventRecord")))
+			return st;
+		prevAccumulated = acc;
+		if(p > 0 && (st = deliver(schedule[p - 1], lane[(p - 1) % lanes].s)))
+			return st;
 	}
-	unsigned long long totals[2] = {0, 0};
-	if((st = hipCheck(hipMemcpyAsync(totals, R->pTotals.ptr, 16, hipMemcpyDeviceToHost, stream), "stats download")) ||
-		(st = hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize")))
+	if((st = deliver(schedule.back(), lane[(schedule.size() - 1) % lanes].s)))
 		return st;
+	const uint32_t passes = uint32_t(schedule.size());
+	unsigned long long totals[2] = {0, 0};
+	for(Lane & ln : lane) {
+		unsigned long long t[2] = {0, 0};
+		if((st = hipCheck(hipMemcpyAsync(t, (*ln.lease)->pTotals.ptr, 16, hipMemcpyDeviceToHost, ln.s), "stats download")) ||
+			(st = hipCheck(hipStreamSynchronize(ln.s), "hipStreamSynchronize")))
+			return st;
+		totals[0] += t[0];
+		totals[1] += t[1];
+	}
 	// Completion time of each job from the call's start: host time to the first enqueued work plus
 	// the device time from there to the job's last copy.
 	const double hostLead = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -883,7 +883,8 @@ int ptRebaseWide(float4 * dst, const float4 * src, uint32_t nodes, int32_t nodeO
 }
 
 int ptRenderBatchPass(const PtBatch & B, const DevBVH & arena, const PtPaths & P, uint32_t nPaths, uint32_t maxDepth,
-	bool shadows, bool masked, int maxShadowLayers, PtScratch & scratch, const TraceTuning & tune, hipStream_t stream) {
+	bool shadows, bool masked, int maxShadowLayers, PtScratch & scratch, const TraceTuning & tune, hipStream_t stream,
+	hipEvent_t accumulateAfter) {
 	if(nPaths == 0 || B.nChunks == 0)
 		return 0;
 	TraceTuning tn = tune;
@@ -957,6 +958,11 @@ int ptRenderBatchPass(const PtBatch & B, const DevBVH & arena, const PtPaths & P
 		}
 		hipLaunchKernelGGL(ptAdvance, dim3(1), dim3(64), 0, stream, P.counts, P.totals);
 		q = 1 - q;
+	}
+	if(accumulateAfter) {
+		const hipError_t he = hipStreamWaitEvent(stream, accumulateAfter, 0);
+		if(he != hipSuccess)
+			return int(he);
 	}
 	hipLaunchKernelGGL(ptAccumulateBatch, dim3(blocksFor(B.nSlots)), dim3(kPtBlock), 0, stream, B, P);
 	return int(hipGetLastError());

@@ -144,8 +144,10 @@ struct PtBatch {
 // One batch pass of nPaths paths over the arena tree; maxDepth is the largest job depth;
 // shadows: some job samples lights; masked: some job has cut-outs (then every job's shadow rays
 // take the cut-out walk, which gives the any-hit answer on scenes without cut-outs).
+// accumulateAfter (or null): an event the pass's accumulation into the job images waits for.
 int ptRenderBatchPass(const PtBatch & B, const DevBVH & arena, const PtPaths & P, uint32_t nPaths, uint32_t maxDepth,
-	bool shadows, bool masked, int maxShadowLayers, PtScratch & scratch, const TraceTuning & tune, hipStream_t stream);
+	bool shadows, bool masked, int maxShadowLayers, PtScratch & scratch, const TraceTuning & tune, hipStream_t stream,
+	hipEvent_t accumulateAfter);
 // Copies a tree's 4-wide nodes into an arena at node offset nodeOffset with its triangles at
 // triangle offset triOffset, rebasing the child codes.
 int ptRebaseWide(float4 * dst, const float4 * src, uint32_t nodes, int32_t nodeOffset, int32_t triOffset,
