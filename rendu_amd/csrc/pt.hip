@@ -670,27 +670,34 @@ __global__ __launch_bounds__(kPtBlock) void ptShadeBatch(const PtJob * __restric
 	}
 }
 
-// The overflow entries of a bounce, one chunk at a time per wavefront.
+// The overflow entries of a bounce, one chunk at a time per wavefront. The list is usually a small
+// fraction of the queue, so a small grid strides over it (the whole grid's loop bound is the
+// same device count).
+constexpr unsigned kOverflowBlocks = 512;
 __global__ __launch_bounds__(kPtBlock) void ptShadeBatchOverflow(const PtJob * __restrict__ jobs,
 	const PtChunk * __restrict__ chunks, const uint32_t * __restrict__ pathChunk, const uint32_t * __restrict__ overflow,
 	PtPaths P, const uint32_t * queueIn, uint32_t * queueOut, uint32_t * counts, uint32_t bounce) {
-	const uint32_t o = blockIdx.x * kPtBlock + threadIdx.x;
 	const uint32_t n = counts[kCountActive];
-	bool todo = o < counts[kCountOverflow];
-	const uint32_t i = todo ? overflow[o] : 0u;
-	const uint32_t k = todo ? pathChunk[queueIn[i]] : 0u;
-	for(;;) {
-		const uint64_t m = __ballot(todo);
-		if(!m)
-			break;
-		const uint32_t leader = uint32_t(__ffsll((unsigned long long)m) - 1);
-		const uint32_t kk = uint32_t(__builtin_amdgcn_readlane(int(k), int(leader)));
-		if(todo && k == kk) {
-			const PtChunk c = uniformChunk(chunks, kk);
-			const PtJob J = uniformJob(jobs, c.job);
-			shadeEntry<false>(J.S, J.f, P, queueIn, queueOut, counts, bounce, J.opt, PtCache{}, i, n, c.sampleBase, c.seed,
-				c.pathBegin);
-			todo = false;
+	const uint32_t total = counts[kCountOverflow];
+	const uint32_t stride = gridDim.x * kPtBlock;
+	for(uint32_t base = blockIdx.x * kPtBlock + (threadIdx.x & ~63u); base < total; base += stride) {
+		const uint32_t o = base + (threadIdx.x & 63u);
+		bool todo = o < total;
+		const uint32_t i = todo ? overflow[o] : 0u;
+		const uint32_t k = todo ? pathChunk[queueIn[i]] : 0u;
+		for(;;) {
+			const uint64_t m = __ballot(todo);
+			if(!m)
+				break;
+			const uint32_t leader = uint32_t(__ffsll((unsigned long long)m) - 1);
+			const uint32_t kk = uint32_t(__builtin_amdgcn_readlane(int(k), int(leader)));
+			if(todo && k == kk) {
+				const PtChunk c = uniformChunk(chunks, kk);
+				const PtJob J = uniformJob(jobs, c.job);
+				shadeEntry<false>(J.S, J.f, P, queueIn, queueOut, counts, bounce, J.opt, PtCache{}, i, n, c.sampleBase,
+					c.seed, c.pathBegin);
+				todo = false;
+			}
 		}
 	}
 }
@@ -909,8 +916,8 @@ int ptRenderBatchPass(const PtBatch & B, const DevBVH & arena, const PtPaths & P
 			return e;
 		hipLaunchKernelGGL(ptShadeBatch, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, B.jobs, B.chunks, B.pathChunk,
 			B.overflow, P, P.queue[q], P.queue[1 - q], P.counts, b);
-		hipLaunchKernelGGL(ptShadeBatchOverflow, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, B.jobs, B.chunks,
-			B.pathChunk, B.overflow, P, P.queue[q], P.queue[1 - q], P.counts, b);
+		hipLaunchKernelGGL(ptShadeBatchOverflow, dim3(std::min(blocksFor(nPaths), kOverflowBlocks)), dim3(kPtBlock), 0,
+			stream, B.jobs, B.chunks, B.pathChunk, B.overflow, P, P.queue[q], P.queue[1 - q], P.counts, b);
 		if(shadows) {
 			if(!masked) {
 				TraceOptions so;
