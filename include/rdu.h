@@ -240,8 +240,13 @@ typedef struct {
 	uint32_t seed;       /* key of the counter-based random stream */
 	int32_t estimator;   /* RDU_ESTIMATOR_* */
 	int32_t rr_start;    /* extended: first bounce subject to Russian roulette */
-	uint32_t max_paths;  /* paths in flight per pass (0 = library default: 2^25 for plain renders,
-	                      * 2^22 for cached ones; the image does not depend on it) */
+	uint32_t max_paths;  /* paths per pass (0 = library default: 2^24 for plain renders with several
+	                      * passes in flight, 2^25 with one, 2^22 for cached renders; the image does
+	                      * not depend on it) */
+	uint32_t passes_in_flight; /* plain renders: passes kept in flight at once on streams of the
+	                            * scene's own, each with its own path state (0 = library default, 3;
+	                            * at most 3; cached, graph-replayed and cut-out scenes take one);
+	                            * the image does not depend on it */
 } rdu_render_opts;
 
 typedef struct rdu_scene rdu_scene;

@@ -35,10 +35,17 @@ std::mutex pathsMutex;
 std::vector<std::unique_ptr<RenderPaths>> & pathsAll = *new std::vector<std::unique_ptr<RenderPaths>>();
 std::vector<RenderPaths *> pathsFree;
 
-// A free set of `device` for passes of P paths: the smallest one that already holds P paths,
-// else the largest (which then grows), else a new one.
-RenderPaths * leasePaths(int device, size_t P) {
+// A free set of `device` for passes of P paths: `prefer` (the set the caller's scene used last,
+// so that a captured pass finds its buffers again) if it is free and holds P paths, else the
+// smallest one that already holds P paths, else the largest (which then grows), else a new one.
+RenderPaths * leasePaths(int device, size_t P, const RenderPaths * prefer) {
 	std::lock_guard<std::mutex> g(pathsMutex);
+	for(size_t i = 0; prefer && i < pathsFree.size(); ++i)
+		if(pathsFree[i] == prefer && prefer->device == device && prefer->pathCap >= P) {
+			RenderPaths * r = pathsFree[i];
+			pathsFree.erase(pathsFree.begin() + std::ptrdiff_t(i));
+			return r;
+		}
 	size_t pick = pathsFree.size();
 	for(size_t i = 0; i < pathsFree.size(); ++i) {
 		const RenderPaths * c = pathsFree[i];
@@ -67,7 +74,8 @@ RenderPaths * leasePaths(int device, size_t P) {
 // (already the case after a successful render; an error may leave work in flight).
 class PathsLease {
 public:
-	PathsLease(int device, size_t P, hipStream_t stream) : _r(leasePaths(device, P)), _stream(stream) {}
+	PathsLease(int device, size_t P, hipStream_t stream, const RenderPaths * prefer = nullptr)
+		: _r(leasePaths(device, P, prefer)), _stream(stream) {}
 	~PathsLease() {
 		(void)hipStreamSynchronize(_stream);
 		std::lock_guard<std::mutex> g(pathsMutex);
@@ -81,6 +89,42 @@ public:
 private:
 	RenderPaths * _r;
 	hipStream_t _stream;
+};
+
+// The kernels' view of a set's path buffers.
+PtPaths pathsOf(const RenderPaths & R) {
+	PtPaths P;
+	P.rays = static_cast<float4 *>(R.pRays.ptr);
+	P.hits = static_cast<rdu_hit *>(R.pHits.ptr);
+	P.atten = static_cast<float4 *>(R.pAtten.ptr);
+	P.color = static_cast<float4 *>(R.pColor.ptr);
+	P.queue[0] = static_cast<uint32_t *>(R.pQ0.ptr);
+	P.queue[1] = static_cast<uint32_t *>(R.pQ1.ptr);
+	P.counts = static_cast<uint32_t *>(R.pCounts.ptr);
+	P.totals = static_cast<unsigned long long *>(R.pTotals.ptr);
+	P.shadowRays = static_cast<float4 *>(R.pSRays.ptr);
+	P.shadowContrib = static_cast<float4 *>(R.pSContrib.ptr);
+	P.shadowQueue[0] = static_cast<uint32_t *>(R.pSQ0.ptr);
+	P.shadowQueue[1] = static_cast<uint32_t *>(R.pSQ1.ptr);
+	P.occluded = static_cast<uint8_t *>(R.pOcc.ptr);
+	P.shadowHits = static_cast<rdu_hit *>(R.pSHits.ptr);
+	P.mis = static_cast<float2 *>(R.pMis.ptr);
+	return P;
+}
+
+// Events of one call, destroyed when it returns.
+struct EventSet {
+	std::vector<hipEvent_t> all;
+	int make(hipEvent_t & e, unsigned flags) {
+		const hipError_t he = hipEventCreateWithFlags(&e, flags);
+		if(he == hipSuccess)
+			all.push_back(e);
+		return he == hipSuccess ? 0 : int(he);
+	}
+	~EventSet() {
+		for(hipEvent_t e : all)
+			(void)hipEventDestroy(e);
+	}
 };
 } // namespace
 
@@ -116,6 +160,8 @@ struct rdu_scene {
 	rdu_cache_stats lastCache{};
 	size_t lastTrainPairs = 0; // pairs left in cTrainQ / cTrainT by the last cached pass
 	hipStream_t stream = nullptr; // renders of this scene run here (created on first render)
+	hipStream_t laneStreams[2] = {nullptr, nullptr}; // further passes in flight of a plain render
+	const rdu::RenderPaths * lastPaths = nullptr;     // path set of the last render (sets live until exit)
 	// Graph replay of full passes (rdu_scene_set_graph): the captured pass, the launch
 	// configuration it was captured with, and that of the last eager full pass.
 	bool useGraph = false;
@@ -141,12 +187,18 @@ struct rdu_scene {
 		// scene is idle and is freed without waiting for the rest of the device.
 		if(stream)
 			(void)hipStreamSynchronize(stream);
+		for(hipStream_t ls : laneStreams)
+			if(ls)
+				(void)hipStreamSynchronize(ls);
 		for(rdu::DevBuffer * b : {&dTexels, &dTextures, &dNrm, &dTan, &dBit, &dUv, &dIdx, &dMeshes, &dObjects, &dMaterials,
 				 &dLights, &dEnvPmf, &dEnvRow, &dEnvCol, &dEmTris, &dEmCdf, &dEmBase, &cQuery, &cQAtten, &cQPath, &cPred, &cRec,
 				 &cRecPath, &cTrainQ, &cTrainT, &cHoldQ, &cHoldT, &cCounts})
 			b->releaseOn(nullptr);
 		if(rc)
 			rdu_raycaster_destroy(rc);
+		for(hipStream_t ls : laneStreams)
+			if(ls)
+				(void)hipStreamDestroy(ls);
 		if(stream)
 			(void)hipStreamDestroy(stream);
 	}
@@ -730,11 +782,19 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 	const PtFrame f = makeFrame(cam, camFrame, width, height, spp);
 
 	const size_t pixels = size_t(width) * height;
-	// Default pass size: 2^25 paths (about 6 GB of path state) for plain renders, so that the sparse
-	// late bounces of a pass run over large queues (a 1080p 64-spp frame in 4 passes takes 0.64x
-	// the time of 32 passes, and the image is bit-identical: samples accumulate in sample order
-	// whatever the split). Cached renders train between passes, so they keep passes of 2^22 paths.
-	const size_t maxPaths = opt.max_paths ? opt.max_paths : (cache ? (size_t(1) << 22) : (size_t(1) << 25));
+	// Passes in flight (plain renders; see the pass loop) and the default pass size: with one pass
+	// at a time, 2^25 paths (about 6 GB of path state), so that the sparse late bounces of a pass run
+	// over large queues (a 1080p 64-spp frame in 4 passes takes 0.64x the time of 32 passes); with
+	// several in flight, 2^24 (the frame: 0.144 s against 0.147 s at 2^25 and 0.172 s with one pass
+	// at a time). The image is the same bit for bit either way: samples accumulate in sample order
+	// whatever the split. Cached renders train between passes, so they keep passes of 2^22 paths.
+	constexpr uint32_t kRenderLanes = 3;
+	static_assert(kRenderLanes == 1 + sizeof(s->laneStreams) / sizeof(s->laneStreams[0]), "a stream per further lane");
+	const uint32_t wantLanes = (!cache && !s->useGraph && !s->dev.anyMasked)
+		? std::min(opt.passes_in_flight ? opt.passes_in_flight : kRenderLanes, kRenderLanes)
+		: 1u;
+	const size_t maxPaths = opt.max_paths ? opt.max_paths
+	                                      : (cache ? (size_t(1) << 22) : (wantLanes > 1 ? size_t(1) << 24 : size_t(1) << 25));
 	if(pixels > (size_t(1) << 31))
 		return fail(RDU_ERR_UNSUPPORTED, "image larger than 2^31 pixels");
 	size_t perPass = maxPaths / pixels;
@@ -746,7 +806,8 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 	if(!s->stream && (st = hipCheck(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate")))
 		return st;
 	hipStream_t stream = s->stream;
-	PathsLease R(s->rc->device, pixels * perPass, stream);
+	PathsLease R(s->rc->device, pixels * perPass, stream, s->lastPaths);
+	s->lastPaths = &*R;
 	if((st = ensurePaths(*R, s, pixels * perPass, stream)))
 		return st;
 	const size_t imgBytes = pixels * components * sizeof(float);
@@ -756,22 +817,7 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 		return st;
 	if((st = hipCheck(hipMemsetAsync(R->pTotals.ptr, 0, 16, stream), "hipMemsetAsync")))
 		return st;
-	PtPaths P;
-	P.rays = static_cast<float4 *>(R->pRays.ptr);
-	P.hits = static_cast<rdu_hit *>(R->pHits.ptr);
-	P.atten = static_cast<float4 *>(R->pAtten.ptr);
-	P.color = static_cast<float4 *>(R->pColor.ptr);
-	P.queue[0] = static_cast<uint32_t *>(R->pQ0.ptr);
-	P.queue[1] = static_cast<uint32_t *>(R->pQ1.ptr);
-	P.counts = static_cast<uint32_t *>(R->pCounts.ptr);
-	P.totals = static_cast<unsigned long long *>(R->pTotals.ptr);
-	P.shadowRays = static_cast<float4 *>(R->pSRays.ptr);
-	P.shadowContrib = static_cast<float4 *>(R->pSContrib.ptr);
-	P.shadowQueue[0] = static_cast<uint32_t *>(R->pSQ0.ptr);
-	P.shadowQueue[1] = static_cast<uint32_t *>(R->pSQ1.ptr);
-	P.occluded = static_cast<uint8_t *>(R->pOcc.ptr);
-	P.shadowHits = static_cast<rdu_hit *>(R->pSHits.ptr);
-	P.mis = static_cast<float2 *>(R->pMis.ptr);
+	PtPaths P = pathsOf(*R);
 	PtScratch scratch;
 	scratch.gstack = static_cast<int *>(R->pGstack.ptr);
 	scratch.walkLeft = R->walkLeft;
@@ -872,7 +918,70 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 	else
 		(void)tuneCacheLookup(sceneTuneKey(s, passPaths), tune);
 	uint32_t pass = 0, graphPasses = 0, graphCaptures = 0;
-	for(size_t base = 0; base < spp; base += perPass, ++pass) {
+	// Plain renders of several passes keep up to wantLanes passes in flight, each lane with its
+	// own stream and path state, so that one pass's traversal overlaps another's shading and the
+	// sparse late bounces of a pass share the GPU with the dense early ones of the next. Each pass's
+	// accumulation waits (by an event) for the previous pass's, so every pixel sums its samples in
+	// order: the image is the same bit for bit as with one pass at a time.
+	const size_t passCount = (spp + perPass - 1) / perPass;
+	const size_t lanes = std::min(size_t(wantLanes), passCount);
+	std::vector<std::unique_ptr<PathsLease>> laneLeases;
+	EventSet events;
+	unsigned long long laneTotals[2] = {0, 0};
+	if(lanes > 1) {
+		struct Lane {
+			hipStream_t s;
+			PtPaths P;
+			PtScratch scratch;
+		};
+		std::vector<Lane> lane(lanes);
+		lane[0] = Lane{stream, P, scratch};
+		hipEvent_t setupDone;
+		if((st = hipCheck(hipError_t(events.make(setupDone, hipEventDisableTiming)), "hipEventCreate")) ||
+			(st = hipCheck(hipEventRecord(setupDone, stream), "hipEventRecord")))
+			return st;
+		for(size_t k = 1; k < lanes; ++k) {
+			hipStream_t & ls = s->laneStreams[k - 1];
+			if(!ls && (st = hipCheck(hipStreamCreateWithFlags(&ls, hipStreamNonBlocking), "hipStreamCreate")))
+				return st;
+			if((st = hipCheck(hipStreamWaitEvent(ls, setupDone, 0), "hipStreamWaitEvent")))
+				return st;
+			laneLeases.push_back(std::make_unique<PathsLease>(s->rc->device, passPaths, ls));
+			RenderPaths & Rk = **laneLeases.back();
+			if((st = ensurePaths(Rk, s, passPaths, ls)) ||
+				(st = hipCheck(hipMemsetAsync(Rk.pTotals.ptr, 0, 16, ls), "hipMemsetAsync")))
+				return st;
+			lane[k].s = ls;
+			lane[k].P = pathsOf(Rk);
+			lane[k].scratch.gstack = static_cast<int *>(Rk.pGstack.ptr);
+			lane[k].scratch.walkLeft = Rk.walkLeft;
+		}
+		hipEvent_t accumulated = nullptr;
+		for(size_t base = 0; base < spp; base += perPass, ++pass) {
+			const size_t n = (spp - base) < perPass ? (spp - base) : perPass;
+			Lane & ln = lane[pass % lanes];
+			hipEvent_t e;
+			if((st = hipCheck(hipError_t(ptSetSampleBase(ln.P.counts, uint32_t(base), opt.seed, ln.s)), "sample base")) ||
+				(st = hipCheck(hipError_t(ptRenderPass(s->dev, s->rc->dev, f, ln.P, uint32_t(n), po, dImage, components,
+								   ln.scratch, nullptr, tune, ln.s, accumulated)),
+						"path tracing pass")) ||
+				(st = hipCheck(hipError_t(events.make(e, hipEventDisableTiming)), "hipEventCreate")) ||
+				(st = hipCheck(hipEventRecord(e, ln.s), "hipEventRecord")))
+				return st;
+			accumulated = e;
+		}
+		if((st = hipCheck(hipStreamWaitEvent(stream, accumulated, 0), "hipStreamWaitEvent")))
+			return st;
+		for(size_t k = 1; k < lanes; ++k) {
+			unsigned long long t[2] = {0, 0};
+			if((st = hipCheck(hipMemcpyAsync(t, lane[k].P.totals, 16, hipMemcpyDeviceToHost, lane[k].s), "stats download")) ||
+				(st = hipCheck(hipStreamSynchronize(lane[k].s), "hipStreamSynchronize")))
+				return st;
+			laneTotals[0] += t[0];
+			laneTotals[1] += t[1];
+		}
+	}
+	for(size_t base = 0; lanes == 1 && base < spp; base += perPass, ++pass) {
 		const size_t n = (spp - base) < perPass ? (spp - base) : perPass;
 		if(cache) {
 			// Shift the training pattern every pass (and with the seed) so every pixel trains.
@@ -957,8 +1066,8 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 	const auto t1 = std::chrono::steady_clock::now();
 	s->last.seconds = std::chrono::duration<double>(t1 - t0).count();
 	s->last.paths = uint64_t(pixels) * spp;
-	s->last.extension_rays = totals[0];
-	s->last.shadow_rays = totals[1];
+	s->last.extension_rays = totals[0] + laneTotals[0];
+	s->last.shadow_rays = totals[1] + laneTotals[1];
 	s->last.samples = uint32_t(spp);
 	s->last.passes = pass;
 	s->last.graph_passes = graphPasses;
@@ -1210,22 +1319,7 @@ int rdu_pathtracer_render_batch(const rdu_batch_job * jobs, size_t n, uint32_t m
 		if((st = ensurePathsFor(R, arena, masked, passCap, true, ln.s)) ||
 			(st = hipCheck(hipMemsetAsync(R.pTotals.ptr, 0, 16, ln.s), "hipMemsetAsync")))
 			return st;
-		PtPaths & P = ln.P;
-		P.rays = static_cast<float4 *>(R.pRays.ptr);
-		P.hits = static_cast<rdu_hit *>(R.pHits.ptr);
-		P.atten = static_cast<float4 *>(R.pAtten.ptr);
-		P.color = static_cast<float4 *>(R.pColor.ptr);
-		P.queue[0] = static_cast<uint32_t *>(R.pQ0.ptr);
-		P.queue[1] = static_cast<uint32_t *>(R.pQ1.ptr);
-		P.counts = static_cast<uint32_t *>(R.pCounts.ptr);
-		P.totals = static_cast<unsigned long long *>(R.pTotals.ptr);
-		P.shadowRays = static_cast<float4 *>(R.pSRays.ptr);
-		P.shadowContrib = static_cast<float4 *>(R.pSContrib.ptr);
-		P.shadowQueue[0] = static_cast<uint32_t *>(R.pSQ0.ptr);
-		P.shadowQueue[1] = static_cast<uint32_t *>(R.pSQ1.ptr);
-		P.occluded = static_cast<uint8_t *>(R.pOcc.ptr);
-		P.shadowHits = static_cast<rdu_hit *>(R.pSHits.ptr);
-		P.mis = static_cast<float2 *>(R.pMis.ptr);
+		ln.P = pathsOf(R);
 		ln.scratch.gstack = static_cast<int *>(R.pGstack.ptr);
 		ln.scratch.walkLeft = R.walkLeft;
 		ln.B.jobs = static_cast<const PtJob *>(L.jobsDev.ptr);

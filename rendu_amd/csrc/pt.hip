@@ -775,7 +775,7 @@ int ptSetSampleBase(uint32_t * counts, uint32_t base, uint32_t seed, hipStream_t
 
 int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, const PtPaths & P, uint32_t samplesInPass,
 	const PtOptions & opt, float * dImage, uint32_t components, PtScratch & scratch, const PtCache * cache,
-	const TraceTuning & tune, hipStream_t stream) {
+	const TraceTuning & tune, hipStream_t stream, hipEvent_t accumulateAfter) {
 	const uint32_t pixels = f.width * f.height;
 	const uint32_t nPaths = pixels * samplesInPass;
 	const PtCache C = cache ? *cache : PtCache{};
@@ -865,6 +865,11 @@ int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, cons
 			return e;
 		hipLaunchKernelGGL(ptCacheApply, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, P, C);
 		hipLaunchKernelGGL(ptCacheTargets, dim3(blocksFor(C.recCapacity)), dim3(kPtBlock), 0, stream, P, C);
+	}
+	if(accumulateAfter) {
+		const hipError_t he = hipStreamWaitEvent(stream, accumulateAfter, 0);
+		if(he != hipSuccess)
+			return int(he);
 	}
 	hipLaunchKernelGGL(ptAccumulate, dim3(blocksFor(pixels)), dim3(kPtBlock), 0, stream, f, P, dImage, components,
 		samplesInPass);

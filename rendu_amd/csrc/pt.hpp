@@ -107,7 +107,8 @@ struct PtCache {
 // into scratch.walkLeft between rounds of steps (a host synchronisation), so it runs eagerly.
 int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, const PtPaths & P, uint32_t samplesInPass,
 	const PtOptions & opt, float * dImage, uint32_t components, PtScratch & scratch, const PtCache * cache,
-	const TraceTuning & tune, hipStream_t stream);
+	const TraceTuning & tune, hipStream_t stream,
+	hipEvent_t accumulateAfter = nullptr);
 int ptSetSampleBase(uint32_t * counts, uint32_t base, uint32_t seed, hipStream_t stream);
 
 // Batch passes: the paths of several jobs (scene, camera, image, samples, depth) in one set of

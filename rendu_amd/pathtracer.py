@@ -70,12 +70,15 @@ class PathTracer:
         return _BorrowedRaycaster(lib.rdu_scene_raycaster(self._h), self)
 
     def render(self, camera, samples: int, depth: int, image: np.ndarray, seed: int = 0,
-               estimator: int = S.ESTIMATOR_REFERENCE, rr_start: int = 3, max_paths: int = 0) -> dict:
-        """PathTracer::render(camera, samples, depth, image). image: (h, w, c >= 3) float32."""
+               estimator: int = S.ESTIMATOR_REFERENCE, rr_start: int = 3, max_paths: int = 0,
+               passes_in_flight: int = 0) -> dict:
+        """PathTracer::render(camera, samples, depth, image). image: (h, w, c >= 3) float32.
+        max_paths / passes_in_flight: paths per pass and passes kept in flight at once (0 = the
+        library's defaults); the image does not depend on them."""
         if image.dtype != np.float32 or not image.flags.c_contiguous or image.ndim != 3:
             raise ValueError("image must be a C-contiguous (height, width, components) float32 array")
         h, w, c = image.shape
-        opts = S.RenderOpts(seed, estimator, rr_start, max_paths)
+        opts = S.RenderOpts(seed, estimator, rr_start, max_paths, passes_in_flight)
         cam = S.camera_desc(camera)
         check(lib.rdu_pathtracer_render(self._h, ctypes.byref(cam), samples, depth, fptr(image), w, h, c,
                                         ctypes.byref(opts)))

@@ -58,7 +58,8 @@ class CameraDesc(ctypes.Structure):
 
 
 class RenderOpts(ctypes.Structure):
-    _fields_ = [("seed", c_uint32), ("estimator", c_int32), ("rr_start", c_int32), ("max_paths", c_uint32)]
+    _fields_ = [("seed", c_uint32), ("estimator", c_int32), ("rr_start", c_int32), ("max_paths", c_uint32),
+                ("passes_in_flight", c_uint32)]
 
 
 class BatchJob(ctypes.Structure):

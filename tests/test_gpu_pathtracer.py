@@ -63,18 +63,24 @@ def test_pass_split_does_not_change_the_image():
     (extended estimator, point light and Russian roulette: every kernel of a pass)."""
     from rendu_amd.pathtracer import PathTracer
     scn, cam_of = scenes.cornell_spheres(seed=4, subdivisions=2, point_light=True)
-    log_case("pass split: 1, 16 and 6 passes of a 64x48 16-spp render")
+    log_case("pass split: 1, 16 and 6 passes of a 64x48 16-spp render, 1-3 passes in flight")
     pt = PathTracer(scn)
     w, h = 64, 48
-    imgs, passes = [], []
-    for mp in (0, w * h, w * h * 3):
+    imgs, passes, rays = [], [], []
+    # (max_paths, passes in flight): one pass; 16 one-sample passes one at a time, 2 and 3 at once;
+    # 6 uneven passes 3 at once (each pass's accumulation waits for the previous one's).
+    for mp, lanes in ((0, 0), (w * h, 1), (w * h, 2), (w * h, 3), (w * h * 3, 3), (w * h * 3, 0)):
         img = np.zeros((h, w, 3), np.float32)
-        st = pt.render(cam_of(w, h), 16, 5, img, seed=9, estimator=S.ESTIMATOR_EXTENDED, rr_start=2, max_paths=mp)
+        st = pt.render(cam_of(w, h), 16, 5, img, seed=9, estimator=S.ESTIMATOR_EXTENDED, rr_start=2, max_paths=mp,
+                       passes_in_flight=lanes)
         imgs.append(img)
         passes.append(st["passes"])
-    assert passes == [1, 16, 6]
+        rays.append((st["extension_rays"], st["shadow_rays"]))
+    assert passes == [1, 16, 16, 16, 6, 6]
     assert imgs[0].mean() > 0.01
-    assert np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[0], imgs[2])
+    for img in imgs[1:]:
+        assert np.array_equal(imgs[0], img)
+    assert len(set(rays)) == 1 and rays[0][1] > 0
 
 
 def feature_scene(bg_mode):
@@ -211,9 +217,10 @@ def test_terrain_spheres_full_geometry():
 def test_single_frame_at_full_resolution():
     """The single frame at its configured size, 1920x1080 over the full scene: 2 spp sample for
     sample against the oracle (about 4M paths on the host cores), then the configured 64 spp on
-    the GPU, where size-independent properties hold: the same bits from 4 passes of 2^25 paths and
-    from 32 passes of 2^22, every pixel finite and non-negative, and a mean within 5 standard errors
-    of the mean of the 2-spp oracle render (the two are independent estimates of one image)."""
+    the GPU, where size-independent properties hold: the same bits from the default (8 passes of
+    2^24 paths, 3 in flight), from 4 passes of 2^25 one at a time and from 32 passes of 2^22, every
+    pixel finite and non-negative, and a mean within 5 standard errors of the mean of the 2-spp
+    oracle render (the two are independent estimates of one image)."""
     scn, cam_of = scenes.terrain_spheres()
     w, h = 1920, 1080
     cam = cam_of(w, h)
@@ -221,15 +228,16 @@ def test_single_frame_at_full_resolution():
     got, exp, st = render_both(scn, cam, w, h, 2, 8, seed=4, estimator=S.ESTIMATOR_EXTENDED, rr_start=3)
     assert_close(got, exp)
     assert st["paths"] == w * h * 2
-    log_case("single frame 1920x1080 64spp depth8 extended: 4 vs 32 passes")
+    log_case("single frame 1920x1080 64spp depth8 extended: 8, 4 and 32 passes")
     pt = PathTracer(scn)
     imgs = []
-    for mp in (0, 1 << 22):
+    for mp, lanes, passes in ((0, 0, 8), (1 << 25, 1, 4), (1 << 22, 0, 32)):
         img = np.zeros((h, w, 3), F32)
-        st = pt.render(cam, 64, 8, img, seed=11, estimator=S.ESTIMATOR_EXTENDED, rr_start=3, max_paths=mp)
+        st = pt.render(cam, 64, 8, img, seed=11, estimator=S.ESTIMATOR_EXTENDED, rr_start=3, max_paths=mp,
+                       passes_in_flight=lanes)
         imgs.append(img)
-        assert st["passes"] == (4 if mp == 0 else 32)
-    assert np.array_equal(imgs[0], imgs[1])
+        assert st["passes"] == passes
+    assert np.array_equal(imgs[0], imgs[1]) and np.array_equal(imgs[0], imgs[2])
     full = imgs[0].astype(np.float64)
     assert np.all(np.isfinite(full)) and full.min() >= 0.0
     # Standard error of the 2-spp image mean, from its per-pixel spread (pixels are independent).
