@@ -590,10 +590,17 @@ def main():
     # (8 threads; lower p50 latency), and 4 threads in list order (lower p99 latency).
     # Warm-up: one whole batch (kernel loads, and the library's memory pools grow to the batch's
     # peak once), so that every schedule below is timed in the steady state of a job service.
+    # The throughput configuration is timed three times; the line reports the median batch (the
+    # batch-to-batch spread of a concurrent schedule is several jobs/s) with every wall listed.
     J.run_batch(job_list, workers=8, order="lpt")
-    res_j, wall_j = J.run_batch(job_list, workers=6, order="lpt", build_workers=2)
-    extra["jobs64"] = dict(J.summary(res_j, wall_j), workers=6, build_workers=2, order="longest first")
-    del res_j
+    runs = []
+    for _ in range(3):
+        res_j, wall_j = J.run_batch(job_list, workers=6, order="lpt", build_workers=2)
+        runs.append((wall_j, J.summary(res_j, wall_j)))
+        del res_j
+    runs.sort(key=lambda r: r[0])
+    extra["jobs64"] = dict(runs[1][1], workers=6, build_workers=2, order="longest first",
+                           walls_s=[round(r[0], 3) for r in runs], batch="median of 3")
     res_j, wall_j = J.run_batch(job_list, workers=8, order="lpt")
     extra["jobs64_build_in_job"] = dict(J.summary(res_j, wall_j), workers=8, order="longest first")
     del res_j

@@ -38,6 +38,7 @@ class Job:
     # default: the batch already runs several jobs at once, and more streams per job cost the batch
     # throughput (64 jobs: 53-55 jobs/s with 3, 56-63 with 1) for a lower p99 latency.
     passes_in_flight: int = 1
+    max_paths: int = 0  # paths per pass (0 = the library's default)
 
 
 @dataclass
@@ -64,7 +65,7 @@ def _render(job: Job, pt: PathTracer, t0: float, t1: float) -> JobResult:
     img = np.zeros((job.height, job.width, 3), np.float32)
     r0 = time.perf_counter()
     st = pt.render(job.camera, job.samples, job.depth, img, seed=job.seed, estimator=job.estimator,
-                   rr_start=job.rr_start, passes_in_flight=job.passes_in_flight)
+                   rr_start=job.rr_start, max_paths=job.max_paths, passes_in_flight=job.passes_in_flight)
     t2 = time.perf_counter()
     return JobResult(-1, img, t2 - t0, t1 - t0, t2 - r0, st["paths"], st["extension_rays"] + st["shadow_rays"],
                      stats=st)
