@@ -51,7 +51,8 @@ def parse():
     ap.add_argument("--cache-ref-spp", type=int, default=4096, help="samples of the reference frame")
     ap.add_argument("--profile-frame", action="store_true",
                     help="only the single frame (1920x1080, 64 spp, depth 8), plain then with a fresh radiance "
-                         "cache: the workload of the path-tracer kernel profiles")
+                         "cache, after one untimed render of each and a 1 s pause: the workload of the "
+                         "path-tracer kernel profiles")
     ap.add_argument("--jobs-only", action="store_true",
                     help="only the 64 batched jobs: a warm-up batch, a 1.5 s pause, then the timed batch with "
                          "2 build and 6 render threads (the workload of the job-batch kernel trace)")
@@ -85,6 +86,13 @@ def profile_frame():
     pt = PathTracer(scn)
     cam = cam_of(1920, 1080)
     img = np.zeros((1080, 1920, 3), np.float32)
+    # Warm-up of both renders (kernel loads, the working sets' first allocation), then a 1 s pause
+    # that separates the timed renders in a trace.
+    pt.render(cam, 64, 8, img, seed=1, estimator=S.ESTIMATOR_EXTENDED, rr_start=3)
+    pt.render_cached(RadianceCache(*S.scene_bounds(scn), seed=4), cam, 64, 8, img, seed=1,
+                     estimator=S.ESTIMATOR_EXTENDED, rr_start=3)
+    time.sleep(1.0)
+    img[:] = 0
     t0 = time.perf_counter()
     st = pt.render(cam, 64, 8, img, seed=2, estimator=S.ESTIMATOR_EXTENDED, rr_start=3)
     t1 = time.perf_counter()
