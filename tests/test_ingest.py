@@ -231,3 +231,31 @@ def test_cutout_cornellbox_loads():
     assert [l.type for l in scn.lights] == [S.LIGHT_SPOT, S.LIGHT_SPOT]
     assert scn.lights[0].angles == pytest.approx((0.5, 0.6)) and scn.lights[1].angles == pytest.approx((0.35, 0.4))
     assert scn.lights[0].radius == 5.0 and scn.lights[0].intensity == pytest.approx((6.0, 5.4, 3.0))
+
+
+def test_parallax_cornellbox_loads():
+    """resources/pbrdemo/cornellbox-parallax.scene: two Parallax materials of four JPEG textures
+    (colour, normal, roughness/metalness/AO, depth; the path tracer reads the first three), a
+    rotated and scaled sphere, one spot light, a colour background."""
+    scn, cam_of = ingest.load_scene(os.path.join(PBR, "cornellbox-parallax.scene"), ROOTS)
+    assert len(scn.objects) == 3 and [l.type for l in scn.lights] == [S.LIGHT_SPOT]
+    par = [m for m in scn.materials if m.type == ingest.MATERIAL_TYPES["Parallax"]]
+    assert len(par) == 2
+    for m in par:
+        col = scn.textures[m.textures[0]]
+        assert col.shape == (512, 512, 4) and 0.0 <= col.min() and col.max() <= 1.0 and col[..., :3].std() > 0.01
+    assert scn.background.mode == S.BG_COLOR and scn.background.color == pytest.approx((0.039,) * 3)
+    sphere = [o for o in scn.objects if np.asarray(o.model)[1, 3] == pytest.approx(0.75)]
+    assert len(sphere) == 1 and sphere[0].use_texcoords
+
+
+def test_textured_cornellbox_loads():
+    """resources/pbrdemo/cornellbox-textured.scene: wood, tiles and brushed metal JPEG maps, painted
+    walls, an emissive light, and the kiara_dawn skybox (first level: 512^2 half-float EXR faces)."""
+    scn, _ = ingest.load_scene(os.path.join(PBR, "cornellbox-textured.scene"), ROOTS)
+    assert len(scn.objects) == 8 and len(scn.materials) == 7
+    assert scn.background.mode == S.BG_SKYBOX and len(scn.background.cube) == 6
+    faces = [scn.textures[f] for f in scn.background.cube]
+    assert all(f.shape[:2] == (512, 512) and f[..., :3].std() > 0.01 for f in faces)
+    jpeg = [scn.textures[m.textures[0]] for m in scn.materials if scn.textures[m.textures[0]].shape[0] > 8]
+    assert len(jpeg) == 3
