@@ -506,7 +506,15 @@ def main():
     extra["cornell_256_16spp_depth5_msamples_per_s"] = round(best["paths"] / best["seconds"] / 1e6, 2)
     extra["cornell_256_16spp_depth5_ms"] = round(best["seconds"] * 1e3, 3)
     extra["cornell_rays_per_sample"] = round((best["extension_rays"] + best["shadow_rays"]) / best["paths"], 3)
-    del pt
+    # Image error of the 16-spp render: per-pixel RMSE against a 4096-spp render of the same scene
+    # (another seed, so the two are independent estimates).
+    ref = np.zeros((256, 256, 3), np.float32)
+    pt.render(cam_of(256, 256), 4096, 5, ref, seed=4096)
+    img[:] = 0
+    pt.render(cam_of(256, 256), 16, 5, img, seed=1)
+    extra["cornell_256_16spp_depth5_rmse_vs_4096spp"] = round(float(np.sqrt(np.mean((img - ref) ** 2))), 5)
+    extra["cornell_256_mean"] = round(float(ref.mean()), 5)
+    del pt, ref
     # End to end (PathTracer construction = raycaster build + upload, then the render).
     t0 = time.perf_counter()
     pt = PathTracer(scn)
