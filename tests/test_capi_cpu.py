@@ -205,3 +205,25 @@ def test_extension_entry_points_reject_bad_arguments():
     assert L.rdu_raycaster_set_kernel_timing(None, 1) == _lib.RDU_ERR_INVALID
     ms, k = ctypes.c_double(), ctypes.c_uint64()
     assert L.rdu_raycaster_kernel_time(None, ctypes.byref(ms), ctypes.byref(k)) == _lib.RDU_ERR_INVALID
+
+
+def test_batch_render_rejects_bad_jobs_without_a_gpu():
+    """rdu_pathtracer_render_batch validates its job list before touching a device: an empty or
+    null list, a job without a committed scene, a null camera or image, fewer than three
+    components, an empty image or zero samples are argument or state errors."""
+    from rendu_amd import scene as S
+    L = _lib.lib
+    assert L.rdu_pathtracer_render_batch(None, 1, 0, None, None) == _lib.RDU_ERR_INVALID
+    jobs = (S.BatchJob * 1)()
+    assert L.rdu_pathtracer_render_batch(ctypes.cast(jobs, ctypes.c_void_p), 0, 0, None, None) == _lib.RDU_ERR_INVALID
+    h = ctypes.c_void_p()
+    assert L.rdu_scene_create(ctypes.byref(h)) == _lib.RDU_OK
+    try:
+        cam = S.CameraDesc()
+        img = np.zeros((4, 4, 3), F32)
+        jobs[0] = S.BatchJob(h.value, ctypes.pointer(cam), None, 4, 2, _lib.fptr(img), 4, 4, 3, S.RenderOpts(1, 0, 3, 0))
+        # The scene is not committed.
+        assert L.rdu_pathtracer_render_batch(ctypes.cast(jobs, ctypes.c_void_p), 1, 0, None, None) == _lib.RDU_ERR_STATE
+        assert b"committed" in L.rdu_last_error()
+    finally:
+        L.rdu_scene_destroy(h)
