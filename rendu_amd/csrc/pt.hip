@@ -558,6 +558,9 @@ __global__ __launch_bounds__(kPtBlock) void ptNormalize(float * image, uint32_t 
 
 inline unsigned blocksFor(size_t n) { return unsigned((n + kPtBlock - 1) / kPtBlock); }
 
+// Refill threshold of the path tracer's traversal launches when no tuning sets one.
+constexpr uint32_t kPassRefillMin = 24;
+
 // ---- batch passes ---------------------------------------------------------------------------------
 
 // Index of the last entry of a sorted array `begin` (of n >= 1 entries, begin[0] = 0) not above v.
@@ -780,14 +783,15 @@ int ptRenderPass(const DevScene & S, const DevBVH & bvh, const PtFrame & f, cons
 	const uint32_t nPaths = pixels * samplesInPass;
 	const PtCache C = cache ? *cache : PtCache{};
 	// Queue-ordered traversal defaults: 64-ray interleaved runs per wavefront (a die's wavefronts
-	// then share one window of neighbouring paths, and so one part of the tree in that die's L2)
-	// and a refill once 8 lanes idle. Measured on the 1080p single frame: 5.6 ms per 2-spp pass
-	// against 10.1 ms with contiguous runs and a refill threshold of 16.
+	// then share one window of neighbouring paths, and so one part of the tree in that die's L2;
+	// on the 1080p single frame 5.6 ms per 2-spp pass against 10.1 ms with contiguous runs) and a
+	// refill once 24 lanes idle (the frame with three passes in flight: 0.139-0.143 s with any
+	// threshold from 8 to 48; the 64-job batch call: 1.03-1.05 s at 24 against 1.12-1.14 s at 8).
 	TraceTuning tn = tune;
 	if(tn.chunk == 0)
 		tn.chunk = 64;
 	if(tn.refillMin == 0)
-		tn.refillMin = 8;
+		tn.refillMin = kPassRefillMin;
 	hipLaunchKernelGGL(ptResetCounts, dim3(1), dim3(64), 0, stream, P.counts, nPaths, samplesInPass);
 	if(cache)
 		hipLaunchKernelGGL(ptCacheReset, dim3(1), dim3(64), 0, stream, C.counts);
@@ -899,11 +903,11 @@ int ptRenderBatchPass(const PtBatch & B, const DevBVH & arena, const PtPaths & P
 	hipEvent_t accumulateAfter) {
 	if(nPaths == 0 || B.nChunks == 0)
 		return 0;
-	TraceTuning tn = tune;
+	TraceTuning tn = tune; // the defaults of ptRenderPass
 	if(tn.chunk == 0)
 		tn.chunk = 64;
 	if(tn.refillMin == 0)
-		tn.refillMin = 8;
+		tn.refillMin = kPassRefillMin;
 	hipLaunchKernelGGL(ptResetCounts, dim3(1), dim3(64), 0, stream, P.counts, nPaths, 0u);
 	hipLaunchKernelGGL(ptGenerateBatch, dim3(blocksFor(nPaths)), dim3(kPtBlock), 0, stream, B, P, nPaths);
 	int q = 0;
