@@ -628,6 +628,31 @@ def main():
     res_j, wall_j = J.run_merged(job_list, build_workers=3, groups=4)
     extra["jobs64_merged_calls"] = dict(J.summary(res_j, wall_j), calls=4, build_workers=3)
     del res_j
+    # Render time alone of all 64 built jobs in one rdu_pathtracer_render_batch call (warm-up call
+    # first), beside the concurrent single renders of the same built jobs on 6 threads.
+    from rendu_amd.pathtracer import PathTracer as _PT
+    built = [(_PT(j.scene), j) for j in job_list]
+    J.render_batch(built)
+    t0 = time.perf_counter()
+    _, _, st_b = J.render_batch(built)
+    one_call = time.perf_counter() - t0
+
+    def _solo(item):
+        p, j = item
+        im = np.zeros((j.height, j.width, 3), np.float32)
+        p.render(j.camera, j.samples, j.depth, im, seed=j.seed, passes_in_flight=1)
+
+    order = sorted(built, key=lambda it: -J.estimated_cost(it[1]))
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=6) as ex:
+        list(ex.map(_solo, order))
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=6) as ex:
+        list(ex.map(_solo, order))
+    solo = time.perf_counter() - t0
+    extra["jobs64_render_only"] = {"one_call_s": round(one_call, 4), "one_call_passes": st_b["passes"],
+                                   "concurrent_single_renders_6_threads_s": round(solo, 4)}
+    del built, order
     for j in job_list:
         j.gpu_build = True
     J.run_batch(job_list, workers=8, order="lpt")  # warm-up of the GPU build's own temporaries
