@@ -66,6 +66,7 @@ def _load() -> ctypes.CDLL:
         "rdu_raycaster_update_hierarchy": (c_int, [c_void_p]),
         "rdu_raycaster_update_hierarchy_gpu": (c_int, [c_void_p, c_void_p]),
         "rdu_scene_set_gpu_build": (c_int, [c_void_p, c_int]),
+        "rdu_scene_set_instancing": (c_int, [c_void_p, c_int]),
         "rdu_scene_set_graph": (c_int, [c_void_p, c_int]),
         "rdu_scene_autotune": (c_int, [c_void_p, c_void_p, c_size_t, c_size_t, c_uint32, c_uint32, c_void_p, c_void_p]),
         "rdu_raycaster_intersect": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p, c_uint32, c_void_p]),

@@ -258,7 +258,139 @@ void parallelFor(size_t n, const F & body) {
 }
 } // namespace
 
-void HostScene::flatten(DeviceLayout & out) const {
+// Collapses the binary tree `ext` below node `top` into 4-wide nodes appended to `wide` (wideCount
+// counting them): each 4-wide node takes its binary node's two children and keeps opening its
+// internal child of largest surface area until it holds 4 children, numbered depth-first in
+// preorder; a leaf child gets leafCodeOf(child). Returns the code of `top` (its leaf code when it
+// is a leaf) and sets `depth` to the number of 4-wide levels.
+template<typename LeafFn>
+int32_t collapseWide(const std::vector<HNode> & ext, uint64_t top, const LeafFn & leafCodeOf, std::vector<float> & wide,
+	uint32_t & wideCount, uint32_t & depth) {
+	depth = 0;
+	if(ext[top].leaf)
+		return leafCodeOf(ext[top]);
+	auto area = [&](uint64_t id) {
+		const f3 s = ext[id].box.mx - ext[id].box.mn;
+		return s.x * s.y + s.y * s.z + s.z * s.x;
+	};
+	struct WItem {
+		uint64_t id; // internal node heading this 4-wide node
+		uint32_t slot; // wide index assigned to it
+		uint32_t depth;
+	};
+	std::vector<WItem> wst;
+	const int32_t topCode = int32_t(wideCount++);
+	wide.resize(size_t(wideCount) * 32, 0.0f);
+	wst.push_back({top, uint32_t(topCode), 1});
+	while(!wst.empty()) {
+		const WItem it = wst.back();
+		wst.pop_back();
+		depth = std::max(depth, it.depth);
+		uint64_t kids[4] = {ext[it.id].left, ext[it.id].right, 0, 0};
+		size_t nk = 2;
+		while(nk < 4) {
+			int best = -1;
+			float bestArea = -1.0f;
+			for(size_t k = 0; k < nk; ++k)
+				if(!ext[kids[k]].leaf && area(kids[k]) > bestArea) {
+					best = int(k);
+					bestArea = area(kids[k]);
+				}
+			if(best < 0)
+				break;
+			const uint64_t e = kids[size_t(best)];
+			for(size_t k = nk; k > size_t(best) + 1; --k) // open a slot after `best`
+				kids[k] = kids[k - 1];
+			kids[size_t(best)] = ext[e].left;
+			kids[size_t(best) + 1] = ext[e].right;
+			++nk;
+		}
+		int32_t codes[4] = {kWideEmpty, kWideEmpty, kWideEmpty, kWideEmpty};
+		float * o = wide.data() + size_t(it.slot) * 32;
+		WItem children[4];
+		size_t nc = 0;
+		for(size_t k = 0; k < nk; ++k) {
+			const HNode & c = ext[kids[k]];
+			o[0 + k] = c.box.mn.x;
+			o[4 + k] = c.box.mx.x;
+			o[8 + k] = c.box.mn.y;
+			o[12 + k] = c.box.mx.y;
+			o[16 + k] = c.box.mn.z;
+			o[20 + k] = c.box.mx.z;
+			if(c.leaf) {
+				codes[k] = leafCodeOf(c);
+			} else {
+				codes[k] = int32_t(wideCount++);
+				children[nc++] = {kids[k], uint32_t(codes[k]), it.depth + 1};
+			}
+		}
+		wide.resize(size_t(wideCount) * 32, 0.0f);
+		o = wide.data() + size_t(it.slot) * 32;
+		for(int k = 0; k < 4; ++k)
+			o[24 + k] = fromBits(uint32_t(codes[k]));
+		for(size_t k = nc; k > 0; --k)
+			wst.push_back(children[k - 1]);
+	}
+	return topCode;
+}
+
+int32_t buildInstanceTop(const std::vector<HBox> & boxes, std::vector<float> & wide, uint32_t & wideCount,
+	uint32_t & depth, HBox & topBox) {
+	const uint64_t n = boxes.size();
+	std::vector<HNode> ext(n);
+	for(uint64_t i = 0; i < n; ++i) {
+		ext[i].box = boxes[i];
+		ext[i].leaf = true;
+		ext[i].left = i;
+		ext[i].right = 1;
+	}
+	uint64_t top = 0;
+	if(n > 1) {
+		struct TWork {
+			uint64_t id, begin, count;
+		};
+		std::vector<uint64_t> ids(n);
+		for(uint64_t i = 0; i < n; ++i)
+			ids[i] = i;
+		ext.reserve(2 * n);
+		ext.emplace_back();
+		top = ext.size() - 1;
+		std::vector<TWork> tw{{top, 0, n}};
+		while(!tw.empty()) {
+			const TWork w = tw.back();
+			tw.pop_back();
+			HBox box = boxes[ids[w.begin]];
+			for(uint64_t k = 1; k < w.count; ++k)
+				mergeInto(box, boxes[ids[w.begin + k]]);
+			ext[w.id].box = box;
+			ext[w.id].leaf = false;
+			const f3 sz = box.mx - box.mn;
+			const int axis = (sz.x >= sz.y && sz.x >= sz.z) ? 0 : (sz.y >= sz.z ? 1 : 2);
+			const uint64_t half = w.count / 2;
+			auto cen = [&](uint64_t id) { return comp(boxes[id].mn, axis) + comp(boxes[id].mx, axis); };
+			std::nth_element(ids.begin() + w.begin, ids.begin() + w.begin + half, ids.begin() + w.begin + w.count,
+				[&](uint64_t x, uint64_t y) { return cen(x) < cen(y); });
+			uint64_t kid[2];
+			const uint64_t beg[2] = {w.begin, w.begin + half}, cnt[2] = {half, w.count - half};
+			for(int k = 0; k < 2; ++k) {
+				if(cnt[k] == 1) {
+					kid[k] = ids[beg[k]];
+				} else {
+					ext.emplace_back();
+					kid[k] = ext.size() - 1;
+					tw.push_back({kid[k], beg[k], cnt[k]});
+				}
+			}
+			ext[w.id].left = kid[0];
+			ext[w.id].right = kid[1];
+		}
+	}
+	topBox = ext[top].box;
+	const auto instLeaf = [](const HNode & l) { return kInstCode0 + int32_t(l.left); };
+	return collapseWide(ext, top, instLeaf, wide, wideCount, depth);
+}
+
+void HostScene::flatten(DeviceLayout & out, bool perMeshRoots) const {
 	// Depth-first preorder numbering of internal nodes (left subtree first) so that a parent and
 	// its near subtree share cache lines as traversal descends.
 	std::vector<int32_t> gpuIndex(nodes.size(), -1);
@@ -347,7 +479,7 @@ void HostScene::flatten(DeviceLayout & out) const {
 		ext[i].right = ext.size() - 1;
 	}
 	uint64_t top = 0;
-	if(meshCount > 1) {
+	if(meshCount > 1 && !perMeshRoots) {
 		struct TWork {
 			uint64_t id, begin, count;
 		};
@@ -389,76 +521,22 @@ void HostScene::flatten(DeviceLayout & out) const {
 	}
 	out.wideTopBox = ext[top].box;
 
-	// 4-wide collapse from the single top node, depth-first preorder numbering.
+	// 4-wide collapse, depth-first preorder numbering: from the single top node, or from each mesh
+	// root on its own.
 	out.wide.clear();
+	out.wide.reserve((ext.size() / 2 + 1) * 32);
 	out.wideCount = 0;
 	out.wideDepth = 0;
-	auto area = [&](uint64_t id) {
-		const f3 s = ext[id].box.mx - ext[id].box.mn;
-		return s.x * s.y + s.y * s.z + s.z * s.x;
-	};
-	struct WItem {
-		uint64_t id; // internal node heading this 4-wide node
-		uint32_t slot; // wide index assigned to it
-		uint32_t depth;
-	};
-	std::vector<WItem> wst;
-	out.wide.reserve((ext.size() / 2 + 1) * 32);
-	if(ext[top].leaf) {
-		out.wideTopCode = leafCode(ext[top].left, ext[top].right);
+	const auto triLeaf = [](const HNode & n) { return leafCode(n.left, n.right); };
+	if(perMeshRoots) {
+		out.meshWideCodes.assign(meshCount, 0);
+		for(uint32_t r = 0; r < meshCount; ++r) {
+			uint32_t d = 0;
+			out.meshWideCodes[r] = collapseWide(ext, r, triLeaf, out.wide, out.wideCount, d);
+			out.wideDepth = std::max(out.wideDepth, d);
+		}
 	} else {
-		out.wideTopCode = int32_t(out.wideCount++);
-		out.wide.resize(size_t(out.wideCount) * 32, 0.0f);
-		wst.push_back({top, uint32_t(out.wideTopCode), 1});
-	}
-	while(!wst.empty()) {
-		const WItem it = wst.back();
-		wst.pop_back();
-		out.wideDepth = std::max(out.wideDepth, it.depth);
-		uint64_t kids[4] = {ext[it.id].left, ext[it.id].right, 0, 0};
-		size_t nk = 2;
-		while(nk < 4) {
-			int best = -1;
-			float bestArea = -1.0f;
-			for(size_t k = 0; k < nk; ++k)
-				if(!ext[kids[k]].leaf && area(kids[k]) > bestArea) {
-					best = int(k);
-					bestArea = area(kids[k]);
-				}
-			if(best < 0)
-				break;
-			const uint64_t e = kids[size_t(best)];
-			for(size_t k = nk; k > size_t(best) + 1; --k) // open a slot after `best`
-				kids[k] = kids[k - 1];
-			kids[size_t(best)] = ext[e].left;
-			kids[size_t(best) + 1] = ext[e].right;
-			++nk;
-		}
-		int32_t codes[4] = {kWideEmpty, kWideEmpty, kWideEmpty, kWideEmpty};
-		float * o = out.wide.data() + size_t(it.slot) * 32;
-		WItem children[4];
-		size_t nc = 0;
-		for(size_t k = 0; k < nk; ++k) {
-			const HNode & c = ext[kids[k]];
-			o[0 + k] = c.box.mn.x;
-			o[4 + k] = c.box.mx.x;
-			o[8 + k] = c.box.mn.y;
-			o[12 + k] = c.box.mx.y;
-			o[16 + k] = c.box.mn.z;
-			o[20 + k] = c.box.mx.z;
-			if(c.leaf) {
-				codes[k] = leafCode(c.left, c.right);
-			} else {
-				codes[k] = int32_t(out.wideCount++);
-				children[nc++] = {kids[k], uint32_t(codes[k]), it.depth + 1};
-			}
-		}
-		out.wide.resize(size_t(out.wideCount) * 32, 0.0f);
-		o = out.wide.data() + size_t(it.slot) * 32;
-		for(int k = 0; k < 4; ++k)
-			o[24 + k] = fromBits(uint32_t(codes[k]));
-		for(size_t k = nc; k > 0; --k)
-			wst.push_back(children[k - 1]);
+		out.wideTopCode = collapseWide(ext, top, triLeaf, out.wide, out.wideCount, out.wideDepth);
 	}
 
 	out.roots.assign(size_t(meshCount) * 8, 0.0f);

@@ -50,6 +50,9 @@ struct HNode {
 
 struct DeviceLayout {
 	std::vector<float> nodes, tris, roots, wide;
+	// flatten(out, true): each mesh root collapsed on its own, with no level above the meshes; the
+	// 4-wide code of mesh m's root (an instanced scene's per-mesh tree).
+	std::vector<int32_t> meshWideCodes;
 	uint32_t internalCount = 0;
 	uint32_t maxDepth = 0;
 	uint32_t wideCount = 0;
@@ -71,10 +74,19 @@ public:
 	void addMesh(const float * xyz, size_t nv, const uint32_t * idx, size_t ni, const float * model);
 	// Raycaster.cpp:54-142.
 	void build();
-	void flatten(DeviceLayout & out) const;
+	// perMeshRoots: no top level over the meshes; every mesh root gets a 4-wide root of its own
+	// (out.meshWideCodes), for trees shared by instances.
+	void flatten(DeviceLayout & out, bool perMeshRoots = false) const;
 	// Sizes the stores for meshes still to come (vertices, triangles, meshes in total), so that
 	// adding many meshes does not regrow them mesh by mesh. Changes no content.
 	void reserve(size_t nVertices, size_t nTriangles, size_t nMeshes);
 };
+
+// Top level of an instanced scene: a 4-wide tree over instance boxes (median split of the box
+// centres on the largest axis, as the top level over mesh roots), appended to `wide` (8 float4 per
+// node, wideCount counting them); instance i's leaf code is kInstCode0 + i. Returns the root's
+// code (an instance code when there is one instance) and sets its box and the tree's depth.
+int32_t buildInstanceTop(const std::vector<HBox> & boxes, std::vector<float> & wide, uint32_t & wideCount,
+	uint32_t & depth, HBox & topBox);
 
 } // namespace rdu

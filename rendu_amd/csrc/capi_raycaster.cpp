@@ -787,3 +787,28 @@ int rdu_raycaster_trace_stats(const rdu_raycaster * rc, const rdu_ray * rays, si
 }
 
 } // extern "C"
+
+int rdu::buildInstanceBase(rdu_raycaster * rc) {
+	if(rc->scene.built || rc->gpuBuilt)
+		return fail(RDU_ERR_STATE, "updateHierarchy called twice");
+	for(size_t m = 0; m < rc->scene.meshCount; ++m)
+		if(rc->scene.nodes[m].right == 0)
+			return fail(RDU_ERR_INVALID, "mesh with no triangle (Rendu reads past its triangle list here)");
+	if(rc->scene.tris.size() >= (size_t(1) << 29))
+		return fail(RDU_ERR_UNSUPPORTED, "instanced scene with 2^29 distinct triangles or more");
+	rc->scene.build();
+	rc->scene.flatten(rc->layout, true);
+	int st = currentDevice(&rc->device);
+	if(st)
+		return st;
+	if((st = rc->tris.upload(rc->layout.tris.data(), rc->layout.tris.size() * 4)) ||
+		(st = rc->wide.upload(rc->layout.wide.data(), rc->layout.wide.size() * 4)))
+		return st;
+	rc->dev = DevBVH{};
+	rc->dev.tris = static_cast<const float4 *>(rc->tris.ptr);
+	rc->dev.wide = static_cast<const float4 *>(rc->wide.ptr);
+	rc->dev.nRoots = rc->scene.meshCount;
+	rc->dev.wideDepth = rc->layout.wideDepth;
+	rc->uploaded = true;
+	return RDU_OK;
+}

@@ -161,6 +161,13 @@ struct rdu_scene {
 	size_t lastTrainPairs = 0; // pairs left in cTrainQ / cTrainT by the last cached pass
 	hipStream_t stream = nullptr; // renders of this scene run here (created on first render)
 	hipStream_t laneStreams[2] = {nullptr, nullptr}; // further passes in flight of a plain render
+	// Instanced scenes (rdu_scene_set_instancing): the raycaster holds each mesh once in object space
+	// with a tree per mesh; a top level over the objects' boxes and the objects' inverse models make
+	// the tree the path tracer traverses.
+	bool instancing = false;
+	rdu::DevBuffer dInstWide, dInst;
+	rdu::DevBVH ibvh;
+	const rdu::DevBVH & traceBvh() const { return instancing ? ibvh : rc->dev; }
 	const rdu::RenderPaths * lastPaths = nullptr;     // path set of the last render (sets live until exit)
 	// Graph replay of full passes (rdu_scene_set_graph): the captured pass, the launch
 	// configuration it was captured with, and that of the last eager full pass.
@@ -191,7 +198,7 @@ struct rdu_scene {
 			if(ls)
 				(void)hipStreamSynchronize(ls);
 		for(rdu::DevBuffer * b : {&dTexels, &dTextures, &dNrm, &dTan, &dBit, &dUv, &dIdx, &dMeshes, &dObjects, &dMaterials,
-				 &dLights, &dEnvPmf, &dEnvRow, &dEnvCol, &dEmTris, &dEmCdf, &dEmBase, &cQuery, &cQAtten, &cQPath, &cPred, &cRec,
+				 &dLights, &dEnvPmf, &dEnvRow, &dEnvCol, &dEmTris, &dEmCdf, &dEmBase, &dInstWide, &dInst, &cQuery, &cQAtten, &cQPath, &cPred, &cRec,
 				 &cRecPath, &cTrainQ, &cTrainT, &cHoldQ, &cHoldT, &cCounts})
 			b->releaseOn(nullptr);
 		if(rc)
@@ -245,7 +252,7 @@ int uploadVec(DevBuffer & b, const std::vector<T> & v) {
 // some small) is regrown in full by the next render that leases it instead of being used as is.
 int ensurePathsFor(RenderPaths & R, const DevBVH & bvh, bool masked, size_t P, bool batch, hipStream_t stream);
 int ensurePaths(RenderPaths & R, const rdu_scene * s, size_t P, hipStream_t stream) {
-	return ensurePathsFor(R, s->rc->dev, s->dev.anyMasked != 0, P, false, stream);
+	return ensurePathsFor(R, s->traceBvh(), s->dev.anyMasked != 0, P, false, stream);
 }
 int ensurePathsFor(RenderPaths & R, const DevBVH & bvh, bool masked, size_t P, bool batch, hipStream_t stream) {
 	int st;
@@ -305,6 +312,31 @@ int ensureCacheBuffers(rdu_scene * s, size_t P, size_t records) {
 // vertices as the raycaster transformed them) weighted by area times the mean luminance of their
 // emission texture; environment vs triangles 1/2 each when both exist, area lights vs analytic
 // lights 1/2 each when both exist.
+// A mesh's vertices under an object's model, in the raycaster's own arithmetic (HostScene::addMesh:
+// glm mat4 * vec4 order, identity models copied as they are).
+std::vector<f3> worldVertices(const float * xyz, size_t nv, const float * model) {
+	std::vector<f3> v(nv);
+	bool identity = true;
+	for(int c = 0; c < 4; ++c)
+		for(int r = 0; r < 4; ++r)
+			identity = identity && (model[4 * c + r] == (c == r ? 1.0f : 0.0f));
+	for(size_t i = 0; i < nv; ++i) {
+		const f3 p = mk(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+		if(identity) {
+			v[i] = p;
+			continue;
+		}
+		float o[3];
+		for(int r = 0; r < 3; ++r) {
+			const float a0 = model[r] * p.x + model[4 + r] * p.y;
+			const float a1 = model[8 + r] * p.z + model[12 + r] * 1.0f;
+			o[r] = a0 + a1;
+		}
+		v[i] = mk(o[0], o[1], o[2]);
+	}
+	return v;
+}
+
 struct AreaTables {
 	std::vector<float> envPmf, envRow, envCol, emCdf;
 	std::vector<float4> emTris;
@@ -368,6 +400,7 @@ AreaTables buildAreaTables(const rdu_scene * s) {
 	A.emBase.assign(s->objects.size(), -1);
 	std::vector<double> w;
 	const std::vector<f3> & verts = s->rc->scene.vertices;
+	std::vector<f3> own; // instanced scenes: an emissive object's world vertices
 	size_t vbase = 0;
 	for(size_t o = 0; o < s->objects.size(); ++o) {
 		const rdu_object_desc & obj = s->objects[o];
@@ -383,9 +416,14 @@ AreaTables buildAreaTables(const rdu_scene * s) {
 				}
 			mean /= double(t.w) * double(t.h);
 			A.emBase[o] = int32_t(w.size());
+			const f3 * vp = verts.data() + vbase;
+			if(s->instancing) {
+				own = worldVertices(s->positions.data() + 3 * size_t(m.vtx), m.nv, obj.model);
+				vp = own.data();
+			}
 			for(uint32_t k = 0; k + 2 < m.ni; k += 3) {
 				const uint32_t * ix = s->indices.data() + m.idx + k;
-				const f3 v0 = verts[vbase + ix[0]], v1 = verts[vbase + ix[1]], v2 = verts[vbase + ix[2]];
+				const f3 v0 = vp[ix[0]], v1 = vp[ix[1]], v2 = vp[ix[2]];
 				const f3 e1 = v1 - v0, e2 = v2 - v0;
 				const f3 n = cross(e1, e2);
 				const double nx = n.x, ny = n.y, nz = n.z;
@@ -448,8 +486,9 @@ std::vector<unsigned char> passKey(const PtFrame & f, const PtOptions & po, cons
 }
 
 TuneKey sceneTuneKey(const rdu_scene * s, size_t passPaths) {
-	const uint32_t cls = kTunePathPass | (s->rc->gpuBuilt ? kTuneGpuTree : 0u);
-	return makeTuneKey(s->rc->scene.tris.size(), s->rc->dev.wideDepth, s->rc->scene.meshCount, passPaths, cls);
+	const uint32_t cls = kTunePathPass | (s->rc->gpuBuilt ? kTuneGpuTree : 0u) | (s->instancing ? kTuneInstanced : 0u);
+	const rdu::DevBVH & b = s->traceBvh();
+	return makeTuneKey(s->rc->scene.tris.size(), b.wideDepth, s->rc->scene.meshCount, passPaths, cls);
 }
 
 // Captures what `enqueue` launches on `stream` into the scene's graph (replacing any previous one).
@@ -489,6 +528,16 @@ int rdu_scene_create(rdu_scene ** out) {
 }
 
 void rdu_scene_destroy(rdu_scene * s) { delete s; }
+
+int rdu_scene_set_instancing(rdu_scene * s, int enable) {
+	clearError();
+	if(!s)
+		return fail(RDU_ERR_INVALID, "null scene");
+	if(s->committed)
+		return fail(RDU_ERR_STATE, "scene already committed");
+	s->instancing = enable != 0;
+	return RDU_OK;
+}
 
 int rdu_scene_set_gpu_build(rdu_scene * s, int enable) {
 	clearError();
@@ -593,6 +642,97 @@ int rdu_scene_set_background(rdu_scene * s, const rdu_background_desc * d) {
 	return RDU_OK;
 }
 
+namespace {
+// Instanced commit: every mesh once, in object space, each with a tree of its own (Rendu's build
+// per mesh); a 4-wide top level over the objects' world boxes (each mesh root box mapped through
+// the object's model, widened by a relative 1e-5 so that float rounding never lets it cull a ray
+// the object-space test keeps); per object the inverse model and its mesh tree's root.
+int commitInstanced(rdu_scene * s) {
+	int st;
+	for(const rdu_scene::MeshRec & m : s->meshes)
+		if((st = rdu_raycaster_add_mesh(s->rc, s->positions.data() + 3 * size_t(m.vtx), m.nv, s->indices.data() + m.idx, m.ni,
+				nullptr)))
+			return st;
+	if((st = buildInstanceBase(s->rc)))
+		return st;
+	const DeviceLayout & L = s->rc->layout;
+	std::vector<HBox> boxes(s->objects.size());
+	std::vector<float4> table(4 * s->objects.size());
+	for(size_t o = 0; o < s->objects.size(); ++o) {
+		const rdu_object_desc & ob = s->objects[o];
+		const float * M = ob.model; // column-major: element (row r, column c) at M[4 c + r]
+		const HBox & mb = s->rc->scene.nodes[ob.mesh].box;
+		double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+		for(int k = 0; k < 8; ++k) {
+			const double p[3] = {(k & 1) ? mb.mx.x : mb.mn.x, (k & 2) ? mb.mx.y : mb.mn.y, (k & 4) ? mb.mx.z : mb.mn.z};
+			for(int r = 0; r < 3; ++r) {
+				const double w = double(M[r]) * p[0] + double(M[4 + r]) * p[1] + double(M[8 + r]) * p[2] + double(M[12 + r]);
+				lo[r] = std::min(lo[r], w);
+				hi[r] = std::max(hi[r], w);
+			}
+		}
+		double ext = 0.0;
+		for(int r = 0; r < 3; ++r)
+			ext = std::max(ext, hi[r] - lo[r]);
+		const double pad = 1e-5 * ext + 1e-7;
+		for(int r = 0; r < 3; ++r) {
+			lo[r] -= pad + 1e-6 * std::fabs(lo[r]);
+			hi[r] += pad + 1e-6 * std::fabs(hi[r]);
+		}
+		boxes[o] = HBox{mk(float(lo[0]), float(lo[1]), float(lo[2])), mk(float(hi[0]), float(hi[1]), float(hi[2]))};
+		// Inverse of the affine model: [A t]^-1 = [A^-1, -A^-1 t], in double.
+		double A[3][3];
+		for(int r = 0; r < 3; ++r)
+			for(int c = 0; c < 3; ++c)
+				A[r][c] = M[4 * c + r];
+		const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+			A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+		if(!(std::fabs(det) > 0.0) || !std::isfinite(det))
+			return fail(RDU_ERR_INVALID, "object with a singular model matrix in an instanced scene");
+		double I[3][3];
+		I[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) / det;
+		I[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) / det;
+		I[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) / det;
+		I[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) / det;
+		I[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) / det;
+		I[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) / det;
+		I[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) / det;
+		I[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) / det;
+		I[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) / det;
+		const double t[3] = {M[12], M[13], M[14]};
+		for(int r = 0; r < 3; ++r) {
+			const double tr = -(I[r][0] * t[0] + I[r][1] * t[1] + I[r][2] * t[2]);
+			table[4 * o + size_t(r)] = make_float4(float(I[r][0]), float(I[r][1]), float(I[r][2]), float(tr));
+		}
+		int32_t root = L.meshWideCodes[ob.mesh];
+		int32_t oid = int32_t(o);
+		float rootBits, oidBits;
+		std::memcpy(&rootBits, &root, 4);
+		std::memcpy(&oidBits, &oid, 4);
+		table[4 * o + 3] = make_float4(rootBits, oidBits, 0.0f, 0.0f);
+	}
+	std::vector<float> wide = L.wide;
+	uint32_t wideCount = L.wideCount, topDepth = 0;
+	HBox top;
+	const int32_t topCode = buildInstanceTop(boxes, wide, wideCount, topDepth, top);
+	if((st = uploadVec(s->dInstWide, wide)) || (st = uploadVec(s->dInst, table)))
+		return st;
+	DevBVH & b = s->ibvh;
+	b = s->rc->dev;
+	b.wide = static_cast<const float4 *>(s->dInstWide.ptr);
+	b.inst = static_cast<const float4 *>(s->dInst.ptr);
+	b.wideTopCode = topCode;
+	const float tl[3] = {top.mn.x, top.mn.y, top.mn.z}, th[3] = {top.mx.x, top.mx.y, top.mx.z};
+	std::memcpy(b.wideTopMin, tl, sizeof(tl));
+	std::memcpy(b.wideTopMax, th, sizeof(th));
+	std::memcpy(b.sceneMin, tl, sizeof(tl));
+	std::memcpy(b.sceneMax, th, sizeof(th));
+	// Stack entries: the top level's levels, a mesh tree's, and the return marker.
+	b.wideDepth = topDepth + L.wideDepth + 1;
+	return RDU_OK;
+}
+} // namespace
+
 int rdu_scene_commit(rdu_scene * s) {
 	clearError();
 	if(!s)
@@ -624,6 +764,14 @@ int rdu_scene_commit(rdu_scene * s) {
 			}
 		}
 	} dropOnFailure{s};
+	if(s->instancing) {
+		if(s->gpuBuild)
+			return fail(RDU_ERR_UNSUPPORTED, "instancing builds its per-mesh trees on the host (GPU build unsupported)");
+		if(s->objects.size() > size_t(kInstMax))
+			return fail(RDU_ERR_UNSUPPORTED, "too many objects for an instanced scene");
+		if((st = commitInstanced(s)))
+			return st;
+	} else {
 	// PathTracer::PathTracer: every object's mesh, with its model, in object order (the raycaster's
 	// stores sized once for all of them).
 	size_t totalVerts = 0, totalTris = 0;
@@ -646,6 +794,7 @@ int rdu_scene_commit(rdu_scene * s) {
 			return st;
 	} else if((st = rdu_raycaster_update_hierarchy(s->rc))) {
 		return st;
+	}
 	}
 	std::vector<DevTexture> tex;
 	for(const auto & t : s->textures)
@@ -708,7 +857,7 @@ int rdu_scene_commit(rdu_scene * s) {
 	return RDU_OK;
 }
 
-const rdu_raycaster * rdu_scene_raycaster(const rdu_scene * s) { return s ? s->rc : nullptr; }
+const rdu_raycaster * rdu_scene_raycaster(const rdu_scene * s) { return (s && !s->instancing) ? s->rc : nullptr; }
 
 
 // tuneOut: instead of rendering, time one pass per candidate of the traversal launch parameters
@@ -886,7 +1035,7 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 					break;
 				if(rep > 0 && (e = hipCheck(hipEventRecord(e0, stream), "hipEventRecord")))
 					break;
-				if((e = hipCheck(hipError_t(ptRenderPass(s->dev, s->rc->dev, f, P, n, po, dImage, components, scratch, nullptr,
+				if((e = hipCheck(hipError_t(ptRenderPass(s->dev, s->traceBvh(), f, P, n, po, dImage, components, scratch, nullptr,
 									 t, stream)),
 						"path tracing pass")))
 					break;
@@ -962,7 +1111,7 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 			Lane & ln = lane[pass % lanes];
 			hipEvent_t e;
 			if((st = hipCheck(hipError_t(ptSetSampleBase(ln.P.counts, uint32_t(base), opt.seed, ln.s)), "sample base")) ||
-				(st = hipCheck(hipError_t(ptRenderPass(s->dev, s->rc->dev, f, ln.P, uint32_t(n), po, dImage, components,
+				(st = hipCheck(hipError_t(ptRenderPass(s->dev, s->traceBvh(), f, ln.P, uint32_t(n), po, dImage, components,
 								   ln.scratch, nullptr, tune, ln.s, accumulated)),
 						"path tracing pass")) ||
 				(st = hipCheck(hipError_t(events.make(e, hipEventDisableTiming)), "hipEventCreate")) ||
@@ -990,7 +1139,7 @@ static int renderImpl(rdu_scene * s, const rdu_camera_desc * cam, size_t samples
 			C.trainY = (k >> 2) & 3u;
 		}
 		auto enqueue = [&]() -> int {
-			return ptRenderPass(s->dev, s->rc->dev, f, P, uint32_t(n), po, dImage, components, scratch, cache ? &C : nullptr,
+			return ptRenderPass(s->dev, s->traceBvh(), f, P, uint32_t(n), po, dImage, components, scratch, cache ? &C : nullptr,
 				tune, stream);
 		};
 		// Scenes with cut-outs run eagerly: their shadow walk reads a count on the host.
@@ -1112,6 +1261,8 @@ int rdu_pathtracer_render_batch(const rdu_batch_job * jobs, size_t n, uint32_t m
 			return fail(RDU_ERR_UNSUPPORTED, "batch job image larger than 2^26 pixels");
 		if(device >= 0 && b.scene->rc->device != device)
 			return fail(RDU_ERR_INVALID, "batch jobs on different devices");
+		if(b.scene->instancing)
+			return fail(RDU_ERR_UNSUPPORTED, "instanced scenes render alone, not in batches");
 		device = b.scene->rc->device;
 	}
 	int st = setDevice(device);

@@ -18,6 +18,13 @@ namespace rdu {
 
 // Child code of an unused slot of a 4-wide node.
 constexpr int32_t kWideEmpty = 0x7fffffff;
+// Instanced scenes: below kInstLimit, codes name instances (kInstCode0 + i, i < kInstMax) or the
+// marker that returns a ray from an instance's tree to the top level; triangle leaf codes
+// (~(first << 1 | count - 1), first < 2^29) stay at kInstLimit or above.
+constexpr int32_t kInstCode0 = int32_t(0x80000000u);
+constexpr int32_t kInstMax = 0x3ffffffe;
+constexpr int32_t kInstReturn = int32_t(0x80000000u) + 0x3fffffff;
+constexpr int32_t kInstLimit = -(1 << 30);
 
 struct f3 {
 	float x, y, z;

@@ -372,7 +372,9 @@ __global__ __launch_bounds__(kSimpleBlock) void traceKernel(DevBVH bvh, const fl
 // unsorted visiting order; NFA selects the near/far plane fetch for (sorted) any hit.
 // BATCH: rays of several scenes whose trees share one node arena; ray k starts at the root of scene
 // rayJob[k], whose top box and code are jobRoots[2 rayJob[k]] (min, code bits) and [+1] (max).
-template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA = false, bool BATCH = false>
+// INST: an instanced scene (bvh.inst): the top level's leaves are instances, whose shared trees
+// are traversed with the ray moved into the instance's object space.
+template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA = false, bool BATCH = false, bool INST = false>
 __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
 	const uint32_t * __restrict__ perm, const uint32_t * __restrict__ countDev, rdu_hit * __restrict__ hits,
 	uint8_t * __restrict__ occ, int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint32_t chunk,
@@ -400,7 +402,7 @@ uint32_t refillMin(const TraceOptions & opt) {
 	return opt.spatialOrder ? 32u : 16u;
 }
 
-template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA, bool BATCH>
+template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA, bool BATCH, bool INST = false>
 PersistentGrid persistentGrid(size_t n, uint32_t blocksPerCuCap);
 
 template<bool ANY>
@@ -426,12 +428,14 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 		RDU_LAUNCH(R, W, false, ST)
 	const char * kern = std::getenv("RDU_TRACE_KERNEL");
 	const bool persistent = wide && !(kern && kern[0] == 's'); // RDU_TRACE_KERNEL=simple: one ray per lane
+	if(bvh.inst && !persistent)
+		return int(hipErrorInvalidValue); // instanced trees: persistent 4-wide traversal only
 	if(persistent) {
 		if(n > 0xffffffffull)
 			return int(hipErrorInvalidValue);
-#define RDU_PERSIST_B(S, ST, ALT, NFA, B)                                                                         \
+#define RDU_PERSIST_X(S, ST, ALT, NFA, B, I)                                                                      \
 	{                                                                                                            \
-		const PersistentGrid g = persistentGrid<ANY, S, ST, ALT, NFA, B>(n, opt.blocksPerCu);                         \
+		const PersistentGrid g = persistentGrid<ANY, S, ST, ALT, NFA, B, I>(n, opt.blocksPerCu);                      \
 		const uint32_t waves = (g.blocks / kXcds) * (kBlock / 64);                                               \
 		const uint64_t slice = (uint64_t(n) + kXcds - 1) / kXcds;                                                \
 		/* sorted batches: interleaved 64-ray chunks; input order: one contiguous run per wavefront; */        \
@@ -440,17 +444,31 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 		chunk = chunk < 64 ? 64 : chunk;                                                                        \
 		if(opt.chunk >= 16 && opt.chunk <= (1u << 20))                                                          \
 			chunk = opt.chunk;                                                                                  \
-		hipLaunchKernelGGL((tracePersistent<ANY, S, ST, ALT, NFA, B>), dim3(g.blocks), dim3(kBlock), 0, stream, bvh,     \
+		hipLaunchKernelGGL((tracePersistent<ANY, S, ST, ALT, NFA, B, I>), dim3(g.blocks), dim3(kBlock), 0, stream, bvh,  \
 			rays, uint32_t(n), perm, opt.countDev, hits, occ, gstack, lds, kXcds, refillMin(opt), uint32_t(chunk),      \
 			opt.rayJob, opt.jobRoots, stats);                                                                   \
 	}
+#define RDU_PERSIST_B(S, ST, ALT, NFA, B) RDU_PERSIST_X(S, ST, ALT, NFA, B, false)
+#define RDU_PERSIST_I(S, ALT, NFA) RDU_PERSIST_X(S, false, ALT, NFA, false, true)
 #define RDU_PERSIST(S, ST, ALT, NFA) RDU_PERSIST_B(S, ST, ALT, NFA, false)
 		// Kernel variant: the near/far plane fetch (fewer instructions per node, more registers:
 		// 5 waves per SIMD instead of 6 for closest hit and of 7 for any hit; pays on incoherent rays), or
 		// for any hit on coherent rays the unsorted visiting order.
 		const bool alt = ANY ? opt.unorderedAny : opt.nearFar;
 		const bool nfa = ANY && !opt.unorderedAny && opt.nearFar;
-		if(opt.jobRoots) {
+		if(bvh.inst) {
+			// Instanced scenes (the path tracer's passes): closest hit with either fetch, any hit
+			// with the near/far fetch or in node order; no statistics, no batches.
+			if(stats || opt.jobRoots)
+				return int(hipErrorInvalidValue);
+			if(ANY && nfa) {
+				if(spill) RDU_PERSIST_I(true, false, true) else RDU_PERSIST_I(false, false, true)
+			} else if(alt) {
+				if(spill) RDU_PERSIST_I(true, true, false) else RDU_PERSIST_I(false, true, false)
+			} else {
+				if(spill) RDU_PERSIST_I(true, false, false) else RDU_PERSIST_I(false, false, false)
+			}
+		} else if(opt.jobRoots) {
 			// Batch passes of the path tracer: closest hit with either fetch, any hit with the
 			// near/far fetch; no statistics.
 			if(stats || (ANY && !nfa))
@@ -482,7 +500,9 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 			}
 		}
 #undef RDU_PERSIST
+#undef RDU_PERSIST_I
 #undef RDU_PERSIST_B
+#undef RDU_PERSIST_X
 		return int(hipGetLastError());
 	}
 	if(stats) {
@@ -530,7 +550,7 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 // so never above the true entry): a popped child whose entry already exceeds the current maxi
 // is dropped without being fetched, exactly as its own box test would drop it (the test fails
 // once entry > maxi). Levels spilled to global memory carry no entry and are always fetched.
-template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA, bool BATCH>
+template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA, bool BATCH, bool INST>
 __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
 	const uint32_t * __restrict__ perm, const uint32_t * __restrict__ countDev, rdu_hit * __restrict__ hits,
 	uint8_t * __restrict__ occ, int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint32_t chunk,
@@ -559,6 +579,7 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 	int c = 0, kk = 0;
 	float maxi = 0.0f, bt = FLT_MAX, bu = 0.0f, bv = 0.0f;
 	int best = -1;
+	int inst = -1, bestInst = -1; // instanced scenes: the object whose tree the lane is in, of the best hit
 	auto writeResult = [&]() {
 		if(ANY) {
 			occ[ri] = best >= 0 ? 1 : 0;
@@ -570,7 +591,7 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 				h.u = bu;
 				h.v = bv;
 				h.hit = 1u;
-				h.mesh_id = __float_as_uint(t2.y);
+				h.mesh_id = INST ? uint32_t(bestInst) : __float_as_uint(t2.y);
 				h.local_id = __float_as_uint(t2.z);
 			} else {
 				h.t = FLT_MAX;
@@ -611,6 +632,7 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 					}
 					maxi = r.maxi;
 					best = -1;
+					inst = bestInst = -1;
 					bt = FLT_MAX;
 					bu = bv = 0.0f;
 					st.sp = 0;
@@ -669,6 +691,16 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 					a5 = nd[5];
 				}
 				a6 = nd[6];
+			} else if(INST && c < kInstLimit) {
+				// An instance record (its inverse model rows, then {tree root code, object}); the
+				// return marker fetches nothing.
+				if(c != kInstReturn) {
+					const float4 * ir = bvh.inst + 4 * size_t(c - kInstCode0);
+					a0 = ir[0];
+					a1 = ir[1];
+					a2 = ir[2];
+					a3 = ir[3];
+				}
 			} else {
 				const int ti = ((~c) >> 1) + kk;
 				a0 = bvh.tris[3 * ti + 0];
@@ -681,7 +713,39 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 		// Process.
 		const bool exact = !__all(clean);
 		bool next = false; // pop, or go to the next root, or finish
-		if(c >= 0) {
+		if(INST && c < kInstLimit) {
+			if(c == kInstReturn) {
+				// Back from an instance's tree: the world ray again (the query's own, rebuilt as at
+				// refill), then on with the top level's stack.
+				const float4 ra = rays[2 * size_t(ri)];
+				const float4 rb = rays[2 * size_t(ri) + 1];
+				r = makeRay(xyz(ra), xyz(rb), ra.w, rb.w);
+				inst = -1;
+				next = true;
+			} else {
+				// Into an instance: the ray moves to object space, p' = M^-1 p and d' = M^-1 d (not
+				// renormalised, so t keeps its world value), and the top level resumes at the
+				// marker pushed under the instance tree's nodes (its entry never culls it).
+				const f3 p = r.pos, d = r.dir;
+				r.pos = mk(a0.x * p.x + a0.y * p.y + a0.z * p.z + a0.w, a1.x * p.x + a1.y * p.y + a1.z * p.z + a1.w,
+					a2.x * p.x + a2.y * p.y + a2.z * p.z + a2.w);
+				r.dir = mk(a0.x * d.x + a0.y * d.y + a0.z * d.z, a1.x * d.x + a1.y * d.y + a1.z * d.z,
+					a2.x * d.x + a2.y * d.y + a2.z * d.z);
+				r.inv = rcp3(r.dir);
+				if(!ANY && (!SPILL || st.sp < st.lds))
+					le[st.sp][threadIdx.x] = _Float16(-__builtin_inff());
+				st.push<SPILL>(kInstReturn);
+				inst = __float_as_int(a3.y);
+				c = __float_as_int(a3.x);
+				kk = 0;
+			}
+			clean = finite3(r.inv) && finite3(r.pos);
+			if constexpr(NF) {
+				ox = r.inv.x < 0.0f ? 16u : 0u;
+				oy = r.inv.y < 0.0f ? 48u : 32u;
+				oz = r.inv.z < 0.0f ? 80u : 64u;
+			}
+		} else if(c >= 0) {
 			if(STATS)
 				++nodeVisits;
 			int c0 = __float_as_int(a6.x), c1 = __float_as_int(a6.y), c2 = __float_as_int(a6.z), c3 = __float_as_int(a6.w);
@@ -775,6 +839,8 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 				bu = uu;
 				bv = vv;
 				maxi = tt;
+				if(INST)
+					bestInst = inst;
 			}
 			if(ANY && best >= 0) {
 				st.sp = 0;
@@ -813,7 +879,7 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 	}
 }
 
-template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA, bool BATCH>
+template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA, bool BATCH, bool INST>
 PersistentGrid persistentGrid(size_t n, uint32_t blocksPerCuCap) {
 	struct Occupancy {
 		int blocksPerCu = 1, cus = 1;
@@ -824,7 +890,7 @@ PersistentGrid persistentGrid(size_t n, uint32_t blocksPerCuCap) {
 		int dev = 0;
 		(void)hipGetDevice(&dev);
 		(void)hipDeviceGetAttribute(&o.cus, hipDeviceAttributeMultiprocessorCount, dev);
-		(void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocksPerCu, tracePersistent<ANY, SPILL, STATS, ALT, NFA, BATCH>,
+		(void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o.blocksPerCu, tracePersistent<ANY, SPILL, STATS, ALT, NFA, BATCH, INST>,
 			kBlock, 0);
 		o.blocksPerCu = o.blocksPerCu < 1 ? 1 : o.blocksPerCu;
 		o.cus = o.cus < 1 ? 1 : o.cus;

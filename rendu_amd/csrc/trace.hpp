@@ -13,6 +13,10 @@ struct DevBVH {
 	const float4 * tris = nullptr;
 	const float4 * roots = nullptr;
 	const float4 * wide = nullptr;
+	// Instanced scenes: 4 float4 per instance, rows 0-2 of the inverse model matrix (object from
+	// world) and {root code of its mesh's tree, object index, 0, 0 as int bits}; the top level's
+	// leaves carry kInstCode0 + instance. Null for flattened scenes.
+	const float4 * inst = nullptr;
 	uint32_t nRoots = 0;
 	uint32_t maxDepth = 0;
 	uint32_t wideDepth = 0;

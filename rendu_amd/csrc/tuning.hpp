@@ -12,7 +12,7 @@
 namespace rdu {
 
 // Classes of batch; bits of TuneKey::cls.
-enum : uint32_t { kTuneIncoherent = 1, kTuneAny = 2, kTuneGpuTree = 4, kTunePathPass = 8 };
+enum : uint32_t { kTuneIncoherent = 1, kTuneAny = 2, kTuneGpuTree = 4, kTunePathPass = 8, kTuneInstanced = 16 };
 
 struct TuneKey {
 	uint32_t triLog2 = 0;   // floor(log2(triangles))

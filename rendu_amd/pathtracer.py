@@ -36,13 +36,17 @@ class _BorrowedRaycaster:
 
 
 class PathTracer:
-    def __init__(self, scene: S.Scene, gpu_build: bool = False):
-        """gpu_build: build the tree on the GPU (rdu_scene_set_gpu_build) instead of Rendu's host build."""
+    def __init__(self, scene: S.Scene, gpu_build: bool = False, instancing: bool = False):
+        """gpu_build: build the tree on the GPU (rdu_scene_set_gpu_build) instead of Rendu's host build.
+        instancing: one tree per mesh shared by the objects that use it, under a top level over the
+        objects (rdu_scene_set_instancing; an extension: hits differ from Rendu's in the last bits)."""
         h = ctypes.c_void_p()
         check(lib.rdu_scene_create(ctypes.byref(h)))
         self._h = h
         if gpu_build:
             check(lib.rdu_scene_set_gpu_build(h, 1))
+        if instancing:
+            check(lib.rdu_scene_set_instancing(h, 1))
         keep, tex, meshes, mats, objs, lights, bg = scene.descriptors()
         i = ctypes.c_int32()
         for t in tex:
@@ -66,8 +70,11 @@ class PathTracer:
             self._h = None
 
     def raycaster(self):
-        """PathTracer::raycaster() (PathTracer.hpp:29)."""
-        return _BorrowedRaycaster(lib.rdu_scene_raycaster(self._h), self)
+        """PathTracer::raycaster() (PathTracer.hpp:29). Instanced scenes have none."""
+        p = lib.rdu_scene_raycaster(self._h)
+        if not p:
+            raise RuntimeError("an instanced scene has no raycaster of its own")
+        return _BorrowedRaycaster(p, self)
 
     def render(self, camera, samples: int, depth: int, image: np.ndarray, seed: int = 0,
                estimator: int = S.ESTIMATOR_REFERENCE, rr_start: int = 3, max_paths: int = 0,

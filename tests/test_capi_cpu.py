@@ -181,12 +181,14 @@ def test_extension_entry_points_reject_bad_arguments():
     # Output step: components out of range, empty image is fine.
     assert L.rdu_image_to_ldr(None, 4, 4, 5, None, 0, None) == _lib.RDU_ERR_INVALID
     assert L.rdu_image_to_ldr(None, 0, 4, 3, None, 0, None) == _lib.RDU_OK
-    # Scene: GPU-build switch and cached render on a null / uncommitted scene.
+    # Scene: GPU-build and instancing switches and cached render on a null / uncommitted scene.
     assert L.rdu_scene_set_gpu_build(None, 1) == _lib.RDU_ERR_INVALID
+    assert L.rdu_scene_set_instancing(None, 1) == _lib.RDU_ERR_INVALID
     h = c_void()
     assert L.rdu_scene_create(ctypes.byref(h)) == _lib.RDU_OK
     try:
         assert L.rdu_scene_set_gpu_build(h, 1) == _lib.RDU_OK
+        assert L.rdu_scene_set_instancing(h, 1) == _lib.RDU_OK and L.rdu_scene_set_instancing(h, 0) == _lib.RDU_OK
         assert L.rdu_pathtracer_render_cached(h, None, None, 1, 1, None, 1, 1, 3, None, None) == _lib.RDU_ERR_INVALID
         stats = (ctypes.c_uint64 * 5)()
         assert L.rdu_scene_last_cache_stats(h, ctypes.byref(stats)) == _lib.RDU_OK

@@ -1,0 +1,135 @@
+"""Instanced scenes (rdu_scene_set_instancing, an extension): one tree per mesh in object space,
+shared by its objects, under a top level over the objects' boxes; rays enter an object's tree
+transformed by its inverse model. Hits differ from the flattened scene's (Rendu's world-space
+triangles) only in the transform's rounding, so renders are compared with the flattened scene's
+renders of the same seeds: the same paths almost everywhere (a path changes only where an ulp of
+its hit point flips a later decision), and equal means."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import log_case
+from rendu_amd import jobs as J
+from rendu_amd import scene as S
+from rendu_amd import scenes
+from rendu_amd.pathtracer import PathTracer
+from rendu_amd.workloads import icosphere, quad
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+
+
+def shared_mesh_scene(seed=5, n=40, emissive=False, light=True):
+    """A ground quad and n objects of one icosphere mesh under random rotations, non-uniform scales
+    and translations; a point light (any-hit shadow rays); optionally emissive instances and a
+    lat-long sky (the extended estimator's area-light sampling)."""
+    rng = np.random.default_rng(seed)
+    scn = S.Scene()
+    t_flat = scn.add_texture(S.constant_texture([0.5, 0.5, 1.0, 1.0]))
+    t_rm = scn.add_texture(S.constant_texture([0.5, 0.0, 1.0, 1.0]))
+    g = scn.add_material(S.Material(S.MATERIAL_REGULAR, (scn.add_texture(S.constant_texture([0.6, 0.6, 0.55, 1])), t_flat, t_rm)))
+    ground = scn.add_mesh(quad((-6, 0, 6), (6, 0, 6), (6, 0, -6), (-6, 0, -6)))
+    scn.objects.append(S.Object(ground, g))
+    sph = scn.add_mesh(icosphere(2, radius=1.0))
+    for k in range(n):
+        a, b = rng.uniform(0, 2 * math.pi, 2)
+        ca, sa, cb, sb = math.cos(a), math.sin(a), math.cos(b), math.sin(b)
+        rot = np.array([[ca, -sa, 0], [sa, ca, 0], [0, 0, 1]]) @ np.array([[1, 0, 0], [0, cb, -sb], [0, sb, cb]])
+        scale = np.diag(rng.uniform(0.2, 0.6, 3))
+        model = np.eye(4, dtype=F32)
+        model[:3, :3] = (rot @ scale).astype(F32)
+        model[:3, 3] = (rng.uniform(-4, 4), rng.uniform(0.3, 1.5), rng.uniform(-4, 2))
+        col = rng.uniform(0.2, 0.9, 3)
+        if emissive and k % 8 == 0:
+            mat = scn.add_material(S.Material(S.MATERIAL_EMISSIVE, (scn.add_texture(S.constant_texture([4.0, 3.0, 2.0, 1])), t_flat, t_rm)))
+        else:
+            mat = scn.add_material(S.Material(S.MATERIAL_REGULAR, (scn.add_texture(S.constant_texture([*col, 1])), t_flat,
+                                                                   scn.add_texture(S.constant_texture([rng.uniform(0.1, 0.9), float(k % 2), 1, 1])))))
+        scn.objects.append(S.Object(sph, mat, model))
+    if light:
+        scn.lights.append(S.Light(S.LIGHT_POINT, (30.0, 28.0, 25.0), (2.0, 6.0, 3.0), radius=20.0))
+    if emissive:
+        env = np.zeros((32, 64, 4), F32)
+        env[..., 0] = np.linspace(0.2, 1.2, 64)[None, :]
+        env[..., 1] = np.linspace(0.9, 0.1, 32)[:, None]
+        env[..., 2] = 0.6
+        env[..., 3] = 1
+        scn.background = S.Background(S.BG_LATLONG, texture=scn.add_texture(env))
+    else:
+        scn.background = S.Background(S.BG_COLOR, (0.2, 0.25, 0.3))
+
+    def cam_of(w, h):
+        from rendu_amd.camera import Camera
+        c = Camera()
+        c.pose([0.0, 3.0, 9.0], [0.0, 0.5, 0.0], [0.0, 1.0, 0.0])
+        c.projection(ratio=w / h, fov=0.9)
+        return c
+
+    return scn, cam_of
+
+
+def both(scn, cam, w, h, spp, depth, seed=3, **kw):
+    imgs = []
+    for inst in (False, True):
+        img = np.zeros((h, w, 3), F32)
+        st = PathTracer(scn, instancing=inst).render(cam, spp, depth, img, seed=seed, **kw)
+        imgs.append((img, st))
+    return imgs
+
+
+@pytest.mark.parametrize("depth", [2, 5])
+def test_instanced_render_matches_flattened(depth):
+    scn, cam_of = shared_mesh_scene()
+    w, h = 96, 64
+    log_case(f"instanced vs flattened, {len(scn.objects)} objects of 2 meshes, 96x64 16spp depth {depth}")
+    (a, sa), (b, sb) = both(scn, cam_of(w, h), w, h, 16, depth)
+    assert a.mean() > 0.01 and sa["shadow_rays"] > 0 and sb["shadow_rays"] > 0
+    close = np.abs(a - b) <= 1e-4 * np.abs(a) + 1e-5
+    # Depth 2: camera rays, the light's shadow rays and one bounce; deeper paths may branch where
+    # an ulp of a hit point flips a later choice.
+    assert close.mean() >= (0.99 if depth == 2 else 0.97), close.mean()
+    assert abs(float(a.mean()) - float(b.mean())) <= 2e-3 * float(a.mean())
+    assert abs(sa["extension_rays"] - sb["extension_rays"]) <= 2e-3 * sa["extension_rays"]
+
+
+def test_instanced_extended_estimator_with_emissive_instances():
+    """Emissive instances (their world triangles sampled by the extended estimator), a lat-long
+    sky, one and three passes in flight."""
+    scn, cam_of = shared_mesh_scene(seed=9, emissive=True, light=False)
+    w, h = 80, 60
+    log_case("instanced vs flattened, emissive instances, extended estimator, 80x60 32spp depth 4")
+    (a, _), (b, _) = both(scn, cam_of(w, h), w, h, 32, 4, estimator=S.ESTIMATOR_EXTENDED, rr_start=2,
+                           max_paths=w * h * 8)
+    close = np.abs(a - b) <= 1e-4 * np.abs(a) + 1e-5
+    assert close.mean() >= 0.95, close.mean()
+    assert abs(float(a.mean()) - float(b.mean())) <= 5e-3 * float(a.mean())
+    c = np.zeros_like(b)
+    PathTracer(scn, instancing=True).render(cam_of(w, h), 32, 4, c, seed=3, estimator=S.ESTIMATOR_EXTENDED, rr_start=2,
+                                            max_paths=w * h * 8, passes_in_flight=1)
+    np.testing.assert_array_equal(b, c)
+
+
+def test_instanced_frame_scene_matches_flattened():
+    """The single frame's scene (terrain + 2000 instances of one sphere mesh) at a small size."""
+    scn, cam_of = scenes.terrain_spheres()
+    w, h = 96, 54
+    log_case("instanced vs flattened, terrain + 2000 sphere instances, 96x54 4spp depth 8 extended")
+    (a, _), (b, _) = both(scn, cam_of(w, h), w, h, 4, 8, seed=4, estimator=S.ESTIMATOR_EXTENDED, rr_start=3)
+    close = np.abs(a - b) <= 1e-4 * np.abs(a) + 1e-5
+    assert close.mean() >= 0.95, close.mean()
+    assert abs(float(a.mean()) - float(b.mean())) <= 1e-2 * float(a.mean())
+
+
+def test_instanced_scene_limits():
+    scn, cam_of = shared_mesh_scene(n=4)
+    pt = PathTracer(scn, instancing=True)
+    with pytest.raises(RuntimeError):
+        pt.raycaster()
+    job = J.Job(scn, cam_of(16, 16), 16, 16, 4, 2)
+    from rendu_amd._lib import RduError, RDU_ERR_UNSUPPORTED
+    with pytest.raises(RduError) as e:
+        J.render_batch([(pt, job)])
+    assert e.value.code == RDU_ERR_UNSUPPORTED
+    with pytest.raises(RduError):
+        PathTracer(scn, gpu_build=True, instancing=True)
