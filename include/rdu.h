@@ -267,13 +267,16 @@ int rdu_scene_commit(rdu_scene* scene);
  * (rdu_raycaster_update_hierarchy_gpu) instead of Rendu's host build. */
 int rdu_scene_set_gpu_build(rdu_scene* scene, int enable);
 /* Before commit (no reference counterpart, an extension; Rendu bakes every object's model into
- * world-space triangles, Raycaster.cpp:19-26): instancing. Each mesh gets one tree of its own in
- * object space (Rendu's build of that mesh alone), shared by every object that uses it; a 4-wide top
- * level over the objects' world boxes leads rays into them, transformed by the inverse model. Hits
- * then differ from Rendu's in the last bits (the transform's rounding), so renders match the
- * flattened scene's statistically, not bit for bit. Such a scene has no raycaster handle
- * (rdu_scene_raycaster returns NULL), renders alone (not in rdu_pathtracer_render_batch) and
- * builds on the host; its models must be invertible. */
+ * world-space triangles, Raycaster.cpp:19-26): instancing. A mesh shared by several objects gets one
+ * tree of its own in object space (Rendu's build of that mesh alone), shared by those objects; a
+ * mesh used by one object is baked into world space as Rendu does it. A 4-wide top level over the
+ * objects' world boxes leads rays into the trees, transformed by the inverse model where the tree is
+ * in object space (not for an identity model). Hits on shared meshes then differ from Rendu's in
+ * the last bits (the transform's rounding), so renders match the flattened scene's statistically,
+ * not bit for bit; a scene without shared meshes (or whose shared meshes' objects all have identity
+ * models) renders as the flattened scene does. Such a scene has no raycaster handle
+ * (rdu_scene_raycaster returns NULL), renders alone (not in rdu_pathtracer_render_batch) and builds
+ * on the host; the models of objects on shared meshes must be invertible. */
 int rdu_scene_set_instancing(rdu_scene* scene, int enable);
 
 /* The raycaster owned by a committed scene (PathTracer::raycaster(), PathTracer.hpp:29); NULL for

@@ -643,32 +643,90 @@ int rdu_scene_set_background(rdu_scene * s, const rdu_background_desc * d) {
 }
 
 namespace {
-// Instanced commit: every mesh once, in object space, each with a tree of its own (Rendu's build
-// per mesh); a 4-wide top level over the objects' world boxes (each mesh root box mapped through
-// the object's model, widened by a relative 1e-5 so that float rounding never lets it cull a ray
-// the object-space test keeps); per object the inverse model and its mesh tree's root.
+bool isIdentity(const float * M) {
+	for(int k = 0; k < 16; ++k)
+		if(M[k] != ((k % 5 == 0) ? 1.0f : 0.0f))
+			return false;
+	return true;
+}
+
+// Instanced commit. A mesh used by one object is added in world space (the object's model applied
+// as Rendu applies it), so its triangles and tree are the flattened scene's; a mesh shared by
+// several objects is added once in object space. Each gets a tree of its own (Rendu's build per
+// mesh); a 4-wide top level goes over the objects' world boxes. Per object the table holds the
+// inverse model and its tree's root, with a flag for objects whose tree is already in world space
+// (single-use meshes and identity models): their rays enter untransformed. The world box of a
+// transformed object bounds its mesh's transformed vertices (the transformed corners of the mesh
+// box once a vertex budget is spent), widened by a relative 1e-5 so that float rounding never lets
+// it cull a ray the object-space test keeps.
 int commitInstanced(rdu_scene * s) {
 	int st;
-	for(const rdu_scene::MeshRec & m : s->meshes)
-		if((st = rdu_raycaster_add_mesh(s->rc, s->positions.data() + 3 * size_t(m.vtx), m.nv, s->indices.data() + m.idx, m.ni,
-				nullptr)))
+	std::vector<uint32_t> users(s->meshes.size(), 0), single(s->meshes.size(), 0);
+	for(size_t o = 0; o < s->objects.size(); ++o) {
+		++users[s->objects[o].mesh];
+		single[s->objects[o].mesh] = uint32_t(o);
+	}
+	std::vector<uint32_t> rcMesh(s->meshes.size(), UINT32_MAX);
+	uint32_t added = 0;
+	for(size_t m = 0; m < s->meshes.size(); ++m) {
+		if(!users[m])
+			continue;
+		const rdu_scene::MeshRec & r = s->meshes[m];
+		const float * model = users[m] == 1 ? s->objects[single[m]].model : nullptr;
+		if((st = rdu_raycaster_add_mesh(s->rc, s->positions.data() + 3 * size_t(r.vtx), r.nv, s->indices.data() + r.idx, r.ni,
+				model)))
 			return st;
+		rcMesh[m] = added++;
+	}
 	if((st = buildInstanceBase(s->rc)))
 		return st;
 	const DeviceLayout & L = s->rc->layout;
 	std::vector<HBox> boxes(s->objects.size());
 	std::vector<float4> table(4 * s->objects.size());
+	size_t vertexBudget = size_t(1) << 26; // transformed vertices for tight boxes, over all objects
 	for(size_t o = 0; o < s->objects.size(); ++o) {
 		const rdu_object_desc & ob = s->objects[o];
 		const float * M = ob.model; // column-major: element (row r, column c) at M[4 c + r]
-		const HBox & mb = s->rc->scene.nodes[ob.mesh].box;
+		const uint32_t rm = rcMesh[ob.mesh];
+		const HBox & mb = s->rc->scene.nodes[rm].box;
+		int32_t root = L.meshWideCodes[rm];
+		int32_t oid = int32_t(o);
+		const int32_t worldFlag = 1;
+		float rootBits, oidBits, worldBits;
+		std::memcpy(&rootBits, &root, 4);
+		std::memcpy(&oidBits, &oid, 4);
+		std::memcpy(&worldBits, &worldFlag, 4);
+		if(users[ob.mesh] == 1 || isIdentity(M)) {
+			// The tree is in world space: the box is its root's, the ray enters as it is.
+			boxes[o] = mb;
+			table[4 * o + 0] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+			table[4 * o + 1] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
+			table[4 * o + 2] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+			table[4 * o + 3] = make_float4(rootBits, oidBits, worldBits, 0.0f);
+			continue;
+		}
 		double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-		for(int k = 0; k < 8; ++k) {
-			const double p[3] = {(k & 1) ? mb.mx.x : mb.mn.x, (k & 2) ? mb.mx.y : mb.mn.y, (k & 4) ? mb.mx.z : mb.mn.z};
+		const auto extend = [&](const double p[3]) {
 			for(int r = 0; r < 3; ++r) {
 				const double w = double(M[r]) * p[0] + double(M[4 + r]) * p[1] + double(M[8 + r]) * p[2] + double(M[12 + r]);
 				lo[r] = std::min(lo[r], w);
 				hi[r] = std::max(hi[r], w);
+			}
+		};
+		const rdu_scene::MeshRec & mr = s->meshes[ob.mesh];
+		if(mr.ni && mr.nv <= vertexBudget) {
+			vertexBudget -= mr.nv;
+			// Every vertex an index names (the triangles' hull); unreferenced vertices add nothing.
+			const float * P = s->positions.data() + 3 * size_t(mr.vtx);
+			const uint32_t * ix = s->indices.data() + mr.idx;
+			for(size_t i = 0; i < mr.ni; ++i) {
+				const double p[3] = {P[3 * size_t(ix[i])], P[3 * size_t(ix[i]) + 1], P[3 * size_t(ix[i]) + 2]};
+				extend(p);
+			}
+		} else {
+			for(int k = 0; k < 8; ++k) {
+				const double p[3] = {(k & 1) ? mb.mx.x : mb.mn.x, (k & 2) ? mb.mx.y : mb.mn.y, (k & 4) ? mb.mx.z : mb.mn.z};
+				extend(p);
 			}
 		}
 		double ext = 0.0;
@@ -704,11 +762,6 @@ int commitInstanced(rdu_scene * s) {
 			const double tr = -(I[r][0] * t[0] + I[r][1] * t[1] + I[r][2] * t[2]);
 			table[4 * o + size_t(r)] = make_float4(float(I[r][0]), float(I[r][1]), float(I[r][2]), float(tr));
 		}
-		int32_t root = L.meshWideCodes[ob.mesh];
-		int32_t oid = int32_t(o);
-		float rootBits, oidBits;
-		std::memcpy(&rootBits, &root, 4);
-		std::memcpy(&oidBits, &oid, 4);
 		table[4 * o + 3] = make_float4(rootBits, oidBits, 0.0f, 0.0f);
 	}
 	std::vector<float> wide = L.wide;

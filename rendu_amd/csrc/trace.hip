@@ -372,10 +372,18 @@ __global__ __launch_bounds__(kSimpleBlock) void traceKernel(DevBVH bvh, const fl
 // unsorted visiting order; NFA selects the near/far plane fetch for (sorted) any hit.
 // BATCH: rays of several scenes whose trees share one node arena; ray k starts at the root of scene
 // rayJob[k], whose top box and code are jobRoots[2 rayJob[k]] (min, code bits) and [+1] (max).
-// INST: an instanced scene (bvh.inst): the top level's leaves are instances, whose shared trees
-// are traversed with the ray moved into the instance's object space.
+// INST: an instanced scene (bvh.inst): the top level's leaves are objects; a shared tree is
+// traversed with the ray moved into the object's space, a world-space tree with the ray as it is.
+// Waves per SIMD the instanced variants are held to, which a few more registers would otherwise
+// cost a wave: any hit 7 (64 registers; 5 with the near/far fetch, 88), closest hit 5 (94
+// registers with the near/far fetch; without it, 30 KB of LDS per workgroup with the world rays
+// allows no more).
+template<bool ANY, bool NFA, bool INST>
+constexpr int persistentMinWaves() {
+	return !INST ? 1 : ANY ? (NFA ? 5 : 7) : 5;
+}
 template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA = false, bool BATCH = false, bool INST = false>
-__global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
+__global__ __launch_bounds__(kBlock, (persistentMinWaves<ANY, NFA, INST>())) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
 	const uint32_t * __restrict__ perm, const uint32_t * __restrict__ countDev, rdu_hit * __restrict__ hits,
 	uint8_t * __restrict__ occ, int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint32_t chunk,
 	const uint32_t * __restrict__ rayJob, const float4 * __restrict__ jobRoots, uint64_t * stats);
@@ -551,7 +559,7 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 // is dropped without being fetched, exactly as its own box test would drop it (the test fails
 // once entry > maxi). Levels spilled to global memory carry no entry and are always fetched.
 template<bool ANY, bool SPILL, bool STATS, bool ALT, bool NFA, bool BATCH, bool INST>
-__global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
+__global__ __launch_bounds__(kBlock, (persistentMinWaves<ANY, NFA, INST>())) void tracePersistent(DevBVH bvh, const float4 * __restrict__ rays, uint32_t n,
 	const uint32_t * __restrict__ perm, const uint32_t * __restrict__ countDev, rdu_hit * __restrict__ hits,
 	uint8_t * __restrict__ occ, int * __restrict__ gstack, int lds, uint32_t xcds, uint32_t refillMin, uint32_t chunk,
 	const uint32_t * __restrict__ rayJob, const float4 * __restrict__ jobRoots, uint64_t * stats) {
@@ -559,6 +567,8 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 	constexpr bool UNSORTED = ALT && ANY; // children in node order
 	__shared__ int ls[kWideLds][kBlock];
 	__shared__ _Float16 le[ANY ? 1 : kWideLds][kBlock];
+	// Instanced scenes: the lane's world ray (position, direction) while it is in an object's space.
+	__shared__ float wr[INST ? 6 : 1][INST ? kBlock : 1];
 	if(countDev)
 		n = *countDev < n ? *countDev : n;
 	const uint32_t gtid = blockIdx.x * kBlock + threadIdx.x;
@@ -692,15 +702,12 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 				}
 				a6 = nd[6];
 			} else if(INST && c < kInstLimit) {
-				// An instance record (its inverse model rows, then {tree root code, object}); the
-				// return marker fetches nothing.
-				if(c != kInstReturn) {
-					const float4 * ir = bvh.inst + 4 * size_t(c - kInstCode0);
-					a0 = ir[0];
-					a1 = ir[1];
-					a2 = ir[2];
-					a3 = ir[3];
-				}
+				// An instance record (its inverse model rows, then {tree root code, object, world flag}).
+				const float4 * ir = bvh.inst + 4 * size_t(c - kInstCode0);
+				a0 = ir[0];
+				a1 = ir[1];
+				a2 = ir[2];
+				a3 = ir[3];
 			} else {
 				const int ti = ((~c) >> 1) + kk;
 				a0 = bvh.tris[3 * ti + 0];
@@ -714,19 +721,24 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 		const bool exact = !__all(clean);
 		bool next = false; // pop, or go to the next root, or finish
 		if(INST && c < kInstLimit) {
-			if(c == kInstReturn) {
-				// Back from an instance's tree: the world ray again (the query's own, rebuilt as at
-				// refill), then on with the top level's stack.
-				const float4 ra = rays[2 * size_t(ri)];
-				const float4 rb = rays[2 * size_t(ri) + 1];
-				r = makeRay(xyz(ra), xyz(rb), ra.w, rb.w);
-				inst = -1;
-				next = true;
+			if(__float_as_int(a3.z)) {
+				// Into an object whose tree is in world space: the ray goes on as it is and the
+				// top level's stack entries follow the tree's nodes, so no marker is needed.
+				inst = __float_as_int(a3.y);
+				c = __float_as_int(a3.x);
+				kk = 0;
 			} else {
 				// Into an instance: the ray moves to object space, p' = M^-1 p and d' = M^-1 d (not
 				// renormalised, so t keeps its world value), and the top level resumes at the
-				// marker pushed under the instance tree's nodes (its entry never culls it).
+				// marker pushed under the instance tree's nodes (its entry never culls it), whose
+				// pop brings the world ray back from LDS.
 				const f3 p = r.pos, d = r.dir;
+				wr[0][threadIdx.x] = p.x;
+				wr[1][threadIdx.x] = p.y;
+				wr[2][threadIdx.x] = p.z;
+				wr[3][threadIdx.x] = d.x;
+				wr[4][threadIdx.x] = d.y;
+				wr[5][threadIdx.x] = d.z;
 				r.pos = mk(a0.x * p.x + a0.y * p.y + a0.z * p.z + a0.w, a1.x * p.x + a1.y * p.y + a1.z * p.z + a1.w,
 					a2.x * p.x + a2.y * p.y + a2.z * p.z + a2.w);
 				r.dir = mk(a0.x * d.x + a0.y * d.y + a0.z * d.z, a1.x * d.x + a1.y * d.y + a1.z * d.z,
@@ -857,10 +869,33 @@ __global__ __launch_bounds__(kBlock) void tracePersistent(DevBVH bvh, const floa
 				--st.sp;
 		}
 		if(next) {
-			if(st.sp > 0) {
+			bool more = st.sp > 0;
+			if(more) {
 				c = st.pop<SPILL>();
 				kk = 0;
-			} else {
+			}
+			if(INST && more && c == kInstReturn) {
+				// Out of an object's space: the world ray again (its direction already normalised,
+				// so the reciprocals are the ones the query started with), then on with the top
+				// level's entries.
+				r.pos = mk(wr[0][threadIdx.x], wr[1][threadIdx.x], wr[2][threadIdx.x]);
+				r.dir = mk(wr[3][threadIdx.x], wr[4][threadIdx.x], wr[5][threadIdx.x]);
+				r.inv = rcp3(r.dir);
+				inst = -1;
+				clean = finite3(r.inv) && finite3(r.pos);
+				if constexpr(NF) {
+					ox = r.inv.x < 0.0f ? 16u : 0u;
+					oy = r.inv.y < 0.0f ? 48u : 32u;
+					oz = r.inv.z < 0.0f ? 80u : 64u;
+				}
+				if(!ANY)
+					while(st.sp > 0 && st.sp <= st.lds && float(le[st.sp - 1][threadIdx.x]) > maxi)
+						--st.sp;
+				more = st.sp > 0;
+				if(more)
+					c = st.pop<SPILL>();
+			}
+			if(!more) {
 				// Done: the result is written at the ray's own index at the next refill.
 				has = false;
 				pend = true;

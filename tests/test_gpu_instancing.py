@@ -121,6 +121,22 @@ def test_instanced_frame_scene_matches_flattened():
     assert abs(float(a.mean()) - float(b.mean())) <= 1e-2 * float(a.mean())
 
 
+def test_instanced_scene_without_shared_meshes_renders_as_flattened():
+    """Meshes used by one object each are baked into world space as Rendu bakes them, and their
+    objects enter the top level untransformed: the render equals the flattened scene's bit for bit."""
+    scn, cam_of = shared_mesh_scene(seed=7, n=12)
+    # Every object its own copy of the sphere mesh: nothing is shared.
+    sph = scn.objects[1].mesh
+    for o in scn.objects[1:]:
+        o.mesh = scn.add_mesh(scn.meshes[sph])
+    w, h = 80, 60
+    log_case("instanced vs flattened, 13 objects without shared meshes, 80x60 16spp depth 4")
+    (a, sa), (b, sb) = both(scn, cam_of(w, h), w, h, 16, 4)
+    assert a.mean() > 0.01 and sa["shadow_rays"] > 0
+    np.testing.assert_array_equal(a, b)
+    assert sa["extension_rays"] == sb["extension_rays"] and sa["shadow_rays"] == sb["shadow_rays"]
+
+
 def test_instanced_scene_limits():
     scn, cam_of = shared_mesh_scene(n=4)
     pt = PathTracer(scn, instancing=True)
