@@ -334,8 +334,8 @@ int32_t collapseWide(const std::vector<HNode> & ext, uint64_t top, const LeafFn 
 	return topCode;
 }
 
-int32_t buildInstanceTop(const std::vector<HBox> & boxes, std::vector<float> & wide, uint32_t & wideCount,
-	uint32_t & depth, HBox & topBox) {
+int32_t buildInstanceTop(const std::vector<HBox> & boxes, const std::vector<int32_t> & leafCodes, std::vector<float> & wide,
+	uint32_t & wideCount, uint32_t & depth, HBox & topBox) {
 	const uint64_t n = boxes.size();
 	std::vector<HNode> ext(n);
 	for(uint64_t i = 0; i < n; ++i) {
@@ -386,8 +386,8 @@ int32_t buildInstanceTop(const std::vector<HBox> & boxes, std::vector<float> & w
 		}
 	}
 	topBox = ext[top].box;
-	const auto instLeaf = [](const HNode & l) { return kInstCode0 + int32_t(l.left); };
-	return collapseWide(ext, top, instLeaf, wide, wideCount, depth);
+	const auto objectLeaf = [&](const HNode & l) { return leafCodes[l.left]; };
+	return collapseWide(ext, top, objectLeaf, wide, wideCount, depth);
 }
 
 void HostScene::flatten(DeviceLayout & out, bool perMeshRoots) const {

@@ -82,11 +82,12 @@ public:
 	void reserve(size_t nVertices, size_t nTriangles, size_t nMeshes);
 };
 
-// Top level of an instanced scene: a 4-wide tree over instance boxes (median split of the box
+// Top level of an instanced scene: a 4-wide tree over object boxes (median split of the box
 // centres on the largest axis, as the top level over mesh roots), appended to `wide` (8 float4 per
-// node, wideCount counting them); instance i's leaf code is kInstCode0 + i. Returns the root's
-// code (an instance code when there is one instance) and sets its box and the tree's depth.
-int32_t buildInstanceTop(const std::vector<HBox> & boxes, std::vector<float> & wide, uint32_t & wideCount,
-	uint32_t & depth, HBox & topBox);
+// node, wideCount counting them); object i's leaf code is leafCodes[i] (an instance code
+// kInstCode0 + i, or the 4-wide code of a tree already in world space). Returns the root's code
+// (object 0's leaf code when there is one object) and sets its box and the tree's depth.
+int32_t buildInstanceTop(const std::vector<HBox> & boxes, const std::vector<int32_t> & leafCodes, std::vector<float> & wide,
+	uint32_t & wideCount, uint32_t & depth, HBox & topBox);
 
 } // namespace rdu

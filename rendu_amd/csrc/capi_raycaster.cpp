@@ -788,7 +788,7 @@ int rdu_raycaster_trace_stats(const rdu_raycaster * rc, const rdu_ray * rays, si
 
 } // extern "C"
 
-int rdu::buildInstanceBase(rdu_raycaster * rc) {
+int rdu::buildInstanceBase(rdu_raycaster * rc, const std::vector<uint32_t> & objectOf) {
 	if(rc->scene.built || rc->gpuBuilt)
 		return fail(RDU_ERR_STATE, "updateHierarchy called twice");
 	for(size_t m = 0; m < rc->scene.meshCount; ++m)
@@ -798,6 +798,14 @@ int rdu::buildInstanceBase(rdu_raycaster * rc) {
 		return fail(RDU_ERR_UNSUPPORTED, "instanced scene with 2^29 distinct triangles or more");
 	rc->scene.build();
 	rc->scene.flatten(rc->layout, true);
+	// Triangle record {e2.z, mesh id, local id, 0}: the mesh id is float 9 of the 12.
+	std::vector<float> & T = rc->layout.tris;
+	for(size_t t = 0; t < T.size() / 12; ++t) {
+		uint32_t m;
+		std::memcpy(&m, &T[12 * t + 9], 4);
+		if(m < objectOf.size() && objectOf[m] != UINT32_MAX)
+			std::memcpy(&T[12 * t + 9], &objectOf[m], 4);
+	}
 	int st = currentDevice(&rc->device);
 	if(st)
 		return st;

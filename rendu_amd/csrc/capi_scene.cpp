@@ -651,11 +651,11 @@ bool isIdentity(const float * M) {
 }
 
 // Instanced commit. A mesh used by one object is added in world space (the object's model applied
-// as Rendu applies it), so its triangles and tree are the flattened scene's; a mesh shared by
-// several objects is added once in object space. Each gets a tree of its own (Rendu's build per
-// mesh); a 4-wide top level goes over the objects' world boxes. Per object the table holds the
-// inverse model and its tree's root, with a flag for objects whose tree is already in world space
-// (single-use meshes and identity models): their rays enter untransformed. The world box of a
+// as Rendu applies it), so its triangles and tree are the flattened scene's, and its root is a leaf
+// of the top level; a mesh shared by several objects is added once in object space. Each gets a
+// tree of its own (Rendu's build per mesh); a 4-wide top level goes over the objects' world boxes.
+// Per object the table holds the inverse model and its tree's root, with a flag for objects whose
+// tree is already in world space (identity models): their rays enter untransformed. The world box of a
 // transformed object bounds its mesh's transformed vertices (the transformed corners of the mesh
 // box once a vertex budget is spent), widened by a relative 1e-5 so that float rounding never lets
 // it cull a ray the object-space test keeps.
@@ -666,7 +666,7 @@ int commitInstanced(rdu_scene * s) {
 		++users[s->objects[o].mesh];
 		single[s->objects[o].mesh] = uint32_t(o);
 	}
-	std::vector<uint32_t> rcMesh(s->meshes.size(), UINT32_MAX);
+	std::vector<uint32_t> rcMesh(s->meshes.size(), UINT32_MAX), objectOf;
 	uint32_t added = 0;
 	for(size_t m = 0; m < s->meshes.size(); ++m) {
 		if(!users[m])
@@ -677,11 +677,13 @@ int commitInstanced(rdu_scene * s) {
 				model)))
 			return st;
 		rcMesh[m] = added++;
+		objectOf.push_back(users[m] == 1 ? single[m] : UINT32_MAX);
 	}
-	if((st = buildInstanceBase(s->rc)))
+	if((st = buildInstanceBase(s->rc, objectOf)))
 		return st;
 	const DeviceLayout & L = s->rc->layout;
 	std::vector<HBox> boxes(s->objects.size());
+	std::vector<int32_t> leafCodes(s->objects.size());
 	std::vector<float4> table(4 * s->objects.size());
 	size_t vertexBudget = size_t(1) << 26; // transformed vertices for tight boxes, over all objects
 	for(size_t o = 0; o < s->objects.size(); ++o) {
@@ -696,6 +698,9 @@ int commitInstanced(rdu_scene * s) {
 		std::memcpy(&rootBits, &root, 4);
 		std::memcpy(&oidBits, &oid, 4);
 		std::memcpy(&worldBits, &worldFlag, 4);
+		// A single-use mesh's tree hangs in the top level as it is (its triangles carry the
+		// object's id); other objects are entered through their record.
+		leafCodes[o] = users[ob.mesh] == 1 ? root : kInstCode0 + oid;
 		if(users[ob.mesh] == 1 || isIdentity(M)) {
 			// The tree is in world space: the box is its root's, the ray enters as it is.
 			boxes[o] = mb;
@@ -767,7 +772,7 @@ int commitInstanced(rdu_scene * s) {
 	std::vector<float> wide = L.wide;
 	uint32_t wideCount = L.wideCount, topDepth = 0;
 	HBox top;
-	const int32_t topCode = buildInstanceTop(boxes, wide, wideCount, topDepth, top);
+	const int32_t topCode = buildInstanceTop(boxes, leafCodes, wide, wideCount, topDepth, top);
 	if((st = uploadVec(s->dInstWide, wide)) || (st = uploadVec(s->dInst, table)))
 		return st;
 	DevBVH & b = s->ibvh;

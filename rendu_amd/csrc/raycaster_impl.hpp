@@ -13,11 +13,12 @@
 struct rdu_raycaster;
 
 namespace rdu {
-// capi_raycaster.cpp: Rendu's host build of every added mesh (object space: added without a model)
-// with each mesh's root collapsed into a 4-wide tree of its own and no level above (the trees an
-// instanced scene shares between its objects); uploads triangles and nodes. The handle then
-// answers no queries of its own.
-int buildInstanceBase(rdu_raycaster * rc);
+// capi_raycaster.cpp: Rendu's host build of every added mesh with each mesh's root collapsed into
+// a 4-wide tree of its own and no level above (the trees an instanced scene's objects use); uploads
+// triangles and nodes. The triangles of mesh m carry objectOf[m] as their mesh id where that is
+// not UINT32_MAX (a mesh in world space, used by that one object). The handle then answers no
+// queries of its own.
+int buildInstanceBase(rdu_raycaster * rc, const std::vector<uint32_t> & objectOf);
 // host_query.cpp: one ray in Rendu's visiting order over the host copy of the flattened tree.
 void hostIntersect(const rdu_raycaster * rc, const rdu_ray & ray, rdu_hit * out);
 bool hostIntersectAny(const rdu_raycaster * rc, const rdu_ray & ray);

@@ -589,7 +589,9 @@ __global__ __launch_bounds__(kBlock, (persistentMinWaves<ANY, NFA, INST>())) voi
 	int c = 0, kk = 0;
 	float maxi = 0.0f, bt = FLT_MAX, bu = 0.0f, bv = 0.0f;
 	int best = -1;
-	int inst = -1, bestInst = -1; // instanced scenes: the object whose tree the lane is in, of the best hit
+	// Instanced scenes: the object whose shared tree the lane is in (-1 in world space, where the
+	// triangles carry their object's id), and that of the best hit.
+	int inst = -1, bestInst = -1;
 	auto writeResult = [&]() {
 		if(ANY) {
 			occ[ri] = best >= 0 ? 1 : 0;
@@ -601,7 +603,7 @@ __global__ __launch_bounds__(kBlock, (persistentMinWaves<ANY, NFA, INST>())) voi
 				h.u = bu;
 				h.v = bv;
 				h.hit = 1u;
-				h.mesh_id = INST ? uint32_t(bestInst) : __float_as_uint(t2.y);
+				h.mesh_id = INST && bestInst >= 0 ? uint32_t(bestInst) : __float_as_uint(t2.y);
 				h.local_id = __float_as_uint(t2.z);
 			} else {
 				h.t = FLT_MAX;
@@ -721,36 +723,31 @@ __global__ __launch_bounds__(kBlock, (persistentMinWaves<ANY, NFA, INST>())) voi
 		const bool exact = !__all(clean);
 		bool next = false; // pop, or go to the next root, or finish
 		if(INST && c < kInstLimit) {
-			if(__float_as_int(a3.z)) {
-				// Into an object whose tree is in world space: the ray goes on as it is and the
-				// top level's stack entries follow the tree's nodes, so no marker is needed.
-				inst = __float_as_int(a3.y);
-				c = __float_as_int(a3.x);
-				kk = 0;
-			} else {
-				// Into an instance: the ray moves to object space, p' = M^-1 p and d' = M^-1 d (not
-				// renormalised, so t keeps its world value), and the top level resumes at the
-				// marker pushed under the instance tree's nodes (its entry never culls it), whose
-				// pop brings the world ray back from LDS.
-				const f3 p = r.pos, d = r.dir;
-				wr[0][threadIdx.x] = p.x;
-				wr[1][threadIdx.x] = p.y;
-				wr[2][threadIdx.x] = p.z;
-				wr[3][threadIdx.x] = d.x;
-				wr[4][threadIdx.x] = d.y;
-				wr[5][threadIdx.x] = d.z;
+			// Into an object's tree, with a marker pushed under its nodes (its entry never culls it)
+			// whose pop brings the world ray back from LDS and leaves the object: then the top
+			// level's entries resume. For an instance the ray moves to object space, p' = M^-1 p
+			// and d' = M^-1 d (not renormalised, so t keeps its world value); an object with an
+			// identity model takes the ray as it is.
+			const f3 p = r.pos, d = r.dir;
+			wr[0][threadIdx.x] = p.x;
+			wr[1][threadIdx.x] = p.y;
+			wr[2][threadIdx.x] = p.z;
+			wr[3][threadIdx.x] = d.x;
+			wr[4][threadIdx.x] = d.y;
+			wr[5][threadIdx.x] = d.z;
+			if(!__float_as_int(a3.z)) {
 				r.pos = mk(a0.x * p.x + a0.y * p.y + a0.z * p.z + a0.w, a1.x * p.x + a1.y * p.y + a1.z * p.z + a1.w,
 					a2.x * p.x + a2.y * p.y + a2.z * p.z + a2.w);
 				r.dir = mk(a0.x * d.x + a0.y * d.y + a0.z * d.z, a1.x * d.x + a1.y * d.y + a1.z * d.z,
 					a2.x * d.x + a2.y * d.y + a2.z * d.z);
 				r.inv = rcp3(r.dir);
-				if(!ANY && (!SPILL || st.sp < st.lds))
-					le[st.sp][threadIdx.x] = _Float16(-__builtin_inff());
-				st.push<SPILL>(kInstReturn);
-				inst = __float_as_int(a3.y);
-				c = __float_as_int(a3.x);
-				kk = 0;
 			}
+			if(!ANY && (!SPILL || st.sp < st.lds))
+				le[st.sp][threadIdx.x] = _Float16(-__builtin_inff());
+			st.push<SPILL>(kInstReturn);
+			inst = __float_as_int(a3.y);
+			c = __float_as_int(a3.x);
+			kk = 0;
 			clean = finite3(r.inv) && finite3(r.pos);
 			if constexpr(NF) {
 				ox = r.inv.x < 0.0f ? 16u : 0u;

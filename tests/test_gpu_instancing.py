@@ -137,6 +137,28 @@ def test_instanced_scene_without_shared_meshes_renders_as_flattened():
     assert sa["extension_rays"] == sb["extension_rays"] and sa["shadow_rays"] == sb["shadow_rays"]
 
 
+def test_instanced_identity_object_of_a_shared_mesh_renders_as_flattened():
+    """A shared mesh whose visible object has an identity model (entered untransformed through its
+    record, then left at the marker) next to world-space meshes (the ground, hung in the top level
+    as it is): hits keep the right object on every path, bit for bit as in the flattened scene. The
+    mesh's other object is a tiny transformed copy far above, which no ray reaches."""
+    scn, cam_of = shared_mesh_scene(seed=8, n=6)
+    sph = scn.objects[1].mesh
+    mat = scn.objects[1].material
+    far = np.diag([1e-3, 1e-3, 1e-3, 1.0]).astype(F32)
+    far[:3, 3] = (0.0, 1000.0, 0.0)
+    scn.objects[1] = S.Object(sph, mat, np.eye(4, dtype=F32))
+    for o in scn.objects[2:]:
+        o.mesh = scn.add_mesh(scn.meshes[sph])
+    scn.objects.append(S.Object(sph, mat, far))
+    w, h = 80, 60
+    log_case("instanced vs flattened, identity-model object of a shared mesh + world-space meshes, 80x60 16spp depth 4")
+    (a, sa), (b, sb) = both(scn, cam_of(w, h), w, h, 16, 4)
+    assert a.mean() > 0.01
+    np.testing.assert_array_equal(a, b)
+    assert sa["extension_rays"] == sb["extension_rays"] and sa["shadow_rays"] == sb["shadow_rays"]
+
+
 def test_instanced_scene_limits():
     scn, cam_of = shared_mesh_scene(n=4)
     pt = PathTracer(scn, instancing=True)
