@@ -53,6 +53,10 @@ def parse():
                     help="only the single frame (1920x1080, 64 spp, depth 8), plain then with a fresh radiance "
                          "cache, after one untimed render of each and a 1 s pause: the workload of the "
                          "path-tracer kernel profiles")
+    ap.add_argument("--profile-frame-instanced", action="store_true",
+                    help="only the single frame, flattened then as an instanced scene (one untimed render of "
+                         "each, a 1 s pause, then one timed render of each): the workload of the instanced "
+                         "frame's kernel profile")
     ap.add_argument("--jobs-only", action="store_true",
                     help="only the 64 batched jobs: a warm-up batch, a 1.5 s pause, then the timed batch with "
                          "2 build and 6 render threads (the workload of the job-batch kernel trace)")
@@ -102,6 +106,31 @@ def profile_frame():
     t2 = time.perf_counter()
     print(json.dumps({"plain_s": round(t1 - t0, 4), "cached_s": round(t2 - t1, 4), "paths": st["paths"],
                       "cache_queries": cst["cache_queries"], "train_steps": cst["train_steps"]}), flush=True)
+
+
+def profile_frame_instanced():
+    from rendu_amd import scene as S
+    from rendu_amd import scenes
+    from rendu_amd.pathtracer import PathTracer
+    scn, cam_of = scenes.terrain_spheres()
+    cam = cam_of(1920, 1080)
+    img = np.zeros((1080, 1920, 3), np.float32)
+    t0 = time.perf_counter()
+    flat = PathTracer(scn)
+    t1 = time.perf_counter()
+    inst = PathTracer(scn, instancing=True)
+    t2 = time.perf_counter()
+    for pt in (flat, inst):
+        pt.render(cam, 64, 8, img, seed=1, estimator=S.ESTIMATOR_EXTENDED, rr_start=3)
+    time.sleep(1.0)
+    out = {"flattened_setup_s": round(t1 - t0, 3), "instanced_setup_s": round(t2 - t1, 3)}
+    for name, pt in (("flattened", flat), ("instanced", inst)):
+        img[:] = 0
+        w0 = time.perf_counter()
+        pt.render(cam, 64, 8, img, seed=2, estimator=S.ESTIMATOR_EXTENDED, rr_start=3)
+        out[name + "_s"] = round(time.perf_counter() - w0, 4)
+        out[name + "_mean"] = round(float(img.mean()), 6)
+    print(json.dumps(out), flush=True)
 
 
 def job_list_64():
@@ -359,6 +388,9 @@ def main():
         return
     if args.profile_frame:
         profile_frame()
+        return
+    if args.profile_frame_instanced:
+        profile_frame_instanced()
         return
     if args.jobs_one_call:
         jobs_one_call()
