@@ -159,6 +159,27 @@ def test_instanced_identity_object_of_a_shared_mesh_renders_as_flattened():
     assert sa["extension_rays"] == sb["extension_rays"] and sa["shadow_rays"] == sb["shadow_rays"]
 
 
+def test_instanced_cutout_layers_shadow_walk():
+    """Alpha cut-out layers as objects of one shared quad mesh (translated copies): the masked
+    shadow walk steps through each layer's instance, and the render equals the flattened scene's
+    (translations round differently in object space, so the comparison is to 1e-4)."""
+    scn, cam_of = scenes.masked_layers(4)
+    layer = quad((-200, 0, 200), (200, 0, 200), (200, 0, -200), (-200, 0, -200))
+    shared = scn.add_mesh(layer)
+    for o in scn.objects[1:5]:
+        y = float(scn.meshes[o.mesh].positions[0][1])
+        o.mesh = shared
+        o.model = np.eye(4, dtype=F32)
+        o.model[1, 3] = y
+    w, h = 64, 48
+    log_case("instanced vs flattened, 4 cut-out layers of one shared quad, 64x48 8spp depth 2")
+    (a, sa), (b, sb) = both(scn, cam_of(w, h), w, h, 8, 2)
+    assert a.mean() > 0.01 and sa["shadow_rays"] > 0 and sb["shadow_rays"] > 0
+    close = np.abs(a - b) <= 1e-4 * np.abs(a) + 1e-5
+    assert close.mean() >= 0.99, close.mean()
+    assert abs(float(a.mean()) - float(b.mean())) <= 1e-3 * float(a.mean())
+
+
 def test_instanced_scene_limits():
     scn, cam_of = shared_mesh_scene(n=4)
     pt = PathTracer(scn, instancing=True)
