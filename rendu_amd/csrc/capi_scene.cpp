@@ -719,13 +719,13 @@ int commitInstanced(rdu_scene * s) {
 			}
 		};
 		const rdu_scene::MeshRec & mr = s->meshes[ob.mesh];
-		if(mr.ni && mr.nv <= vertexBudget) {
+		if(mr.nv <= vertexBudget) {
 			vertexBudget -= mr.nv;
-			// Every vertex an index names (the triangles' hull); unreferenced vertices add nothing.
+			// Every vertex of the mesh, each once: a box that holds the triangles' hull (an
+			// unreferenced vertex can only widen it).
 			const float * P = s->positions.data() + 3 * size_t(mr.vtx);
-			const uint32_t * ix = s->indices.data() + mr.idx;
-			for(size_t i = 0; i < mr.ni; ++i) {
-				const double p[3] = {P[3 * size_t(ix[i])], P[3 * size_t(ix[i]) + 1], P[3 * size_t(ix[i]) + 2]};
+			for(size_t v = 0; v < mr.nv; ++v) {
+				const double p[3] = {P[3 * v], P[3 * v + 1], P[3 * v + 2]};
 				extend(p);
 			}
 		} else {
