@@ -180,6 +180,39 @@ def test_instanced_cutout_layers_shadow_walk():
     assert abs(float(a.mean()) - float(b.mean())) <= 1e-3 * float(a.mean())
 
 
+def test_instanced_scene_autotune_graph_replay_and_cache():
+    """The rest of the render API on an instanced scene: tuned launch parameters and graph-replayed
+    passes change no bit of the image; a cached render (radiance cache from bounce 2) stays close
+    to the plain one."""
+    from rendu_amd.rcache import RadianceCache
+    scn, cam_of = shared_mesh_scene(seed=11, n=24)
+    w, h = 64, 48
+    cam = cam_of(w, h)
+    log_case("instanced scene: autotune, graph replay (4 passes), cached render, 64x48 16spp depth 4")
+    pt = PathTracer(scn, instancing=True)
+    kw = dict(seed=6, estimator=S.ESTIMATOR_EXTENDED, rr_start=2, max_paths=w * h * 4, passes_in_flight=1)
+    ref = np.zeros((h, w, 3), F32)
+    pt.render(cam, 16, 4, ref, **kw)
+    assert ref.mean() > 0.01
+    rep = pt.autotune(cam, 16, 4, w, h, seed=6, estimator=S.ESTIMATOR_EXTENDED, rr_start=2, max_paths=w * h * 4)
+    assert rep["candidates"] > 0
+    tuned = np.zeros_like(ref)
+    pt.render(cam, 16, 4, tuned, **kw)
+    np.testing.assert_array_equal(ref, tuned)
+    pt.set_graph(True)
+    for _ in range(2):
+        rep_img = np.zeros_like(ref)
+        st = pt.render(cam, 16, 4, rep_img, **kw)
+        np.testing.assert_array_equal(ref, rep_img)
+    assert st["graph_passes"] > 0
+    pt.set_graph(False)
+    cached = np.zeros_like(ref)
+    cst = pt.render_cached(RadianceCache(*S.scene_bounds(scn), seed=2), cam, 16, 4, cached, seed=6,
+                           estimator=S.ESTIMATOR_EXTENDED, rr_start=2)
+    assert np.isfinite(cached).all() and cst["cache_queries"] >= 0
+    assert abs(float(cached.mean()) - float(ref.mean())) <= 0.1 * float(ref.mean())
+
+
 def test_instanced_scene_limits():
     scn, cam_of = shared_mesh_scene(n=4)
     pt = PathTracer(scn, instancing=True)
