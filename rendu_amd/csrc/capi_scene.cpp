@@ -166,6 +166,9 @@ struct rdu_scene {
 	// the tree the path tracer traverses.
 	bool instancing = false;
 	rdu::DevBuffer dInstWide, dInst;
+	// Host copies of what dInstWide and dInst hold: batch renders rebase them into their arena.
+	std::vector<float> instWideHost;
+	std::vector<float4> instTableHost;
 	rdu::DevBVH ibvh;
 	const rdu::DevBVH & traceBvh() const { return instancing ? ibvh : rc->dev; }
 	const rdu::RenderPaths * lastPaths = nullptr;     // path set of the last render (sets live until exit)
@@ -643,6 +646,19 @@ int rdu_scene_set_background(rdu_scene * s, const rdu_background_desc * d) {
 }
 
 namespace {
+// A 4-wide child code moved into a batch arena whose job starts at the given node, triangle and
+// instance offsets: nodes and instances shift by their offsets, a triangle leaf ~(first << 1 |
+// count - 1) by twice the triangle offset; empty slots stay empty.
+int32_t rebaseCode(int32_t c, uint32_t nodeOffset, uint32_t triOffset, uint32_t instOffset) {
+	if(c == kWideEmpty)
+		return c;
+	if(c >= 0)
+		return c + int32_t(nodeOffset);
+	if(c < kInstLimit)
+		return c + int32_t(instOffset);
+	return c - 2 * int32_t(triOffset);
+}
+
 bool isIdentity(const float * M) {
 	for(int k = 0; k < 16; ++k)
 		if(M[k] != ((k % 5 == 0) ? 1.0f : 0.0f))
@@ -775,6 +791,8 @@ int commitInstanced(rdu_scene * s) {
 	const int32_t topCode = buildInstanceTop(boxes, leafCodes, wide, wideCount, topDepth, top);
 	if((st = uploadVec(s->dInstWide, wide)) || (st = uploadVec(s->dInst, table)))
 		return st;
+	s->instWideHost = std::move(wide);
+	s->instTableHost = std::move(table);
 	DevBVH & b = s->ibvh;
 	b = s->rc->dev;
 	b.wide = static_cast<const float4 *>(s->dInstWide.ptr);
@@ -1319,8 +1337,6 @@ int rdu_pathtracer_render_batch(const rdu_batch_job * jobs, size_t n, uint32_t m
 			return fail(RDU_ERR_UNSUPPORTED, "batch job image larger than 2^26 pixels");
 		if(device >= 0 && b.scene->rc->device != device)
 			return fail(RDU_ERR_INVALID, "batch jobs on different devices");
-		if(b.scene->instancing)
-			return fail(RDU_ERR_UNSUPPORTED, "instanced scenes render alone, not in batches");
 		device = b.scene->rc->device;
 	}
 	int st = setDevice(device);
@@ -1333,13 +1349,13 @@ int rdu_pathtracer_render_batch(const rdu_batch_job * jobs, size_t n, uint32_t m
 	// Per job: sample count (power of two), frame, options; the tree's node and triangle counts.
 	struct JobInfo {
 		size_t spp, pixels, next = 0, imgOffset = 0;
-		uint32_t nodes = 0, tris = 0, nodeOffset = 0, triOffset = 0;
+		uint32_t nodes = 0, tris = 0, nodeOffset = 0, triOffset = 0, insts = 0, instOffset = 0;
 		int32_t code = 0;
 		float top[6];
 	};
 	std::vector<JobInfo> info(n);
 	std::vector<PtJob> table(n);
-	size_t nodesTotal = 0, trisTotal = 0, imgFloats = 0;
+	size_t nodesTotal = 0, trisTotal = 0, instsTotal = 0, imgFloats = 0;
 	uint32_t maxDepth = 0, wideDepth = 0;
 	bool shadows = false, masked = false;
 	for(size_t j = 0; j < n; ++j) {
@@ -1348,21 +1364,26 @@ int rdu_pathtracer_render_batch(const rdu_batch_job * jobs, size_t n, uint32_t m
 		JobInfo & I = info[j];
 		I.spp = size_t(std::pow(2.0, std::round(std::log2(float(b.samples)))));
 		I.pixels = size_t(b.width) * b.height;
-		I.nodes = rc->gpuBuilt ? rc->gpu.wideCount : uint32_t(rc->layout.wide.size() / 32);
+		const bool inst = b.scene->instancing;
+		const DevBVH & tb = b.scene->traceBvh();
+		I.nodes = inst ? uint32_t(b.scene->instWideHost.size() / 32)
+					   : rc->gpuBuilt ? rc->gpu.wideCount : uint32_t(rc->layout.wide.size() / 32);
 		I.tris = rc->gpuBuilt ? rc->gpu.triangles : uint32_t(rc->layout.tris.size() / 12);
+		I.insts = inst ? uint32_t(b.scene->instTableHost.size() / 4) : 0u;
 		I.nodeOffset = uint32_t(nodesTotal);
 		I.triOffset = uint32_t(trisTotal);
+		I.instOffset = uint32_t(instsTotal);
 		I.imgOffset = imgFloats;
 		nodesTotal += I.nodes;
 		trisTotal += I.tris;
+		instsTotal += I.insts;
 		imgFloats += I.pixels * b.components;
-		const int32_t c = rc->dev.wideTopCode;
-		I.code = c >= 0 ? c + int32_t(I.nodeOffset) : c - 2 * int32_t(I.triOffset);
+		I.code = rebaseCode(tb.wideTopCode, I.nodeOffset, I.triOffset, I.instOffset);
 		for(int a = 0; a < 3; ++a) {
-			I.top[a] = rc->dev.wideTopMin[a];
-			I.top[3 + a] = rc->dev.wideTopMax[a];
+			I.top[a] = tb.wideTopMin[a];
+			I.top[3 + a] = tb.wideTopMax[a];
 		}
-		wideDepth = std::max(wideDepth, rc->dev.wideDepth);
+		wideDepth = std::max(wideDepth, tb.wideDepth);
 		const rdu_render_opts o = b.opts;
 		PtJob & J = table[j];
 		J.S = b.scene->dev;
@@ -1375,8 +1396,49 @@ int rdu_pathtracer_render_batch(const rdu_batch_job * jobs, size_t n, uint32_t m
 		shadows = shadows || J.S.nLights > 0 || (o.estimator == RDU_ESTIMATOR_EXTENDED && J.S.pArea > 0.0f);
 		masked = masked || J.S.anyMasked != 0;
 	}
-	if(nodesTotal >= size_t(0x7fffffff) || trisTotal >= (size_t(1) << 30))
+	if(nodesTotal >= size_t(0x7fffffff) || trisTotal >= (size_t(1) << (instsTotal ? 29 : 30)) || instsTotal > size_t(kInstMax))
 		return fail(RDU_ERR_UNSUPPORTED, "batch trees too large for one arena");
+	// Instanced jobs: their nodes (per-mesh trees and top level) and instance records rebased here
+	// on the host, every code checked against the arena before anything is uploaded.
+	std::vector<std::vector<float>> instNodes(n);
+	std::vector<float4> instTable(4 * instsTotal);
+	const auto codeInArena = [&](int32_t c) {
+		if(c >= 0)
+			return c == kWideEmpty || size_t(c) < nodesTotal;
+		if(c < kInstLimit)
+			return size_t(c - kInstCode0) < instsTotal;
+		const uint32_t code = uint32_t(~c);
+		return size_t(code >> 1) + (code & 1u) < trisTotal;
+	};
+	for(size_t j = 0; j < n; ++j) {
+		const JobInfo & I = info[j];
+		if(!jobs[j].scene->instancing)
+			continue;
+		if(!codeInArena(I.code))
+			return fail(RDU_ERR_STATE, "batch arena: an instanced job's root code out of range");
+		std::vector<float> & W = instNodes[j];
+		W = jobs[j].scene->instWideHost;
+		for(size_t k = 0; k < size_t(I.nodes); ++k)
+			for(int slot = 0; slot < 4; ++slot) {
+				float & f = W[32 * k + 24 + size_t(slot)];
+				const int32_t c = rebaseCode(__builtin_bit_cast(int32_t, f), I.nodeOffset, I.triOffset, I.instOffset);
+				if(!codeInArena(c))
+					return fail(RDU_ERR_STATE, "batch arena: an instanced job's node code out of range");
+				f = __builtin_bit_cast(float, c);
+			}
+		const std::vector<float4> & T = jobs[j].scene->instTableHost;
+		for(size_t i = 0; i < size_t(I.insts); ++i) {
+			float4 * r = &instTable[4 * (size_t(I.instOffset) + i)];
+			r[0] = T[4 * i];
+			r[1] = T[4 * i + 1];
+			r[2] = T[4 * i + 2];
+			r[3] = T[4 * i + 3];
+			const int32_t root = rebaseCode(__builtin_bit_cast(int32_t, r[3].x), I.nodeOffset, I.triOffset, I.instOffset);
+			if(root < kInstLimit || !codeInArena(root))
+				return fail(RDU_ERR_STATE, "batch arena: an instance's tree root out of range");
+			r[3].x = __builtin_bit_cast(float, root);
+		}
+	}
 
 	hipStream_t stream = nullptr;
 	if((st = hipCheck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate")))
@@ -1385,13 +1447,13 @@ int rdu_pathtracer_render_batch(const rdu_batch_job * jobs, size_t n, uint32_t m
 	struct Local {
 		hipStream_t s;
 		std::vector<hipStream_t> more; // the other streams passes run on
-		DevBuffer arenaNodes, arenaTris, roots, jobsDev, chunks, images;
+		DevBuffer arenaNodes, arenaTris, arenaInst, roots, jobsDev, chunks, images;
 		std::vector<hipEvent_t> events;
 		~Local() {
 			for(hipStream_t m : more)
 				(void)hipStreamSynchronize(m);
 			(void)hipStreamSynchronize(s);
-			for(DevBuffer * b : {&arenaNodes, &arenaTris, &roots, &jobsDev, &chunks, &images})
+			for(DevBuffer * b : {&arenaNodes, &arenaTris, &arenaInst, &roots, &jobsDev, &chunks, &images})
 				b->releaseOn(s);
 			for(hipEvent_t e : events)
 				(void)hipEventDestroy(e);
@@ -1404,7 +1466,11 @@ int rdu_pathtracer_render_batch(const rdu_batch_job * jobs, size_t n, uint32_t m
 	L.s = stream;
 	if((st = L.arenaNodes.reserveOn(std::max<size_t>(nodesTotal, 1) * 128, stream)) ||
 		(st = L.arenaTris.reserveOn(std::max<size_t>(trisTotal, 1) * 48, stream)) || (st = L.roots.reserveOn(n * 32, stream)) ||
-		(st = L.jobsDev.reserveOn(n * sizeof(PtJob), stream)) || (st = L.images.reserveOn(imgFloats * 4, stream)))
+		(st = L.jobsDev.reserveOn(n * sizeof(PtJob), stream)) || (st = L.images.reserveOn(imgFloats * 4, stream)) ||
+		(instsTotal && (st = L.arenaInst.reserveOn(instsTotal * 64, stream))))
+		return st;
+	if(instsTotal && (st = hipCheck(hipMemcpyAsync(L.arenaInst.ptr, instTable.data(), instsTotal * 64, hipMemcpyHostToDevice, stream),
+						  "arena instances")))
 		return st;
 	float4 * arenaNodes = static_cast<float4 *>(L.arenaNodes.ptr);
 	float4 * arenaTris = static_cast<float4 *>(L.arenaTris.ptr);
@@ -1413,9 +1479,13 @@ int rdu_pathtracer_render_batch(const rdu_batch_job * jobs, size_t n, uint32_t m
 	for(size_t j = 0; j < n; ++j) {
 		const JobInfo & I = info[j];
 		const rdu_raycaster * rc = jobs[j].scene->rc;
-		if((st = hipCheck(hipError_t(ptRebaseWide(arenaNodes + 8 * size_t(I.nodeOffset), rc->dev.wide, I.nodes,
-								 int32_t(I.nodeOffset), int32_t(I.triOffset), stream)),
-				"arena nodes")) ||
+		if((jobs[j].scene->instancing
+				   ? (st = hipCheck(hipMemcpyAsync(arenaNodes + 8 * size_t(I.nodeOffset), instNodes[j].data(), size_t(I.nodes) * 128,
+										hipMemcpyHostToDevice, stream),
+						  "arena nodes"))
+				   : (st = hipCheck(hipError_t(ptRebaseWide(arenaNodes + 8 * size_t(I.nodeOffset), rc->dev.wide, I.nodes,
+										int32_t(I.nodeOffset), int32_t(I.triOffset), stream)),
+						  "arena nodes"))) ||
 			(I.tris && (st = hipCheck(hipMemcpyAsync(arenaTris + 3 * size_t(I.triOffset), rc->dev.tris, size_t(I.tris) * 48,
 								 hipMemcpyDeviceToDevice, stream),
 							"arena triangles"))))
@@ -1434,6 +1504,7 @@ int rdu_pathtracer_render_batch(const rdu_batch_job * jobs, size_t n, uint32_t m
 	DevBVH arena;
 	arena.wide = arenaNodes;
 	arena.tris = arenaTris;
+	arena.inst = instsTotal ? static_cast<const float4 *>(L.arenaInst.ptr) : nullptr;
 	arena.wideDepth = wideDepth;
 
 	// Schedule, made up front: longest jobs first (pixels x samples x depth), each pass taking as

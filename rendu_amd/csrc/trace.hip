@@ -457,24 +457,35 @@ int launchImpl(const DevBVH & bvh, const float4 * rays, size_t n, rdu_hit * hits
 			opt.rayJob, opt.jobRoots, stats);                                                                   \
 	}
 #define RDU_PERSIST_B(S, ST, ALT, NFA, B) RDU_PERSIST_X(S, ST, ALT, NFA, B, false)
-#define RDU_PERSIST_I(S, ALT, NFA) RDU_PERSIST_X(S, false, ALT, NFA, false, true)
+#define RDU_PERSIST_I(S, ALT, NFA, B) RDU_PERSIST_X(S, false, ALT, NFA, B, true)
 #define RDU_PERSIST(S, ST, ALT, NFA) RDU_PERSIST_B(S, ST, ALT, NFA, false)
 		// Kernel variant: the near/far plane fetch (fewer instructions per node, more registers:
 		// 5 waves per SIMD instead of 6 for closest hit and of 7 for any hit; pays on incoherent rays), or
 		// for any hit on coherent rays the unsorted visiting order.
 		const bool alt = ANY ? opt.unorderedAny : opt.nearFar;
 		const bool nfa = ANY && !opt.unorderedAny && opt.nearFar;
-		if(bvh.inst) {
+		if(bvh.inst && opt.jobRoots) {
+			// Batch passes with instanced jobs in the arena: as the batch variants below.
+			if(stats || (ANY && !nfa))
+				return int(hipErrorInvalidValue);
+			if(ANY) {
+				if(spill) RDU_PERSIST_I(true, false, true, true) else RDU_PERSIST_I(false, false, true, true)
+			} else if(alt) {
+				if(spill) RDU_PERSIST_I(true, true, false, true) else RDU_PERSIST_I(false, true, false, true)
+			} else {
+				if(spill) RDU_PERSIST_I(true, false, false, true) else RDU_PERSIST_I(false, false, false, true)
+			}
+		} else if(bvh.inst) {
 			// Instanced scenes (the path tracer's passes): closest hit with either fetch, any hit
-			// with the near/far fetch or in node order; no statistics, no batches.
-			if(stats || opt.jobRoots)
+			// with the near/far fetch or in node order; no statistics.
+			if(stats)
 				return int(hipErrorInvalidValue);
 			if(ANY && nfa) {
-				if(spill) RDU_PERSIST_I(true, false, true) else RDU_PERSIST_I(false, false, true)
+				if(spill) RDU_PERSIST_I(true, false, true, false) else RDU_PERSIST_I(false, false, true, false)
 			} else if(alt) {
-				if(spill) RDU_PERSIST_I(true, true, false) else RDU_PERSIST_I(false, true, false)
+				if(spill) RDU_PERSIST_I(true, true, false, false) else RDU_PERSIST_I(false, true, false, false)
 			} else {
-				if(spill) RDU_PERSIST_I(true, false, false) else RDU_PERSIST_I(false, false, false)
+				if(spill) RDU_PERSIST_I(true, false, false, false) else RDU_PERSIST_I(false, false, false, false)
 			}
 		} else if(opt.jobRoots) {
 			// Batch passes of the path tracer: closest hit with either fetch, any hit with the

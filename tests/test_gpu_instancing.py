@@ -213,15 +213,36 @@ def test_instanced_scene_autotune_graph_replay_and_cache():
     assert abs(float(cached.mean()) - float(ref.mean())) <= 0.1 * float(ref.mean())
 
 
+@pytest.mark.parametrize("max_paths", [0, 2500], ids=["default-passes", "small-passes"])
+def test_instanced_jobs_in_one_batch_call(max_paths):
+    """Instanced scenes in rdu_pathtracer_render_batch beside flattened ones (instance records,
+    per-mesh trees and top levels rebased into the call's arena): every image equals the job
+    rendered alone, bit for bit."""
+    a, cam_a = shared_mesh_scene(seed=12, n=16)
+    b, cam_b = shared_mesh_scene(seed=13, n=10, emissive=True, light=False)
+    c, cam_c, w_c, h_c, spp_c, depth_c = scenes.random_job(7)
+    js = [J.Job(a, cam_a(40, 30), 40, 30, 8, 4, seed=21),
+          J.Job(c, cam_c(32, 32), 32, 32, 4, depth_c, seed=22),
+          J.Job(b, cam_b(36, 24), 36, 24, 8, 4, seed=23, estimator=S.ESTIMATOR_EXTENDED, rr_start=2)]
+    pts = [PathTracer(a, instancing=True), PathTracer(c), PathTracer(b, instancing=True)]
+    log_case(f"batch of 2 instanced + 1 flattened jobs, max_paths {max_paths}, vs alone")
+    alone = []
+    for j, pt in zip(js, pts):
+        img = np.zeros((j.height, j.width, 3), F32)
+        pt.render(j.camera, j.samples, j.depth, img, seed=j.seed, estimator=j.estimator, rr_start=j.rr_start)
+        alone.append(img)
+    images, secs, st = J.render_batch(list(zip(pts, js)), max_paths=max_paths)
+    for k, (x, y) in enumerate(zip(alone, images)):
+        assert x.mean() > 0.0, k
+        assert np.array_equal(x, y), (k, float(np.abs(x - y).max()))
+    assert st["passes"] >= (1 if max_paths == 0 else 3)
+
+
 def test_instanced_scene_limits():
     scn, cam_of = shared_mesh_scene(n=4)
     pt = PathTracer(scn, instancing=True)
     with pytest.raises(RuntimeError):
         pt.raycaster()
-    job = J.Job(scn, cam_of(16, 16), 16, 16, 4, 2)
-    from rendu_amd._lib import RduError, RDU_ERR_UNSUPPORTED
-    with pytest.raises(RduError) as e:
-        J.render_batch([(pt, job)])
-    assert e.value.code == RDU_ERR_UNSUPPORTED
+    from rendu_amd._lib import RduError
     with pytest.raises(RduError):
         PathTracer(scn, gpu_build=True, instancing=True)
