@@ -275,7 +275,7 @@ int rdu_scene_set_gpu_build(rdu_scene* scene, int enable);
  * the last bits (the transform's rounding), so renders match the flattened scene's statistically,
  * not bit for bit; a scene without shared meshes (or whose shared meshes' objects all have identity
  * models) renders as the flattened scene does. Such a scene has no raycaster handle
- * (rdu_scene_raycaster returns NULL), renders alone (not in rdu_pathtracer_render_batch) and builds
+ * (rdu_scene_raycaster returns NULL), renders alone or in rdu_pathtracer_render_batch, and builds
  * on the host; the models of objects on shared meshes must be invertible. */
 int rdu_scene_set_instancing(rdu_scene* scene, int enable);
 
