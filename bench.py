@@ -708,6 +708,15 @@ def main():
     J.run_batch(job_list, workers=8, order="lpt")  # warm-up of the GPU build's own temporaries
     res_j, wall_j = J.run_batch(job_list, workers=6, order="lpt", build_workers=2)
     extra["jobs64_gpu_build"] = dict(J.summary(res_j, wall_j), workers=6, build_workers=2, order="longest first")
+    # The same jobs as instanced scenes (each job's spheres share one tree; host builds).
+    for j in job_list:
+        j.gpu_build = False
+        j.instancing = True
+    J.run_batch(job_list, workers=8, order="lpt")  # warm-up
+    res_i, wall_i = J.run_batch(job_list, workers=6, order="lpt", build_workers=2)
+    extra["jobs64_instanced"] = dict(J.summary(res_i, wall_i), workers=6, build_workers=2, order="longest first")
+    for j in job_list:
+        j.instancing = False
     del res_j
 
     # Algorithmic (compulsory) bytes per traversal launch, the per-ray figure of the workload's

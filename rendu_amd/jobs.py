@@ -34,6 +34,7 @@ class Job:
     estimator: int = S.ESTIMATOR_REFERENCE
     rr_start: int = 3
     gpu_build: bool = False  # build the tree on the GPU instead of Rendu's host build
+    instancing: bool = False  # an instanced scene (rdu_scene_set_instancing): shared meshes built once
     # Passes of this job's render kept in flight at once (0 = the library's default, 3). One by
     # default: the batch already runs several jobs at once, and more streams per job cost the batch
     # throughput (64 jobs: 53-55 jobs/s with 3, 56-63 with 1) for a lower p99 latency.
@@ -56,7 +57,7 @@ class JobResult:
 
 def _build(job: Job):
     t0 = time.perf_counter()
-    pt = PathTracer(job.scene, gpu_build=job.gpu_build)
+    pt = PathTracer(job.scene, gpu_build=job.gpu_build, instancing=job.instancing)
     return pt, t0, time.perf_counter()
 
 
