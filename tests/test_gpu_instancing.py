@@ -221,11 +221,20 @@ def test_instanced_jobs_in_one_batch_call(max_paths):
     a, cam_a = shared_mesh_scene(seed=12, n=16)
     b, cam_b = shared_mesh_scene(seed=13, n=10, emissive=True, light=False)
     c, cam_c, w_c, h_c, spp_c, depth_c = scenes.random_job(7)
+    # Cut-out layers sharing one quad: the masked shadow walk runs on the arena with instances.
+    d, cam_d = scenes.masked_layers(3)
+    shared = d.add_mesh(quad((-200, 0, 200), (200, 0, 200), (200, 0, -200), (-200, 0, -200)))
+    for o in d.objects[1:4]:
+        y = float(d.meshes[o.mesh].positions[0][1])
+        o.mesh = shared
+        o.model = np.eye(4, dtype=F32)
+        o.model[1, 3] = y
     js = [J.Job(a, cam_a(40, 30), 40, 30, 8, 4, seed=21),
           J.Job(c, cam_c(32, 32), 32, 32, 4, depth_c, seed=22),
-          J.Job(b, cam_b(36, 24), 36, 24, 8, 4, seed=23, estimator=S.ESTIMATOR_EXTENDED, rr_start=2)]
-    pts = [PathTracer(a, instancing=True), PathTracer(c), PathTracer(b, instancing=True)]
-    log_case(f"batch of 2 instanced + 1 flattened jobs, max_paths {max_paths}, vs alone")
+          J.Job(b, cam_b(36, 24), 36, 24, 8, 4, seed=23, estimator=S.ESTIMATOR_EXTENDED, rr_start=2),
+          J.Job(d, cam_d(24, 20), 24, 20, 4, 2, seed=24)]
+    pts = [PathTracer(a, instancing=True), PathTracer(c), PathTracer(b, instancing=True), PathTracer(d, instancing=True)]
+    log_case(f"batch of 3 instanced (one with cut-outs) + 1 flattened jobs, max_paths {max_paths}, vs alone")
     alone = []
     for j, pt in zip(js, pts):
         img = np.zeros((j.height, j.width, 3), F32)
